@@ -1,0 +1,98 @@
+/*
+ * emp_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ restatement of the
+ * reference's CPU search algorithms (quicky2000/edge_matching_puzzle).  Only
+ * the test suite, the smoke check and the CPU-baseline leg of the benchmark
+ * may load it, and only as the checker / the baseline.  The product library
+ * (libemp_hip.so) never links, loads or calls anything in this directory.
+ *
+ * Puzzles are passed as W, H and colours[P][4] (N, E, S, W) for piece ids
+ * 1..P in file order, colour 0 being the frame, exactly as in the reference's
+ * text format (include/emp_pieces_parser.h:53-104).
+ *
+ * Boards are returned row-major, one uint16 per cell:
+ *   ((piece_id - 1) << 2) | orientation, or 0xFFFF for an empty cell.
+ */
+#ifndef EMP_ORACLE_H
+#define EMP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes shared by every entry point. */
+#define ORACLE_OK            0
+#define ORACLE_BAD_ARGUMENT  1
+#define ORACLE_BAD_PUZZLE    2
+#define ORACLE_LIMIT_REACHED 3
+
+/* Position orders (include/emp_strategy_generator_factory.h) */
+#define ORACLE_ORDER_SPIRAL 0
+#define ORACLE_ORDER_BASIC  1
+
+/*
+ * Exhaustive depth-first enumeration, restating emp_strategy::specific_run
+ * (include/emp_strategy.h:332-415) with HANDLE_IDENTICAL_PIECES semantics
+ * for appearance-identical pieces (include/emp_position_strategy.h:189-202).
+ *
+ *   node_limit      stop after that many placements (0 = unlimited); status
+ *                   ORACLE_LIMIT_REACHED then means the counts are partial.
+ *   boards          [max_solutions][P] row-major boards of the first
+ *                   max_solutions solutions found (may be NULL).
+ *   solution_nodes  node index of each stored solution (may be NULL).
+ *   depth_nodes     [P] placements per strategy index (may be NULL).
+ *   depth_dead      [P] placements at index k after which index k+1 had no
+ *                   transition at all (may be NULL).
+ *   trace / trace_capacity  "New max : k" + situation lines as printed by the
+ *                   reference's MAX_DISPLAY build, '\n' separated (may be NULL).
+ */
+int oracle_enumerate(uint32_t width, uint32_t height, const uint32_t *colours,
+                     int order, uint64_t node_limit,
+                     uint64_t max_solutions, uint16_t *boards,
+                     uint64_t *solution_nodes,
+                     uint64_t *n_solutions, uint64_t *n_nodes,
+                     uint64_t *depth_nodes, uint64_t *depth_dead,
+                     char *trace, uint64_t trace_capacity);
+
+/*
+ * results.bin v1 image of an enumeration (include/emp_situation_binary_dumper.h
+ * :72-126): header, strategy positions, one record per solution, total nodes.
+ * Writes at most capacity bytes into out and returns the full size in *size.
+ */
+int oracle_results_bin(uint32_t width, uint32_t height, const uint32_t *colours,
+                       int order, uint8_t *out, uint64_t capacity,
+                       uint64_t *size);
+
+/* Strategy order: positions[i] = (x, y) of the i-th visited position. */
+int oracle_order(uint32_t width, uint32_t height, int order, uint32_t *xy);
+
+/*
+ * glutton_max restatement (include/CUDA_glutton_max.h:742-1637,
+ * include/CUDA_glutton_max_stack.h:38-630).
+ *
+ *   prefix_moves    n_prefix moves (position, piece_index, orientation),
+ *                   played through the same move application as the search
+ *                   itself before the search starts; the search never pops
+ *                   below them.
+ *   push_limit      stop after that many pushes (0 = unlimited).
+ *   board           [P] final board: the full solution when *solved, else the
+ *                   deepest situation seen (the stack's max played info).
+ *   max_depth       deepest level reached (pieces on board, prefix included).
+ *   n_pushes        number of pushes (nodes) performed by the search.
+ *   solved          1 when the stack became full.
+ * Returns ORACLE_LIMIT_REACHED when the push limit stopped the search.
+ */
+int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
+                       uint32_t n_prefix, const uint32_t *prefix_moves,
+                       uint64_t push_limit, uint16_t *board,
+                       uint32_t *max_depth, uint64_t *n_pushes,
+                       uint32_t *solved);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,121 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker, never the product.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+OK, BAD_ARGUMENT, BAD_PUZZLE, LIMIT_REACHED = 0, 1, 2, 3
+SPIRAL, BASIC = 0, 1
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        u32, u64, i32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        p = ctypes.c_void_p
+        _lib.oracle_enumerate.argtypes = [u32, u32, p, i32, u64, u64, p, p, p, p, p, p, p, u64]
+        _lib.oracle_enumerate.restype = i32
+        _lib.oracle_results_bin.argtypes = [u32, u32, p, i32, p, u64, p]
+        _lib.oracle_results_bin.restype = i32
+        _lib.oracle_order.argtypes = [u32, u32, i32, p]
+        _lib.oracle_order.restype = i32
+        _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
+        _lib.oracle_glutton_max.restype = i32
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def parse_puzzle(path):
+    """Reference text format: 'Width,Height' / 'W,H' / 'I,N,E,S,O' / pieces."""
+    lines = [l.strip() for l in open(path) if l.strip()]
+    w, h = (int(v) for v in lines[1].split(","))
+    rows = [[int(v) for v in l.split(",")] for l in lines[3:]]
+    rows.sort(key=lambda r: r[0])
+    colours = np.array([r[1:5] for r in rows], dtype=np.uint32)
+    assert colours.shape == (w * h, 4)
+    return w, h, colours
+
+
+def enumerate_(w, h, colours, order=SPIRAL, node_limit=0, max_solutions=1 << 16, trace=False):
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    P = w * h
+    boards = np.zeros((max_solutions, P), dtype=np.uint16)
+    snodes = np.zeros(max_solutions, dtype=np.uint64)
+    nsol = np.zeros(1, dtype=np.uint64)
+    nnodes = np.zeros(1, dtype=np.uint64)
+    dn = np.zeros(P, dtype=np.uint64)
+    dd = np.zeros(P, dtype=np.uint64)
+    cap = 1 << 20 if trace else 0
+    buf = ctypes.create_string_buffer(cap) if trace else None
+    st = lib().oracle_enumerate(w, h, _ptr(colours), order, node_limit, max_solutions,
+                                _ptr(boards), _ptr(snodes), _ptr(nsol), _ptr(nnodes),
+                                _ptr(dn), _ptr(dd), buf, cap)
+    if st not in (OK, LIMIT_REACHED):
+        raise ValueError("oracle_enumerate status %d" % st)
+    n = int(nsol[0])
+    kept = min(n, max_solutions)
+    return {
+        "status": st, "n_solutions": n, "n_nodes": int(nnodes[0]),
+        "boards": boards[:kept].copy(), "solution_nodes": snodes[:kept].copy(),
+        "depth_nodes": dn, "depth_dead": dd,
+        "trace": buf.value.decode() if trace else None,
+    }
+
+
+def results_bin(w, h, colours, order=SPIRAL):
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    size = np.zeros(1, dtype=np.uint64)
+    st = lib().oracle_results_bin(w, h, _ptr(colours), order, None, 0, _ptr(size))
+    if st != OK:
+        raise ValueError("oracle_results_bin status %d" % st)
+    out = np.zeros(int(size[0]), dtype=np.uint8)
+    lib().oracle_results_bin(w, h, _ptr(colours), order, _ptr(out), out.size, _ptr(size))
+    return out.tobytes()
+
+
+def order(w, h, kind=SPIRAL):
+    xy = np.zeros(2 * w * h, dtype=np.uint32)
+    lib().oracle_order(w, h, kind, _ptr(xy))
+    return [(int(xy[2 * i]), int(xy[2 * i + 1])) for i in range(w * h)]
+
+
+def glutton_max(w, h, colours, prefix=(), push_limit=0):
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    moves = np.array(list(prefix), dtype=np.uint32).reshape(-1, 3) if len(prefix) else np.zeros((0, 3), np.uint32)
+    moves = np.ascontiguousarray(moves)
+    board = np.zeros(w * h, dtype=np.uint16)
+    md = np.zeros(1, dtype=np.uint32)
+    pushes = np.zeros(1, dtype=np.uint64)
+    solved = np.zeros(1, dtype=np.uint32)
+    st = lib().oracle_glutton_max(w, h, _ptr(colours), moves.shape[0], _ptr(moves), push_limit,
+                                  _ptr(board), _ptr(md), _ptr(pushes), _ptr(solved))
+    if st not in (OK, LIMIT_REACHED):
+        raise ValueError("oracle_glutton_max status %d" % st)
+    return {"status": st, "board": board, "max_depth": int(md[0]), "pushes": int(pushes[0]),
+            "solved": bool(solved[0])}
+
+
+def situation_string(board, P):
+    w = len(str(P)) + 1
+    out = []
+    for v in board:
+        v = int(v)
+        if v == 0xFFFF:
+            out.append("-" * w)
+        else:
+            out.append(str((v >> 2) + 1).rjust(w - 1) + "NESW"[v & 3])
+    return "".join(out)
