@@ -1,0 +1,208 @@
+"""edge_matching_puzzle_amd: MI355X (gfx950) search path of edge_matching_puzzle.
+
+Python binding of libemp_hip.so, the C ABI declared in include/emp_hip.h.
+The library is the product: every search runs in its HIP kernels.  There is
+no CPU fallback -- loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemp_hip.so")
+
+OK, BAD_ARGUMENT, BAD_PUZZLE, DEVICE_ERROR, OUT_OF_MEMORY, UNSUPPORTED = range(6)
+ORDER_SPIRAL, ORDER_BASIC = 0, 1
+_ORDERS = {"spiral": ORDER_SPIRAL, "basic": ORDER_BASIC}
+
+
+class EmpError(RuntimeError):
+    """Error status returned by libemp_hip.so."""
+
+    def __init__(self, status, message):
+        super().__init__("emp_hip error %d: %s" % (status, message))
+        self.status = status
+
+
+class Puzzle(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("colours", ctypes.c_void_p)]
+
+
+class EnumOptions(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("waves_per_block", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
+                ("step_budget", ctypes.c_uint64), ("target_items", ctypes.c_uint64),
+                ("node_limit", ctypes.c_uint64), ("max_solutions", ctypes.c_uint64),
+                ("use_graph", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class EnumResult(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_uint64), ("n_solutions", ctypes.c_uint64),
+                ("n_stored", ctypes.c_uint64), ("complete", ctypes.c_uint32),
+                ("status", ctypes.c_uint32), ("solve_ms", ctypes.c_double)]
+
+
+class EnumStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("dfs_kernel_ms", ctypes.c_double),
+                ("expand_kernel_ms", ctypes.c_double), ("dfs_launches", ctypes.c_uint64),
+                ("expand_launches", ctypes.c_uint64), ("items_processed", ctypes.c_uint64),
+                ("splits", ctypes.c_uint64), ("max_words", ctypes.c_uint32),
+                ("lds_tables", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("threads", ctypes.c_uint32)]
+
+
+_lib = None
+
+# Every symbol include/emp_hip.h declares, with (restype, argtypes).
+_P = ctypes.c_void_p
+_U32, _U64, _I32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+SIGNATURES = {
+    "emp_hip_last_error": (ctypes.c_char_p, []),
+    "emp_hip_version": (ctypes.c_char_p, []),
+    "emp_hip_create": (_I32, [_I32, _P]),
+    "emp_hip_destroy": (_I32, [_P]),
+    "emp_hip_device_info": (_I32, [_P, _P, _P, _P]),
+    "emp_hip_generate": (_I32, [_U64, _U32, _U32, _U32, _U32, _P, _P]),
+    "emp_hip_validate_puzzle": (_I32, [_P]),
+    "emp_hip_order": (_I32, [_U32, _U32, _U32, _P]),
+    "emp_hip_situation_string": (_I32, [_U32, _U32, _P, ctypes.c_char_p, _U64]),
+    "emp_hip_enum_default_options": (None, [_P]),
+    "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+}
+
+
+def lib():
+    """Loads libemp_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libemp_hip.so is not built; run `make -C edge_matching_puzzle_amd` "
+                               "(or __graft_entry__.build())")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def _check(status):
+    if status != OK:
+        raise EmpError(status, lib().emp_hip_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def parse_puzzle(path):
+    """Reads the reference's text format (include/emp_pieces_parser.h:53-104):
+    'Width,Height', 'W,H', 'I,N,E,S,O', then 'id,n,e,s,w' per piece."""
+    lines = [l.strip() for l in open(path) if l.strip()]
+    w, h = (int(v) for v in lines[1].split(","))
+    rows = sorted(([int(v) for v in l.split(",")] for l in lines[3:]), key=lambda r: r[0])
+    if [r[0] for r in rows] != list(range(1, w * h + 1)):
+        raise ValueError("%s: piece ids must be 1..%d" % (path, w * h))
+    return w, h, np.array([r[1:5] for r in rows], dtype=np.uint32)
+
+
+def generate(seed, width, height, border_colours, interior_colours):
+    """Seeded synthetic puzzle; returns (colours [P,4] uint32, planted board)."""
+    P = width * height
+    colours = np.zeros((P, 4), dtype=np.uint32)
+    planted = np.zeros(P, dtype=np.uint16)
+    _check(lib().emp_hip_generate(seed, width, height, border_colours, interior_colours,
+                                  _ptr(colours), _ptr(planted)))
+    return colours, planted
+
+
+def validate(width, height, colours):
+    """Raises EmpError(BAD_PUZZLE) when the reference would refuse the puzzle."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    _check(lib().emp_hip_validate_puzzle(ctypes.byref(pz)))
+
+
+def order(width, height, kind="spiral"):
+    xy = np.zeros(2 * width * height, dtype=np.uint32)
+    _check(lib().emp_hip_order(width, height, _ORDERS[kind], _ptr(xy)))
+    return [(int(xy[2 * i]), int(xy[2 * i + 1])) for i in range(width * height)]
+
+
+def situation_string(width, height, board):
+    board = np.ascontiguousarray(board, dtype=np.uint16)
+    cap = width * height * 5 + 1
+    buf = ctypes.create_string_buffer(cap)
+    _check(lib().emp_hip_situation_string(width, height, _ptr(board), buf, cap))
+    return buf.value.decode()
+
+
+class Context:
+    """One HIP stream on one device (emp_hip_context)."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().emp_hip_create(device, ctypes.byref(self._ctx)))
+
+    def close(self):
+        if self._ctx:
+            lib().emp_hip_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        cu, lds, clk = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().emp_hip_device_info(self._ctx, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(clk)))
+        return {"compute_units": cu.value, "lds_per_block": lds.value, "clock_khz": clk.value}
+
+    def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
+                  waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
+                  node_limit=0):
+        """Exhaustive enumeration of each (width, height, colours) puzzle.
+
+        Returns (results, stats): one dict per puzzle with n_nodes, n_solutions,
+        complete, solve_ms and boards ([n_stored, P] uint16 row-major)."""
+        n = len(puzzles)
+        keep = []
+        arr = (Puzzle * n)()
+        boards = []
+        board_ptrs = (ctypes.c_void_p * n)()
+        for i, (w, h, c) in enumerate(puzzles):
+            c = np.ascontiguousarray(c, dtype=np.uint32)
+            keep.append(c)
+            arr[i] = Puzzle(w, h, c.ctypes.data)
+            b = np.zeros((max_solutions, w * h), dtype=np.uint16)
+            boards.append(b)
+            board_ptrs[i] = b.ctypes.data if max_solutions else None
+        opt = EnumOptions()
+        lib().emp_hip_enum_default_options(ctypes.byref(opt))
+        opt.order = _ORDERS[order]
+        opt.waves_per_cu, opt.waves_per_block = waves_per_cu, waves_per_block
+        opt.steps_per_round, opt.step_budget = steps_per_round, step_budget
+        opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
+        res = (EnumResult * n)()
+        st = EnumStats()
+        _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
+                                       ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+        out = []
+        for i in range(n):
+            r = res[i]
+            out.append({"n_nodes": r.n_nodes, "n_solutions": r.n_solutions,
+                        "complete": bool(r.complete), "solve_ms": r.solve_ms,
+                        "boards": boards[i][:r.n_stored].copy()})
+        stats = {f: getattr(st, f) for f, _ in EnumStats._fields_}
+        return out, stats

@@ -1,0 +1,227 @@
+// emp_api.cpp -- extern "C" entry points of libemp_hip.so (declared in
+// include/emp_hip.h).  Exceptions never cross the ABI: each entry point maps
+// them to a status code and keeps the message for emp_hip_last_error().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/emp_hip.h"
+#include "emp_enumerate.h"
+#include "emp_model.h"
+
+struct emp_hip_context {
+    emp::DeviceInfo dev;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const std::invalid_argument &e) {
+        return fail(EMP_HIP_BAD_PUZZLE, e.what());
+    } catch (const emp::OutOfMemory &e) {
+        return fail(EMP_HIP_OUT_OF_MEMORY, e.what());
+    } catch (const emp::Unsupported &e) {
+        return fail(EMP_HIP_UNSUPPORTED, e.what());
+    } catch (const emp::DeviceError &e) {
+        return fail(EMP_HIP_DEVICE_ERROR, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(EMP_HIP_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(EMP_HIP_DEVICE_ERROR, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *emp_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char *emp_hip_version(void) { return "emp_hip 0.1 gfx950"; }
+
+int emp_hip_create(int device, emp_hip_context **context) {
+    return guarded([&] {
+        if (!context) return fail(EMP_HIP_BAD_ARGUMENT, "null context pointer");
+        *context = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            return fail(EMP_HIP_DEVICE_ERROR, "no HIP device available");
+        if (device < 0 || device >= count) return fail(EMP_HIP_BAD_ARGUMENT, "device index out of range");
+        if (hipSetDevice(device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+            return fail(EMP_HIP_DEVICE_ERROR, "hipGetDeviceProperties failed");
+        std::unique_ptr<emp_hip_context> ctx(new emp_hip_context);
+        ctx->dev.device = device;
+        ctx->dev.compute_units = (uint32_t)prop.multiProcessorCount;
+        ctx->dev.lds_per_block = prop.sharedMemPerBlock;
+        ctx->dev.clock_khz = (uint32_t)prop.clockRate;
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(EMP_HIP_DEVICE_ERROR, "hipStreamCreate failed");
+        *context = ctx.release();
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_destroy(emp_hip_context *context) {
+    return guarded([&] {
+        if (!context) return EMP_HIP_OK;
+        (void)hipSetDevice(context->dev.device);
+        if (context->stream) (void)hipStreamDestroy(context->stream);
+        delete context;
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_device_info(emp_hip_context *context, uint32_t *compute_units,
+                        uint32_t *lds_per_block, uint32_t *clock_khz) {
+    return guarded([&] {
+        if (!context) return fail(EMP_HIP_BAD_ARGUMENT, "null context");
+        if (compute_units) *compute_units = context->dev.compute_units;
+        if (lds_per_block) *lds_per_block = (uint32_t)context->dev.lds_per_block;
+        if (clock_khz) *clock_khz = context->dev.clock_khz;
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_generate(uint64_t seed, uint32_t width, uint32_t height,
+                     uint32_t border_colours, uint32_t interior_colours,
+                     uint32_t *colours, uint16_t *planted) {
+    return guarded([&] {
+        if (!colours) return fail(EMP_HIP_BAD_ARGUMENT, "null colour buffer");
+        std::vector<uint16_t> pl;
+        std::vector<uint32_t> c = emp::generate(seed, width, height, border_colours, interior_colours,
+                                                planted ? &pl : nullptr);
+        std::copy(c.begin(), c.end(), colours);
+        if (planted) std::copy(pl.begin(), pl.end(), planted);
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_validate_puzzle(const emp_hip_puzzle *puzzle) {
+    return guarded([&] {
+        if (!puzzle) return fail(EMP_HIP_BAD_ARGUMENT, "null puzzle");
+        emp::Puzzle p(puzzle->width, puzzle->height, puzzle->colours);
+        (void)p;
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_order(uint32_t width, uint32_t height, uint32_t order, uint32_t *xy) {
+    return guarded([&] {
+        if (!xy || width < 1 || height < 1) return fail(EMP_HIP_BAD_ARGUMENT, "bad order arguments");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC)
+            return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        auto o = emp::make_order(width, height, (int)order);
+        for (size_t i = 0; i < o.size(); ++i) {
+            xy[2 * i] = o[i].first;
+            xy[2 * i + 1] = o[i].second;
+        }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_situation_string(uint32_t width, uint32_t height, const uint16_t *board,
+                             char *out, uint64_t capacity) {
+    return guarded([&] {
+        if (!board || !out || !capacity) return fail(EMP_HIP_BAD_ARGUMENT, "bad buffers");
+        std::string s = emp::situation_string(width * height, board);
+        if (s.size() + 1 > capacity) return fail(EMP_HIP_BAD_ARGUMENT, "output buffer too small");
+        std::memcpy(out, s.c_str(), s.size() + 1);
+        return EMP_HIP_OK;
+    });
+}
+
+void emp_hip_enum_default_options(emp_hip_enum_options *options) {
+    if (!options) return;
+    std::memset(options, 0, sizeof(*options));
+    options->order = EMP_HIP_ORDER_SPIRAL;
+    options->max_solutions = 1024;
+}
+
+int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                      const emp_hip_enum_options *options, emp_hip_enum_result *results,
+                      uint16_t *const *boards, emp_hip_enum_stats *stats) {
+    return guarded([&] {
+        if (!context || !puzzles || !results || !n_puzzles)
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument or empty batch");
+        emp_hip_enum_options opt;
+        emp_hip_enum_default_options(&opt);
+        if (options) opt = *options;
+        if (opt.order != EMP_HIP_ORDER_SPIRAL && opt.order != EMP_HIP_ORDER_BASIC)
+            return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        if (opt.waves_per_block > 4 || (opt.waves_per_block & (opt.waves_per_block - 1)))
+            return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be 1, 2 or 4");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        std::vector<std::unique_ptr<emp::Puzzle>> owned;
+        std::vector<const emp::Puzzle *> list;
+        for (uint32_t i = 0; i < n_puzzles; ++i) {
+            try {
+                owned.emplace_back(new emp::Puzzle(puzzles[i].width, puzzles[i].height, puzzles[i].colours));
+            } catch (const std::invalid_argument &e) {
+                return fail(EMP_HIP_BAD_PUZZLE, "puzzle " + std::to_string(i) + ": " + e.what());
+            }
+            list.push_back(owned.back().get());
+        }
+        emp::EnumConfig cfg;
+        cfg.order = (int)opt.order;
+        cfg.waves_per_cu = opt.waves_per_cu;
+        cfg.waves_per_block = opt.waves_per_block;
+        cfg.steps_per_round = opt.steps_per_round;
+        cfg.step_budget = opt.step_budget;
+        cfg.target_items = opt.target_items;
+        cfg.node_limit = opt.node_limit;
+        // Device-side store of up to max_solutions boards per puzzle.
+        cfg.solution_capacity = opt.max_solutions;
+        std::vector<emp::EnumOutcome> out;
+        emp::EnumStats st;
+        emp::enumerate_on_device(context->dev, context->stream, list, cfg, out, st);
+        for (uint32_t i = 0; i < n_puzzles; ++i) {
+            emp_hip_enum_result &r = results[i];
+            const uint32_t P = list[i]->size();
+            const uint64_t have = out[i].boards.size() / P;
+            r.n_nodes = out[i].nodes;
+            r.n_solutions = out[i].solutions;
+            r.complete = out[i].complete ? 1 : 0;
+            r.status = EMP_HIP_OK;
+            r.solve_ms = out[i].solve_ms;
+            const uint64_t keep = std::min<uint64_t>(have, opt.max_solutions);
+            r.n_stored = (boards && boards[i]) ? keep : 0;
+            if (boards && boards[i] && keep)
+                std::memcpy(boards[i], out[i].boards.data(), keep * P * sizeof(uint16_t));
+        }
+        if (stats) {
+            stats->total_ms = st.total_ms;
+            stats->dfs_kernel_ms = st.dfs_kernel_ms;
+            stats->expand_kernel_ms = st.expand_kernel_ms;
+            stats->dfs_launches = st.dfs_launches;
+            stats->expand_launches = st.expand_launches;
+            stats->items_processed = st.items_processed;
+            stats->splits = st.splits;
+            stats->max_words = st.max_words;
+            stats->lds_tables = st.lds_tables;
+            stats->blocks = st.blocks;
+            stats->threads = st.threads;
+        }
+        return EMP_HIP_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+// emp_enum_plan.cpp -- builds the enumeration table of a puzzle.
+#include "emp_enum_plan.h"
+
+#include <cstring>
+#include <stdexcept>
+
+namespace emp {
+
+EnumTableHeader EnumTable::header() const {
+    EnumTableHeader h;
+    std::memcpy(&h, blob.data(), sizeof(h));
+    return h;
+}
+
+EnumTable build_enum_table(const Puzzle &pz, int order) {
+    EnumTable t;
+    EnumTableHeader h{};
+    const uint32_t P = pz.size();
+    h.P = P;
+    h.W = pz.width();
+    h.H = pz.height();
+    h.ncol = pz.max_colour() + 1;
+    t.order = make_order(pz.width(), pz.height(), order);
+    uint32_t n_kind[3];
+    for (int k = 0; k < 3; ++k) {
+        n_kind[k] = (uint32_t)pz.pieces_of(k).size();
+        h.words[k] = (4 * n_kind[k] + 63) / 64;
+        if (h.words[k] == 0) h.words[k] = 1;
+        for (uint32_t idx : pz.pieces_of(k)) t.piece_of[k].push_back((uint16_t)idx);
+    }
+    h.avail_words = h.words[0] + h.words[1] + h.words[2];
+
+    // Appearance classes: kept transitions of the same kind showing the same
+    // four colours, recorded when they span at least two pieces.  Trying
+    // (A, o) clears the same-looking transitions of the other pieces from the
+    // position (HANDLE_IDENTICAL_PIECES select_piece,
+    // include/emp_position_strategy.h:189-202, masks of include/emp_piece_db.h
+    // :848-884), so among a position's candidates only the lowest piece of a
+    // class keeps its transitions.  A half-similar piece is kept at
+    // orientations 0 and 2 (the database steps by 2^similarity,
+    // include/emp_piece_db.h:1266-1272); both look alike and both stay.
+    std::vector<std::vector<uint64_t>> dup[3];
+    for (int k = 0; k < 3; ++k) {
+        std::vector<uint32_t> look(4 * n_kind[k]);
+        std::vector<char> valid(4 * n_kind[k], 0);
+        for (uint32_t ki = 0; ki < n_kind[k]; ++ki) {
+            const Piece &p = pz.piece(t.piece_of[k][ki]);
+            for (unsigned o = 0; o < 4; ++o) {
+                look[4 * ki + o] = shown(p, 0, o) | shown(p, 1, o) << 8 | shown(p, 2, o) << 16 | shown(p, 3, o) << 24;
+                valid[4 * ki + o] = Puzzle::orientation_kept(p, o);
+            }
+        }
+        std::vector<char> seen(4 * n_kind[k], 0);
+        for (uint32_t a = 0; a < 4 * n_kind[k]; ++a) {
+            if (!valid[a] || seen[a]) continue;
+            std::vector<uint64_t> mask(h.words[k], 0);
+            bool several_pieces = false;
+            for (uint32_t b = a; b < 4 * n_kind[k]; ++b) {
+                if (!valid[b] || look[b] != look[a]) continue;
+                seen[b] = 1;
+                mask[b >> 6] |= 1ull << (b & 63);
+                several_pieces |= (b >> 2) != (a >> 2);
+            }
+            if (several_pieces) dup[k].push_back(mask);
+        }
+        h.ndup[k] = (uint32_t)dup[k].size();
+    }
+
+    // Layout.
+    uint32_t off = kHeaderWords;
+    h.off_pos = off;
+    off += P;
+    for (int k = 0; k < 3; ++k) {
+        h.off_cm[k] = off;
+        off += 4 * h.ncol * h.words[k];
+        h.off_valid[k] = off;
+        off += h.words[k];
+        h.off_dup[k] = off;
+        off += h.ndup[k] * h.words[k];
+        h.off_rot[k] = off;
+        off += (4 * n_kind[k] + 1) / 2;
+    }
+    h.total_words = off;
+    t.blob.assign(off, 0);
+    uint64_t *b = t.blob.data();
+
+    // Positions.
+    std::vector<uint32_t> index_of(P);
+    for (uint32_t i = 0; i < P; ++i) index_of[t.order[i].second * h.W + t.order[i].first] = i;
+    std::vector<int> kind_at(P);
+    for (uint32_t i = 0; i < P; ++i) kind_at[i] = pz.position_kind(t.order[i].first, t.order[i].second);
+    for (uint32_t i = 0; i < P; ++i) {
+        uint32_t x = t.order[i].first, y = t.order[i].second;
+        uint64_t rec = (uint64_t)kind_at[i];
+        for (unsigned s = 0; s < 4; ++s) {
+            int nx = (int)x + (s == 1) - (s == 3), ny = (int)y + (s == 2) - (s == 0);
+            uint32_t code, nk = 0;
+            if (nx < 0 || ny < 0 || nx >= (int)h.W || ny >= (int)h.H) code = kOffBoard;
+            else {
+                uint32_t j = index_of[ny * h.W + nx];
+                if (j < i) { code = j; nk = (uint32_t)kind_at[j]; }
+                else code = kUnconstrained;
+            }
+            rec |= (uint64_t)code << (2 + 11 * s);
+            rec |= (uint64_t)nk << (11 + 11 * s);
+        }
+        b[h.off_pos + i] = rec;
+    }
+
+    // Colour masks, valid-orientation masks, classes, rotated colours.
+    for (int k = 0; k < 3; ++k) {
+        uint64_t *cm = b + h.off_cm[k];
+        uint64_t *valid = b + h.off_valid[k];
+        uint32_t *rot = reinterpret_cast<uint32_t *>(b + h.off_rot[k]);
+        for (uint32_t ki = 0; ki < n_kind[k]; ++ki) {
+            const Piece &p = pz.piece(t.piece_of[k][ki]);
+            for (unsigned o = 0; o < 4; ++o) {
+                uint32_t tr = 4 * ki + o;
+                rot[tr] = shown(p, 0, o) | shown(p, 1, o) << 8 | shown(p, 2, o) << 16 | shown(p, 3, o) << 24;
+                if (!Puzzle::orientation_kept(p, o)) continue;
+                valid[tr >> 6] |= 1ull << (tr & 63);
+                for (unsigned s = 0; s < 4; ++s) {
+                    uint32_t c = shown(p, s, o);
+                    cm[((size_t)s * h.ncol + c) * h.words[k] + (tr >> 6)] |= 1ull << (tr & 63);
+                }
+            }
+        }
+        for (uint32_t d = 0; d < h.ndup[k]; ++d)
+            for (uint32_t w = 0; w < h.words[k]; ++w) b[h.off_dup[k] + d * h.words[k] + w] = dup[k][d][w];
+    }
+
+    // Root: first candidate of index 0, where no neighbour is placed yet.
+    {
+        uint64_t rec = b[h.off_pos];
+        int k = (int)(rec & 3);
+        std::vector<uint64_t> c(b + h.off_valid[k], b + h.off_valid[k] + h.words[k]);
+        for (unsigned s = 0; s < 4; ++s) {
+            uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
+            if (code != kOffBoard) continue;
+            const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + 0) * h.words[k];
+            for (uint32_t w = 0; w < h.words[k]; ++w) c[w] &= m[w];
+        }
+        int root = -1;
+        for (uint32_t w = 0; w < h.words[k] && root < 0; ++w)
+            if (c[w]) root = (int)(w * 64 + __builtin_ctzll(c[w]));
+        if (root < 0) throw std::invalid_argument("no piece fits the first position");
+        h.root = (uint32_t)root;
+    }
+    std::memcpy(b, &h, sizeof(h));
+    return t;
+}
+
+void stack_to_board(const EnumTable &t, const uint16_t *stack, uint16_t *board) {
+    EnumTableHeader h = t.header();
+    for (uint32_t i = 0; i < h.P; ++i) board[i] = 0xFFFF;
+    for (uint32_t i = 0; i < h.P; ++i) {
+        int k = (int)(t.blob[h.off_pos + i] & 3);
+        uint32_t tr = stack[i];
+        uint32_t piece = t.piece_of[k][tr >> 2];
+        board[t.order[i].second * h.W + t.order[i].first] = (uint16_t)((piece << 2) | (tr & 3));
+    }
+}
+
+}  // namespace emp

@@ -1,0 +1,65 @@
+// emp_enum_plan.h -- per-puzzle table consumed by the enumeration kernels.
+//
+// A table is one contiguous blob of u64 words (header first) that the DFS
+// kernel copies into LDS (single puzzle) or reads through the cache (batches
+// of puzzles).  For each position kind K the candidate space is the set of
+// transitions t = (kind_index << 2) | o of pieces of that kind, a bitfield of
+// words[K] u64 words; this is the reference's per-kind constraint bitfield
+// (include/emp_piece_db.h:1266-1312, include/emp_position_strategy.h:172-187).
+//
+// Instead of the reference's constraint database indexed by a packed code of
+// four colours, the candidates of a position are the AND of one colour mask
+// per constrained side: cm[K][side][colour] = transitions whose side `side`
+// shows `colour` (colour 0 = off-board frame).  The result is identical:
+// exactly the transitions matching every constrained side.
+#ifndef EMP_ENUM_PLAN_H
+#define EMP_ENUM_PLAN_H
+
+#include <cstdint>
+#include <vector>
+
+#include "emp_model.h"
+
+namespace emp {
+
+// Position record (one u64 per strategy index):
+//   bits 0-1            kind of the position
+//   bits 2+11s .. +8    side s: strategy index of the neighbour when it is
+//                       placed earlier, kOffBoard, or kUnconstrained
+//   bits 11+11s .. +1   kind of that neighbour
+constexpr uint32_t kOffBoard = 0x1FE;
+constexpr uint32_t kUnconstrained = 0x1FF;
+
+struct EnumTableHeader {
+    uint32_t P, W, H, ncol;      // ncol = max colour + 1 (colour 0 = frame)
+    uint32_t words[3];           // u64 words of a kind's candidate bitfield
+    uint32_t ndup[3];            // appearance classes with >= 2 transitions
+    uint32_t off_pos;            // u64 offsets into the table blob
+    uint32_t off_cm[3];          // [4 sides][ncol][words[K]]
+    uint32_t off_valid[3];       // [words[K]] orientations kept per piece
+    uint32_t off_dup[3];         // [ndup[K]][words[K]]
+    uint32_t off_rot[3];         // u32 per transition: colours N|E<<8|S<<16|W<<24
+    uint32_t root;               // transition placed at index 0
+    uint32_t total_words;        // size of the blob in u64
+    uint32_t avail_words;        // words[0] + words[1] + words[2]
+};
+static_assert(sizeof(EnumTableHeader) % 8 == 0, "header must be u64 aligned");
+constexpr uint32_t kHeaderWords = sizeof(EnumTableHeader) / 8;
+
+struct EnumTable {
+    std::vector<uint64_t> blob;
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    std::vector<uint16_t> piece_of[3];   // kind_index -> piece index (id - 1)
+    EnumTableHeader header() const;
+};
+
+// Builds the table; throws std::invalid_argument when the first position has
+// no candidate.
+EnumTable build_enum_table(const Puzzle &puzzle, int order);
+
+// Converts a stack of transitions (one per strategy index) to a row-major board.
+void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *board);
+
+}  // namespace emp
+
+#endif

@@ -1,0 +1,65 @@
+// emp_enumerate.h -- host interface of the enumeration path.
+#ifndef EMP_ENUMERATE_H
+#define EMP_ENUMERATE_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emp_model.h"
+
+namespace emp {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct OutOfMemory : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+struct DeviceInfo {
+    int device = 0;
+    uint32_t compute_units = 0;
+    size_t lds_per_block = 0;
+    uint32_t clock_khz = 0;
+};
+
+struct EnumConfig {
+    int order = kSpiral;
+    uint32_t waves_per_cu = 0;
+    uint32_t waves_per_block = 0;
+    uint32_t steps_per_round = 0;
+    uint64_t step_budget = 0;
+    uint64_t target_items = 0;
+    uint64_t node_limit = 0;
+    uint64_t solution_capacity = 1 << 16;
+    uint32_t max_expansions = 0;
+};
+
+struct EnumOutcome {
+    uint64_t nodes = 0;
+    uint64_t solutions = 0;
+    bool complete = true;
+    double solve_ms = 0;
+    std::vector<uint16_t> boards;   // stored solutions, P cells each
+};
+
+struct EnumStats {
+    double total_ms = 0, dfs_kernel_ms = 0, expand_kernel_ms = 0;
+    uint64_t dfs_launches = 0, expand_launches = 0, items_processed = 0, splits = 0;
+    uint32_t max_words = 0, lds_tables = 0, blocks = 0, threads = 0;
+};
+
+void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
+                         const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
+                         std::vector<EnumOutcome> &out, EnumStats &stats);
+
+}  // namespace emp
+
+#endif

@@ -1,0 +1,723 @@
+// emp_enumerate.hip -- exhaustive enumeration on gfx950.
+//
+// Work decomposition.  A work item is a subtree of the reference's DFS
+// (include/emp_strategy.h:332-415): the transitions placed at strategy indices
+// 0..kmin-1 and, at index kmin, the candidates above `last` (last = -1: all).
+// The root item is the first candidate of index 0 (the reference never
+// revisits index 0, include/emp_strategy.h:389).
+//
+//   expand_kernel  one thread per item: emits one child item per candidate at
+//                  index kmin (prefix split, the breadth step of the
+//                  reference's glutton_wide play_from,
+//                  include/CUDA_glutton_situations.h:713-757).
+//   dfs_kernel     persistent: each lane of a wave64 owns one item and runs
+//                  the DFS below it.  Idle lanes are refilled from the item
+//                  queue with one atomic per wave (ballot / popcount / mbcnt).
+//                  A lane that exceeds the step budget splits what remains of
+//                  its subtree into at most kmin..k residual items written to
+//                  the next pass's queue, so heavy subtrees are shared out.
+//
+// Per-lane state lives in LDS (lane-interleaved so a wave's accesses fall in
+// distinct banks): the stack of chosen transitions (u16 per index) and the
+// available-piece bitfield of each kind.  The candidate bitfield of a position
+// is recomputed on every visit from the colour masks of its placed neighbours
+// (held in LDS when one puzzle is searched), which keeps the per-lane stack at
+// 2 bytes per level instead of one bitfield per level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "emp_enum_plan.h"
+#include "emp_enumerate.h"
+
+namespace emp {
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct DfsArgs {
+    const uint64_t *tables;        // concatenated table blobs
+    const uint32_t *table_off;     // [n_puzzles] u64 offset of each blob
+    uint32_t lds_table_words;      // > 0: puzzle 0's blob is copied into LDS
+    uint32_t avail_words;          // max avail words over puzzles (per lane)
+    uint32_t stride;               // u16 per item stack (max P, even)
+    uint32_t steps_per_round;
+    uint64_t step_budget;
+    // input queue
+    const uint64_t *in_hdr;        // puzzle | kmin << 32 | (last + 1) << 48
+    const uint16_t *in_stack;      // [n_in][stride]
+    uint32_t n_in;
+    unsigned int *in_head;
+    // output queue (residual items)
+    uint64_t *out_hdr;
+    uint16_t *out_stack;
+    uint32_t out_cap;
+    unsigned int *out_count;
+    unsigned int *flags;           // [0] output overflow, [1] splits, [2] items
+    // per puzzle
+    unsigned long long *nodes;
+    unsigned long long *solutions;
+    unsigned long long *first_tick;
+    unsigned long long *last_tick;
+    const unsigned long long *node_limit;   // 0 = none
+    unsigned int *truncated;
+    // solution store: the s-th solution of puzzle p (in discovery order)
+    // goes to slot p * sol_per_puzzle + s while s < sol_per_puzzle
+    uint16_t *sol_stack;           // [n_puzzles * sol_per_puzzle][stride]
+    uint32_t sol_per_puzzle;
+};
+
+__device__ __forceinline__ uint64_t make_hdr(uint32_t puzzle, uint32_t kmin, int last) {
+    return (uint64_t)puzzle | ((uint64_t)kmin << 32) | ((uint64_t)(uint32_t)(last + 1) << 48);
+}
+__device__ __forceinline__ uint32_t hdr_puzzle(uint64_t h) { return (uint32_t)h; }
+__device__ __forceinline__ uint32_t hdr_kmin(uint64_t h) { return (uint32_t)(h >> 32) & 0xFFFF; }
+__device__ __forceinline__ int hdr_last(uint64_t h) { return (int)((uint32_t)(h >> 48) & 0xFFFF) - 1; }
+constexpr uint32_t kDeadItem = 0xFFFF;  // kmin of a slot reserved but never written
+
+// Read-only view of one table blob.  Per-kind fields are kept as three
+// scalars and picked with selects, so a runtime kind never indexes a private
+// array (which would spill it to scratch memory).
+template <typename T>
+__device__ __forceinline__ T sel3(int k, T a0, T a1, T a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
+
+struct TableView {
+    const uint64_t *b;
+    uint32_t P, ncol, off_pos;
+    uint32_t w0, w1, w2, d0, d1, d2;
+    uint32_t cm0, cm1, cm2, va0, va1, va2, du0, du1, du2, ro0, ro1, ro2;
+
+    __device__ __forceinline__ void load(const uint64_t *blob) {
+        b = blob;
+        const uint32_t *h = reinterpret_cast<const uint32_t *>(blob);
+        // field order of EnumTableHeader
+        P = h[0];
+        ncol = h[3];
+        w0 = h[4]; w1 = h[5]; w2 = h[6];
+        d0 = h[7]; d1 = h[8]; d2 = h[9];
+        off_pos = h[10];
+        cm0 = h[11]; cm1 = h[12]; cm2 = h[13];
+        va0 = h[14]; va1 = h[15]; va2 = h[16];
+        du0 = h[17]; du1 = h[18]; du2 = h[19];
+        ro0 = h[20]; ro1 = h[21]; ro2 = h[22];
+    }
+    __device__ __forceinline__ uint32_t words(int k) const { return sel3(k, w0, w1, w2); }
+    __device__ __forceinline__ uint32_t ndup(int k) const { return sel3(k, d0, d1, d2); }
+    __device__ __forceinline__ uint32_t off_cm(int k) const { return sel3(k, cm0, cm1, cm2); }
+    __device__ __forceinline__ uint32_t off_valid(int k) const { return sel3(k, va0, va1, va2); }
+    __device__ __forceinline__ uint32_t off_dup(int k) const { return sel3(k, du0, du1, du2); }
+    __device__ __forceinline__ uint32_t abase(int k) const { return sel3(k, 0u, w0, w0 + w1); }
+    __device__ __forceinline__ uint64_t pos(uint32_t k) const { return b[off_pos + k]; }
+    __device__ __forceinline__ uint32_t rot(int kind, uint32_t t) const {
+        return reinterpret_cast<const uint32_t *>(b + sel3(kind, ro0, ro1, ro2))[t];
+    }
+};
+
+static_assert(sizeof(EnumTableHeader) == 26 * 4, "TableView::load reads EnumTableHeader by field index");
+
+// Candidates of strategy index k: AND of the available pieces of its kind and
+// the colour masks of its constrained sides; in each appearance class only
+// the lowest piece present keeps its transitions; then only transitions above
+// `last`.  The class reduction is applied to the full candidate set, as the
+// reference's identical-piece masks accumulate from the first candidate on.
+// get_choice(j): transition placed at index j; get_avail(K, w): available
+// transition word w of kind K.
+template <int MAXW, typename GetChoice, typename GetAvail>
+__device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, int last,
+                                           GetChoice get_choice, GetAvail get_avail,
+                                           uint64_t (&c)[MAXW], int &kind_out) {
+    const uint64_t rec = tv.pos(k);
+    const int K = (int)(rec & 3);
+    const uint32_t wK = tv.words(K);
+    kind_out = K;
+#pragma unroll
+    for (int w = 0; w < MAXW; ++w) c[w] = (w < (int)wK) ? get_avail(K, w) : 0ull;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
+        if (code == kUnconstrained) continue;
+        uint32_t col = 0;
+        if (code != kOffBoard) {
+            const int nk = (int)(rec >> (11 + 11 * s)) & 3;
+            const uint32_t tn = get_choice(code);
+            col = (tv.rot(nk, tn) >> (8 * ((s + 2) & 3))) & 0xFF;
+        }
+        const uint64_t *cm = tv.b + tv.off_cm(K) + ((uint32_t)s * tv.ncol + col) * wK;
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w)
+            if (w < (int)wK) c[w] &= cm[w];
+    }
+    for (uint32_t d = 0, nd = tv.ndup(K); d < nd; ++d) {
+        const uint64_t *dm = tv.b + tv.off_dup(K) + d * wK;
+        bool seen = false;
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w) {
+            if (w < (int)wK) {
+                const uint64_t m = c[w] & dm[w];
+                if (seen) c[w] &= ~m;
+                else if (m) {
+                    // keep the transitions of the lowest piece of the class
+                    const uint64_t keep = 0xFull << (__builtin_ctzll(m) & ~3);
+                    c[w] &= ~(m & ~keep);
+                    seen = true;
+                }
+            }
+        }
+    }
+    if (last >= 0) {
+        const int from = last + 1;
+        const int fw = from >> 6, fb = from & 63;
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w) {
+            if (w < fw) c[w] = 0;
+            else if (w == fw) c[w] &= ~0ull << fb;
+        }
+    }
+}
+
+template <int MAXW>
+__device__ __forceinline__ int first_bit(const uint64_t (&c)[MAXW]) {
+    int t = -1;
+#pragma unroll
+    for (int w = MAXW - 1; w >= 0; --w)
+        if (c[w]) t = w * 64 + __builtin_ctzll(c[w]);
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Prefix expansion: one thread per item.
+// ---------------------------------------------------------------------------
+template <int MAXW>
+__global__ void expand_kernel(DfsArgs a) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.n_in) return;
+    const uint64_t hdr = a.in_hdr[idx];
+    const uint32_t kmin = hdr_kmin(hdr);
+    if (kmin == kDeadItem) return;
+    const uint32_t puzzle = hdr_puzzle(hdr);
+    const int last = hdr_last(hdr);
+    TableView tv;
+    tv.load(a.tables + a.table_off[puzzle]);
+    const uint16_t *stack = a.in_stack + (size_t)idx * a.stride;
+    uint64_t avail[3][MAXW];
+#pragma unroll
+    for (int K = 0; K < 3; ++K)
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w)
+            avail[K][w] = (w < (int)tv.words(K)) ? tv.b[tv.off_valid(K) + w] : 0ull;
+    for (uint32_t j = 0; j < kmin; ++j) {
+        const int K = (int)(tv.pos(j) & 3);
+        const uint32_t t = stack[j];
+        const uint32_t base = t & ~3u;
+#pragma unroll
+        for (int w = 0; w < MAXW; ++w)
+            if (w == (int)(base >> 6)) avail[K][w] &= ~(0xFull << (base & 63));
+    }
+    uint64_t c[MAXW];
+    int K;
+    candidates<MAXW>(tv, kmin, last, [&](uint32_t j) { return (uint32_t)stack[j]; },
+                     [&](int kk, int w) {
+                         uint64_t v = 0;
+#pragma unroll
+                         for (int q = 0; q < 3; ++q)
+#pragma unroll
+                             for (int x = 0; x < MAXW; ++x)
+                                 if (q == kk && x == w) v = avail[q][x];
+                         return v;
+                     },
+                     c, K);
+    uint32_t n = 0;
+#pragma unroll
+    for (int w = 0; w < MAXW; ++w) n += (uint32_t)__popcll(c[w]);
+    if (n) atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
+    if (!n) return;
+    if (kmin == tv.P - 1) {
+        uint64_t s = atomicAdd(&a.solutions[puzzle], (unsigned long long)n);
+        for (int w = 0; w < MAXW; ++w) {
+            uint64_t v = c[w];
+            while (v) {
+                const uint32_t t = w * 64 + __builtin_ctzll(v);
+                v &= v - 1;
+                if (s < a.sol_per_puzzle) {
+                    uint16_t *dst = a.sol_stack + ((size_t)puzzle * a.sol_per_puzzle + s) * a.stride;
+                    for (uint32_t j = 0; j < kmin; ++j) dst[j] = stack[j];
+                    dst[kmin] = (uint16_t)t;
+                }
+                ++s;
+            }
+        }
+        return;
+    }
+    const uint32_t base = atomicAdd(a.out_count, n);
+    if (base + n > a.out_cap) {
+        atomicOr(&a.flags[0], 1u);
+        return;
+    }
+    uint32_t o = base;
+    for (int w = 0; w < MAXW; ++w) {
+        uint64_t v = c[w];
+        while (v) {
+            const uint32_t t = w * 64 + __builtin_ctzll(v);
+            v &= v - 1;
+            uint16_t *dst = a.out_stack + (size_t)o * a.stride;
+            for (uint32_t j = 0; j < kmin; ++j) dst[j] = stack[j];
+            dst[kmin] = (uint16_t)t;
+            a.out_hdr[o] = make_hdr(puzzle, kmin + 1, -1);
+            ++o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent DFS.
+// ---------------------------------------------------------------------------
+template <int MAXW, bool LDS_TABLE>
+__global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
+    extern __shared__ uint64_t smem[];
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wave = (int)(threadIdx.x / kWave);
+
+    if (LDS_TABLE) {
+        const uint64_t *src = a.tables + a.table_off[0];
+        for (uint32_t i = threadIdx.x; i < a.lds_table_words; i += blockDim.x) smem[i] = src[i];
+        __syncthreads();
+    }
+    const uint32_t per_wave = a.avail_words * kWave + (a.stride * kWave) / 4;
+    uint64_t *avail = smem + a.lds_table_words + (size_t)wave * per_wave;
+    uint16_t *stk = reinterpret_cast<uint16_t *>(avail + a.avail_words * kWave);
+
+    TableView tv;
+    if (LDS_TABLE) tv.load(smem);
+
+    bool active = false;
+    bool queue_empty = false;
+    uint32_t puzzle = 0, k = 0, kmin = 0, P = 0;
+    int last = -1;
+    uint64_t steps = 0, nodes = 0;
+
+    auto A = [&](int K, int w) -> uint64_t & { return avail[(tv.abase(K) + (uint32_t)w) * kWave + lane]; };
+    auto S = [&](uint32_t j) -> uint16_t & { return stk[j * kWave + lane]; };
+    auto finish = [&]() {
+        atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
+        atomicMax(&a.last_tick[puzzle], (unsigned long long)wall_clock64());
+        active = false;
+    };
+
+    while (true) {
+        // Refill idle lanes with one queue atomic per wave.
+        const uint64_t idle = __ballot(!active);
+        if (idle && !queue_empty) {
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(a.in_head, n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= a.n_in) queue_empty = true;
+            if (!active) {
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+                const uint32_t idx = base + rank;
+                if (idx < a.n_in) {
+                    const uint64_t hdr = a.in_hdr[idx];
+                    kmin = hdr_kmin(hdr);
+                    puzzle = hdr_puzzle(hdr);
+                    bool take = kmin != kDeadItem;
+                    if (take && a.node_limit[puzzle] && a.nodes[puzzle] >= a.node_limit[puzzle]) {
+                        atomicOr(&a.truncated[puzzle], 1u);
+                        take = false;
+                    }
+                    if (take) {
+                        if (!LDS_TABLE) tv.load(a.tables + a.table_off[puzzle]);
+                        P = tv.P;
+                        last = hdr_last(hdr);
+                        k = kmin;
+                        steps = 0;
+                        nodes = 0;
+                        const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
+                        for (int K = 0; K < 3; ++K)
+                            for (uint32_t w = 0, nw = tv.words(K); w < nw; ++w) A(K, (int)w) = tv.b[tv.off_valid(K) + w];
+                        for (uint32_t j = 0; j < kmin; ++j) {
+                            const uint32_t t = src[j];
+                            S(j) = (uint16_t)t;
+                            const int K = (int)(tv.pos(j) & 3);
+                            const uint32_t bb = t & ~3u;
+                            A(K, (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
+                        }
+                        atomicMin(&a.first_tick[puzzle], (unsigned long long)wall_clock64());
+                        atomicAdd(&a.flags[2], 1u);
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__ballot(active)) {
+            if (queue_empty) break;
+            continue;
+        }
+        if (!active) continue;
+
+        for (uint32_t r = 0; r < a.steps_per_round; ++r) {
+            uint64_t c[MAXW];
+            int K;
+            candidates<MAXW>(tv, k, last, [&](uint32_t j) { return (uint32_t)S(j); },
+                             [&](int kk, int w) { return A(kk, w); }, c, K);
+            const int t = first_bit<MAXW>(c);
+            ++steps;
+            if (t < 0) {
+                if (k == kmin) {
+                    finish();
+                    break;
+                }
+                --k;
+                const uint32_t tb = S(k);
+                const int Kb = (int)(tv.pos(k) & 3);
+                const uint32_t bb = tb & ~3u;
+                const uint32_t wv = bb >> 6;
+                A(Kb, (int)wv) |= (0xFull << (bb & 63)) & tv.b[tv.off_valid(Kb) + wv];
+                last = (int)tb;
+            } else {
+                S(k) = (uint16_t)t;
+                ++nodes;
+                if (k == P - 1) {
+                    const uint64_t s = atomicAdd(&a.solutions[puzzle], 1ull);
+                    if (s < a.sol_per_puzzle) {
+                        uint16_t *dst = a.sol_stack + ((size_t)puzzle * a.sol_per_puzzle + s) * a.stride;
+                        for (uint32_t j = 0; j < P; ++j) dst[j] = S(j);
+                    }
+                    last = t;
+                } else {
+                    const uint32_t bb = (uint32_t)t & ~3u;
+                    A(K, (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
+                    ++k;
+                    last = -1;
+                }
+            }
+            if (steps >= a.step_budget) {
+                // Split: what remains is, for each index j in [kmin, k], the
+                // candidates above the current choice at j (above `last` at k).
+                const uint32_t n = k - kmin + 1;
+                const uint32_t base = atomicAdd(a.out_count, n);
+                if (base + n <= a.out_cap) {
+                    for (uint32_t j = kmin; j <= k; ++j) {
+                        const uint32_t o = base + (j - kmin);
+                        uint16_t *dst = a.out_stack + (size_t)o * a.stride;
+                        for (uint32_t q = 0; q < j; ++q) dst[q] = S(q);
+                        a.out_hdr[o] = make_hdr(puzzle, j, j < k ? (int)S(j) : last);
+                    }
+                    atomicAdd(&a.flags[1], 1u);
+                    finish();
+                    break;
+                }
+                // No room: mark the reserved slots dead and keep searching.
+                for (uint32_t o = base; o < base + n && o < a.out_cap; ++o) a.out_hdr[o] = make_hdr(0, kDeadItem, -1);
+                atomicOr(&a.flags[0], 1u);
+                steps = 0;
+            }
+        }
+    }
+}
+
+#define EMP_CHECK(call)                                                                   \
+    do {                                                                                  \
+        hipError_t err_ = (call);                                                         \
+        if (err_ != hipSuccess)                                                           \
+            throw DeviceError(std::string(#call) + ": " + hipGetErrorString(err_));       \
+    } while (0)
+
+template <typename T>
+struct DeviceBuffer {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void allocate(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&ptr, n * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw OutOfMemory("hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+        }
+        count = n;
+    }
+};
+
+// Fill kernel used instead of hipMemsetAsync so passes can be graph-captured.
+__global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int MAXW>
+void launch_expand(const DfsArgs &a, hipStream_t s) {
+    const uint32_t threads = 256, blocks = (a.n_in + threads - 1) / threads;
+    if (blocks) hipLaunchKernelGGL(expand_kernel<MAXW>, dim3(blocks), dim3(threads), 0, s, a);
+}
+
+template <int MAXW>
+void launch_dfs(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes,
+                hipStream_t s) {
+    if (lds) hipLaunchKernelGGL((dfs_kernel<MAXW, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    else hipLaunchKernelGGL((dfs_kernel<MAXW, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+}
+
+void dispatch_expand(int maxw, const DfsArgs &a, hipStream_t s) {
+    switch (maxw) {
+        case 1: launch_expand<1>(a, s); break;
+        case 2: launch_expand<2>(a, s); break;
+        case 4: launch_expand<4>(a, s); break;
+        case 8: launch_expand<8>(a, s); break;
+        default: launch_expand<16>(a, s); break;
+    }
+}
+
+void dispatch_dfs(int maxw, const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads,
+                  size_t lds_bytes, hipStream_t s) {
+    switch (maxw) {
+        case 1: launch_dfs<1>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 2: launch_dfs<2>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 4: launch_dfs<4>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 8: launch_dfs<8>(a, lds, blocks, threads, lds_bytes, s); break;
+        default: launch_dfs<16>(a, lds, blocks, threads, lds_bytes, s); break;
+    }
+}
+
+int round_words(uint32_t w) {
+    if (w <= 1) return 1;
+    if (w <= 2) return 2;
+    if (w <= 4) return 4;
+    if (w <= 8) return 8;
+    return 16;
+}
+
+}  // namespace
+
+void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
+                         const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
+                         std::vector<EnumOutcome> &out, EnumStats &stats) {
+    const uint32_t n = (uint32_t)puzzles.size();
+    out.assign(n, EnumOutcome{});
+    stats = EnumStats{};
+    auto wall0 = std::chrono::steady_clock::now();
+
+    // Tables.
+    std::vector<EnumTable> tables;
+    tables.reserve(n);
+    std::vector<uint32_t> table_off(n);
+    uint32_t total_words = 0, max_words = 1, max_avail = 1, max_P = 0, max_table = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        tables.push_back(build_enum_table(*puzzles[i], cfg.order));
+        EnumTableHeader h = tables.back().header();
+        table_off[i] = total_words;
+        total_words += h.total_words;
+        for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
+        max_avail = std::max(max_avail, h.avail_words);
+        max_P = std::max(max_P, h.P);
+        max_table = std::max(max_table, h.total_words);
+    }
+    if (max_words > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
+    const int maxw = round_words(max_words);
+    const uint32_t stride = (max_P + 3) & ~3u;   // u16 per item, multiple of 4
+    stats.max_words = max_words;
+
+    std::vector<uint64_t> blob(total_words);
+    for (uint32_t i = 0; i < n; ++i)
+        std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
+
+    // Launch geometry.
+    const bool lds_table = n == 1;
+    const uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
+    const uint32_t threads = wpb * kWave;
+    const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * 2;
+    const size_t lds_bytes = (lds_table ? (size_t)max_table * 8 : 0) + wpb * per_wave_bytes;
+    if (lds_bytes > dev.lds_per_block)
+        throw Unsupported("per-block LDS of " + std::to_string(lds_bytes) + " bytes exceeds the device limit");
+    const uint32_t wpc = cfg.waves_per_cu ? cfg.waves_per_cu : 8;
+    const uint32_t blocks = std::max(1u, dev.compute_units * wpc / wpb);
+    const uint64_t lanes = (uint64_t)blocks * threads;
+    stats.lds_tables = lds_table;
+    stats.blocks = blocks;
+    stats.threads = threads;
+
+    // Device buffers.
+    DeviceBuffer<uint64_t> d_tables;
+    DeviceBuffer<uint32_t> d_table_off;
+    d_tables.allocate(total_words);
+    d_table_off.allocate(n);
+    EMP_CHECK(hipMemcpyAsync(d_tables.ptr, blob.data(), blob.size() * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_table_off.ptr, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
+
+    const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
+    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 8, (uint64_t)n * 4, 1u << 16});
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(cap64, 1u << 26);
+    DeviceBuffer<uint64_t> d_hdr[2];
+    DeviceBuffer<uint16_t> d_stack[2];
+    for (int q = 0; q < 2; ++q) {
+        d_hdr[q].allocate(cap);
+        d_stack[q].allocate((size_t)cap * stride);
+    }
+    DeviceBuffer<unsigned long long> d_nodes, d_sols, d_first, d_last, d_limit;
+    DeviceBuffer<unsigned int> d_trunc, d_counters;
+    d_nodes.allocate(n);
+    d_sols.allocate(n);
+    d_first.allocate(n);
+    d_last.allocate(n);
+    d_limit.allocate(n);
+    d_trunc.allocate(n);
+    d_counters.allocate(8);   // [0] in_head [1] out_count [2..4] flags
+    const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
+    DeviceBuffer<uint16_t> d_sol_stack;
+    d_sol_stack.allocate((size_t)n * std::max(sol_per_puzzle, 1u) * stride);
+
+    // Root items: index 0 holds its first candidate (one node each).
+    std::vector<uint64_t> h_hdr(n);
+    std::vector<uint16_t> h_stack((size_t)n * stride, 0);
+    std::vector<unsigned long long> h_nodes(n, 1), h_zero(n, 0), h_first(n, ~0ull), h_limit(n, cfg.node_limit);
+    for (uint32_t i = 0; i < n; ++i) {
+        h_hdr[i] = (uint64_t)i | (1ull << 32) | (0ull << 48);
+        h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
+    }
+    EMP_CHECK(hipMemcpyAsync(d_hdr[0].ptr, h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_stack[0].ptr, h_stack.data(), h_stack.size() * 2, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_nodes.ptr, h_nodes.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_sols.ptr, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_first.ptr, h_first.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_last.ptr, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_limit.ptr, h_limit.data(), n * 8, hipMemcpyHostToDevice, stream));
+    std::vector<unsigned int> h_zero32(std::max<uint32_t>(n, 8), 0);
+    EMP_CHECK(hipMemcpyAsync(d_trunc.ptr, h_zero32.data(), n * 4, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_counters.ptr, h_zero32.data(), 8 * 4, hipMemcpyHostToDevice, stream));
+
+    DfsArgs a{};
+    a.tables = d_tables.ptr;
+    a.table_off = d_table_off.ptr;
+    a.lds_table_words = lds_table ? max_table : 0;
+    a.avail_words = max_avail;
+    a.stride = stride;
+    a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 32;
+    a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 18);
+    a.nodes = d_nodes.ptr;
+    a.solutions = d_sols.ptr;
+    a.first_tick = d_first.ptr;
+    a.last_tick = d_last.ptr;
+    a.node_limit = d_limit.ptr;
+    a.truncated = d_trunc.ptr;
+    a.in_head = d_counters.ptr + 0;
+    a.out_count = d_counters.ptr + 1;
+    a.flags = d_counters.ptr + 2;
+    a.sol_stack = d_sol_stack.ptr;
+    a.sol_per_puzzle = sol_per_puzzle;
+
+    hipEvent_t ev0, ev1;
+    EMP_CHECK(hipEventCreate(&ev0));
+    EMP_CHECK(hipEventCreate(&ev1));
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } guard{ev0, ev1};
+
+    // Counters saved before each expansion so that an expansion whose
+    // children do not fit can be undone and its items searched instead.
+    DeviceBuffer<unsigned long long> d_nodes_save, d_sols_save;
+    DeviceBuffer<unsigned int> d_counters_save;
+    d_nodes_save.allocate(n);
+    d_sols_save.allocate(n);
+    d_counters_save.allocate(8);
+
+    int cur = 0;
+    uint64_t n_items = n;
+    uint32_t expansions = 0;
+    const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 64;
+    while (n_items) {
+        a.in_hdr = d_hdr[cur].ptr;
+        a.in_stack = d_stack[cur].ptr;
+        a.n_in = (uint32_t)n_items;
+        a.out_hdr = d_hdr[1 - cur].ptr;
+        a.out_stack = d_stack[1 - cur].ptr;
+        a.out_cap = cap;
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters.ptr, 5u, 0u);
+        const bool expand = n_items < target && expansions < max_expansions;
+        if (expand) {
+            EMP_CHECK(hipMemcpyAsync(d_nodes_save.ptr, d_nodes.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_sols_save.ptr, d_sols.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_counters_save.ptr, d_counters.ptr, 8 * 4, hipMemcpyDeviceToDevice, stream));
+        }
+        EMP_CHECK(hipEventRecord(ev0, stream));
+        if (expand) dispatch_expand(maxw, a, stream);
+        else dispatch_dfs(maxw, a, lds_table, blocks, threads, lds_bytes, stream);
+        EMP_CHECK(hipGetLastError());
+        EMP_CHECK(hipEventRecord(ev1, stream));
+        unsigned int counters[8];
+        EMP_CHECK(hipMemcpyAsync(counters, d_counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+        float ms = 0;
+        EMP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        if (expand) {
+            ++expansions;
+            stats.expand_kernel_ms += ms;
+            ++stats.expand_launches;
+            if (counters[2]) {
+                // Children did not fit: undo the expansion's counts (the
+                // input items are intact) and search the items instead.
+                EMP_CHECK(hipMemcpyAsync(d_nodes.ptr, d_nodes_save.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
+                EMP_CHECK(hipMemcpyAsync(d_sols.ptr, d_sols_save.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
+                EMP_CHECK(hipMemcpyAsync(d_counters.ptr, d_counters_save.ptr, 8 * 4, hipMemcpyDeviceToDevice, stream));
+                expansions = max_expansions;
+                continue;
+            }
+        } else {
+            stats.dfs_kernel_ms += ms;
+            ++stats.dfs_launches;
+            stats.splits += counters[3];
+            stats.items_processed += counters[4];
+        }
+        n_items = std::min<uint64_t>(counters[1], cap);
+        cur = 1 - cur;
+    }
+
+    // Results.
+    std::vector<unsigned long long> r_nodes(n), r_sols(n), r_first(n), r_last(n);
+    std::vector<unsigned int> r_trunc(n);
+    EMP_CHECK(hipMemcpyAsync(r_nodes.data(), d_nodes.ptr, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_sols.data(), d_sols.ptr, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_first.data(), d_first.ptr, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_last.data(), d_last.ptr, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_trunc.data(), d_trunc.ptr, n * 4, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipStreamSynchronize(stream));
+    std::vector<uint16_t> r_stack((size_t)n * sol_per_puzzle * stride);
+    if (!r_stack.empty()) {
+        EMP_CHECK(hipMemcpyAsync(r_stack.data(), d_sol_stack.ptr, r_stack.size() * 2, hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+    }
+    const double tick_ms = 1e-5;   // wall_clock64 runs at 100 MHz on gfx9
+    for (uint32_t i = 0; i < n; ++i) {
+        EnumOutcome &o = out[i];
+        o.nodes = r_nodes[i];
+        o.solutions = r_sols[i];
+        o.complete = !r_trunc[i];
+        o.solve_ms = (r_first[i] != ~0ull && r_last[i] >= r_first[i]) ? (double)(r_last[i] - r_first[i]) * tick_ms : 0.0;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        EnumOutcome &o = out[i];
+        const uint32_t P = tables[i].header().P;
+        const uint64_t kept = std::min<uint64_t>(o.solutions, sol_per_puzzle);
+        o.boards.resize(kept * P);
+        for (uint64_t s = 0; s < kept; ++s)
+            stack_to_board(tables[i], &r_stack[((size_t)i * sol_per_puzzle + s) * stride], &o.boards[s * P]);
+    }
+    stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+}  // namespace emp

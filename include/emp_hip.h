@@ -1,0 +1,152 @@
+/*
+ * emp_hip.h -- C ABI of libemp_hip.so, the MI355X (gfx950) search path of
+ * edge_matching_puzzle.
+ *
+ * The reference binds its GPU search through three free launchers called by
+ * its feature classes (include/feature_CUDA_backtracker.h:68-72,
+ * include/feature_CUDA_glutton_max.h:55-57, include/feature_CUDA_glutton_wide.h
+ * :57-59).  Those launchers take the piece database and board description and
+ * print their result; this ABI takes the same information as plain arrays and
+ * returns results in caller-owned buffers, so any host language can bind it.
+ *
+ * Conventions
+ *   - Every entry point returns EMP_HIP_OK (0) or an error code; the message
+ *     of the last error on the calling thread is emp_hip_last_error().
+ *   - A puzzle is (width, height, colours[P][4]) with colours N, E, S, W of
+ *     piece ids 1..P in file order and 0 for the frame, i.e. the content of
+ *     the reference's puzzle text file (include/emp_pieces_parser.h:53-104).
+ *   - A board is row-major, one uint16 per cell: ((id - 1) << 2) | orientation
+ *     (orientation 0..3 = N, E, S, W as in include/emp_types.h), 0xFFFF empty.
+ *   - A context owns one HIP stream on one device; calls on one context are
+ *     stream-ordered and synchronous on return.  Contexts are
+ *     thread-compatible: use one per host thread.
+ */
+#ifndef EMP_HIP_H
+#define EMP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMP_HIP_OK              0
+#define EMP_HIP_BAD_ARGUMENT    1
+#define EMP_HIP_BAD_PUZZLE      2
+#define EMP_HIP_DEVICE_ERROR    3
+#define EMP_HIP_OUT_OF_MEMORY   4
+#define EMP_HIP_UNSUPPORTED     5
+
+#define EMP_HIP_ORDER_SPIRAL 0  /* include/emp_spiral_strategy_generator.h */
+#define EMP_HIP_ORDER_BASIC  1  /* include/emp_basic_strategy_generator.h  */
+
+typedef struct emp_hip_context emp_hip_context;
+
+typedef struct {
+    uint32_t width;
+    uint32_t height;
+    const uint32_t *colours;  /* [width * height][4] */
+} emp_hip_puzzle;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *emp_hip_last_error(void);
+
+/* Version string of the library. */
+const char *emp_hip_version(void);
+
+/* Creates a context on HIP device `device` (its own stream). */
+int emp_hip_create(int device, emp_hip_context **context);
+int emp_hip_destroy(emp_hip_context *context);
+
+/* Device facts used by the autotuner: compute units, LDS bytes per block,
+ * clock in kHz. */
+int emp_hip_device_info(emp_hip_context *context, uint32_t *compute_units,
+                        uint32_t *lds_per_block, uint32_t *clock_khz);
+
+/*
+ * Seeded synthetic puzzle (random-edge model): frame edges 0, edges between
+ * two frame cells from border colours 1..B, other edges from interior colours
+ * B+1..B+I; interior pieces given a random quarter turn; ids shuffled.
+ * colours receives [W*H][4]; planted (may be NULL) the planted solution.
+ * No counterpart in the reference (its puzzles come from text files).
+ */
+int emp_hip_generate(uint64_t seed, uint32_t width, uint32_t height,
+                     uint32_t border_colours, uint32_t interior_colours,
+                     uint32_t *colours, uint16_t *planted);
+
+/* Validates a puzzle as the reference's piece database does (kinds match the
+ * board, every colour on an even number of edges, colour count equals twice
+ * the inner edges: include/emp_piece_db.h:466-1010).  No device work. */
+int emp_hip_validate_puzzle(const emp_hip_puzzle *puzzle);
+
+/* Strategy order (include/emp_spiral_strategy_generator.h:52-98,
+ * include/emp_basic_strategy_generator.h:44-53): xy[2i], xy[2i+1]. */
+int emp_hip_order(uint32_t width, uint32_t height, uint32_t order, uint32_t *xy);
+
+/* Situation string of a board (include/situation_string_formatter.h:60-81).
+ * Writes at most capacity bytes including the terminating 0. */
+int emp_hip_situation_string(uint32_t width, uint32_t height, const uint16_t *board,
+                             char *out, uint64_t capacity);
+
+/* ------------------------------------------------------------------------ *
+ * Exhaustive enumeration (the reference's CPU strategy emp_strategy,
+ * include/emp_strategy.h:332-415, run on the GPU): index 0 takes its first
+ * candidate only, candidates are visited in ascending (kind_index << 2 | o),
+ * appearance-identical pieces are tried once per position.
+ * Replaces: launch_CUDA_backtracker (src/CUDA_backtracker.cu:235-269) for the
+ * enumeration use of the backtracker feature.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t order;            /* EMP_HIP_ORDER_*                               */
+    uint32_t waves_per_cu;     /* 0 = tuned default                             */
+    uint32_t waves_per_block;  /* 0 = tuned default (1, 2 or 4)                 */
+    uint32_t steps_per_round;  /* steps between work refills, 0 = default       */
+    uint64_t step_budget;      /* steps per work item per pass before it is
+                                  split, 0 = default                           */
+    uint64_t target_items;     /* prefix-split until this many items, 0 = auto  */
+    uint64_t node_limit;       /* per puzzle: stop taking its work items once
+                                  this many nodes were counted, 0 = none      */
+    uint64_t max_solutions;    /* capacity of each boards[i] in solutions       */
+    uint32_t use_graph;        /* capture each pass in a hipGraph               */
+    uint32_t reserved;
+} emp_hip_enum_options;
+
+typedef struct {
+    uint64_t n_nodes;          /* placements (the reference's node count)      */
+    uint64_t n_solutions;      /* exact when complete                          */
+    uint64_t n_stored;         /* boards written (min(n_solutions, capacity))  */
+    uint32_t complete;         /* 1: whole tree enumerated                     */
+    uint32_t status;           /* EMP_HIP_OK or an error for this puzzle       */
+    double   solve_ms;         /* first work item start to last item end       */
+} emp_hip_enum_result;
+
+typedef struct {
+    double   total_ms;         /* wall time of the device part                 */
+    double   dfs_kernel_ms;    /* sum of DFS pass kernel times (HIP events)    */
+    double   expand_kernel_ms; /* sum of prefix-expansion kernel times         */
+    uint64_t dfs_launches;
+    uint64_t expand_launches;
+    uint64_t items_processed;
+    uint64_t splits;
+    uint32_t max_words;        /* u64 words of the widest candidate mask       */
+    uint32_t lds_tables;       /* 1: tables held in LDS                        */
+    uint32_t blocks;           /* grid of the DFS kernel                       */
+    uint32_t threads;          /* block size of the DFS kernel                 */
+} emp_hip_enum_stats;
+
+void emp_hip_enum_default_options(emp_hip_enum_options *options);
+
+/* boards[i] (may be NULL, or the array may be NULL) receives up to
+ * max_solutions boards of puzzle i; results[i] its counts. stats may be NULL. */
+int emp_hip_enumerate(emp_hip_context *context,
+                      const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                      const emp_hip_enum_options *options,
+                      emp_hip_enum_result *results,
+                      uint16_t *const *boards,
+                      emp_hip_enum_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

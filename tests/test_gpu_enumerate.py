@@ -1,0 +1,120 @@
+"""GPU enumeration (libemp_hip.so, through the C ABI) against the CPU oracle.
+
+Bit-exact bar: node count, solution count and the sorted set of solution
+boards must equal the oracle's for every puzzle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import edge_matching_puzzle_amd as emp
+import oracle_bindings as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _sorted_boards(boards):
+    return sorted(tuple(int(v) for v in b) for b in boards)
+
+
+def _check_same(gpu, ref):
+    assert gpu["complete"]
+    assert gpu["n_nodes"] == ref["n_nodes"]
+    assert gpu["n_solutions"] == ref["n_solutions"]
+    assert _sorted_boards(gpu["boards"]) == _sorted_boards(ref["boards"])
+
+
+@pytest.mark.parametrize("name,nodes,sols", [("3_3", 37, 2), ("4_4", 3284, 12), ("5_5", 37938, 1)])
+def test_reference_examples(gpu_context, example_puzzles, name, nodes, sols):
+    w, h, c = example_puzzles[name]
+    print("case: example", name, flush=True)
+    (r,), st = gpu_context.enumerate([(w, h, c)])
+    assert (r["n_nodes"], r["n_solutions"]) == (nodes, sols)
+    _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_reference_examples_basic_order(gpu_context, example_puzzles):
+    for name in ("3_3", "4_4", "5_5"):
+        w, h, c = example_puzzles[name]
+        print("case: basic order", name, flush=True)
+        (r,), _ = gpu_context.enumerate([(w, h, c)], order="basic")
+        _check_same(r, o.enumerate_(w, h, c, order=o.BASIC))
+
+
+def _c0_puzzles(n, seed0=0):
+    return [(5, 5, emp.generate(seed0 + s, 5, 5, 3, 4)[0]) for s in range(n)]
+
+
+def test_c0_single_launches(gpu_context):
+    """5x5, 3 border / 4 interior colours: one launch per puzzle (LDS tables)."""
+    for w, h, c in _c0_puzzles(8):
+        print("case: c0 single", flush=True)
+        (r,), st = gpu_context.enumerate([(w, h, c)])
+        assert st["lds_tables"] == 1
+        _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_c0_batched(gpu_context):
+    """32 seeds in one launch (tables read through the cache)."""
+    puzzles = _c0_puzzles(32, 100)
+    print("case: c0 batched", flush=True)
+    res, st = gpu_context.enumerate(puzzles)
+    assert st["lds_tables"] == 0
+    for (w, h, c), r in zip(puzzles, res):
+        _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_forced_splits_and_tiny_budget(gpu_context):
+    """A step budget of 64 splits nearly every item many times over."""
+    w, h = 6, 6
+    c, _ = emp.generate(1, w, h, 3, 6)
+    print("case: forced splits 6x6", flush=True)
+    (r,), st = gpu_context.enumerate([(w, h, c)], step_budget=64, target_items=1)
+    assert st["splits"] > 1000
+    _check_same(r, o.enumerate_(w, h, c))
+
+
+@pytest.mark.parametrize("w,h,B,I,seed", [(3, 4, 2, 2, 2), (4, 6, 2, 3, 1), (6, 4, 3, 3, 5), (4, 4, 2, 2, 1),
+                                          (4, 4, 1, 2, 3), (5, 5, 2, 2, 4)])
+def test_identical_and_self_similar_pieces(gpu_context, w, h, B, I, seed):
+    """Few colours: many identical and half-similar pieces, rectangles too."""
+    c, _ = emp.generate(seed, w, h, B, I)
+    print("case: identical", w, h, B, I, seed, flush=True)
+    (r,), _ = gpu_context.enumerate([(w, h, c)], max_solutions=1 << 14)
+    _check_same(r, o.enumerate_(w, h, c, max_solutions=1 << 14))
+
+
+def test_solution_capacity_and_counts(gpu_context):
+    w, h = 4, 4
+    c, _ = emp.generate(3, w, h, 1, 2)
+    ref = o.enumerate_(w, h, c)
+    assert ref["n_solutions"] > 8
+    print("case: capacity", flush=True)
+    (r,), _ = gpu_context.enumerate([(w, h, c)], max_solutions=4)
+    assert r["n_solutions"] == ref["n_solutions"] and len(r["boards"]) == 4
+    refset = set(_sorted_boards(ref["boards"]))
+    assert all(b in refset for b in _sorted_boards(r["boards"]))
+
+
+def test_node_limit_marks_incomplete(gpu_context):
+    w, h = 7, 7
+    c, _ = emp.generate(1, w, h, 4, 7)
+    print("case: node limit", flush=True)
+    (r,), _ = gpu_context.enumerate([(w, h, c)], node_limit=1000)
+    assert not r["complete"]
+
+
+def test_batch_of_uneven_sizes(gpu_context):
+    """Mixed sizes in one launch, the C2 shape at sizes the oracle finishes."""
+    rng = np.random.default_rng(5)
+    puzzles = []
+    for s in range(24):
+        w = int(rng.integers(4, 7))
+        B = int(rng.integers(3, 6))
+        I = int(rng.integers(4, 13))
+        puzzles.append((w, w, emp.generate(1000 + s, w, w, B, I)[0]))
+    print("case: uneven batch", flush=True)
+    res, _ = gpu_context.enumerate(puzzles, max_solutions=4096)
+    for (w, h, c), r in zip(puzzles, res):
+        _check_same(r, o.enumerate_(w, h, c, max_solutions=4096))

@@ -1,0 +1,96 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every symbol include/emp_hip.h declares; generator, orders and formats."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import edge_matching_puzzle_amd as emp
+import oracle_bindings as o
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = []
+    for f in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if f.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(emp_hip_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = emp.lib()
+    names = declared_symbols()
+    assert len(names) >= 10
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(emp.SIGNATURES)
+
+
+def test_version_string():
+    assert b"gfx950" in emp.lib().emp_hip_version()
+
+
+def test_generator_is_deterministic_and_seed_sensitive():
+    a, pa = emp.generate(7, 8, 8, 4, 8)
+    b, pb = emp.generate(7, 8, 8, 4, 8)
+    c, _ = emp.generate(8, 8, 8, 4, 8)
+    assert np.array_equal(a, b) and np.array_equal(pa, pb)
+    assert not np.array_equal(a, c)
+
+
+def test_generator_colour_ranges():
+    c, _ = emp.generate(3, 10, 10, 5, 12)
+    frame_side = c.min(axis=1) == 0
+    assert c.max() <= 17
+    # pieces with a frame side carry border colours on their frame-adjacent sides
+    assert frame_side.sum() == 36
+
+
+@pytest.mark.parametrize("w,h", [(3, 3), (4, 4), (5, 5), (8, 8), (16, 16), (3, 5), (6, 4), (10, 7)])
+@pytest.mark.parametrize("kind", ["spiral", "basic"])
+def test_orders_match_oracle(w, h, kind):
+    assert emp.order(w, h, kind) == o.order(w, h, o.SPIRAL if kind == "spiral" else o.BASIC)
+
+
+def test_situation_string_matches_reference_format(golden_dir):
+    w, h, c = emp.parse_puzzle(os.path.join(golden_dir, "puzzles", "5_5.txt"))
+    r = o.enumerate_(w, h, c)
+    assert emp.situation_string(w, h, r["boards"][0]) == \
+        " 1N15E 3N10E21S20S18W17E 7S 4W 2E 9E 8S14N 6S23W13E19E12W16S 5E22N11E24N25N"
+    empty = np.full(9, 0xFFFF, dtype=np.uint16)
+    assert emp.situation_string(3, 3, empty) == "-" * 18
+
+
+def test_parse_puzzle_matches_oracle_parser(golden_dir):
+    for n in ("3_3", "4_4", "5_5"):
+        p = os.path.join(golden_dir, "puzzles", n + ".txt")
+        a, b = emp.parse_puzzle(p), o.parse_puzzle(p)
+        assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2])
+
+
+def test_puzzle_validation(golden_dir):
+    """Refusals of include/emp_piece_db.h:898-924 and kind counts."""
+    w, h, c = emp.parse_puzzle(os.path.join(golden_dir, "puzzles", "3_3.txt"))
+    emp.validate(w, h, c)
+    bad = c.copy()
+    bad[4, 0] = 9                      # colour 9 on one edge only
+    with pytest.raises(emp.EmpError) as e:
+        emp.validate(w, h, bad)
+    assert e.value.status == emp.BAD_PUZZLE and "multiple of 2" in str(e.value)
+    bad = c.copy()
+    bad[4] = [0, 0, 0, 1]              # three frame sides
+    with pytest.raises(emp.EmpError):
+        emp.validate(w, h, bad)
+    with pytest.raises(emp.EmpError):
+        emp.validate(4, 4, np.zeros((16, 4), np.uint32))
+
+
+def test_null_context_is_an_argument_error():
+    with pytest.raises(emp.EmpError) as e:
+        emp._check(emp.lib().emp_hip_enumerate(None, None, 0, None, None, None, None))
+    assert e.value.status == emp.BAD_ARGUMENT
