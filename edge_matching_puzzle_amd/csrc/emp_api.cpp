@@ -17,6 +17,7 @@
 struct emp_hip_context {
     emp::DeviceInfo dev;
     hipStream_t stream = nullptr;
+    emp::Workspace workspace;
 };
 
 namespace {
@@ -35,6 +36,8 @@ int guarded(F &&f) {
         return f();
     } catch (const std::invalid_argument &e) {
         return fail(EMP_HIP_BAD_PUZZLE, e.what());
+    } catch (const std::domain_error &e) {
+        return fail(EMP_HIP_UNSUPPORTED, e.what());
     } catch (const emp::OutOfMemory &e) {
         return fail(EMP_HIP_OUT_OF_MEMORY, e.what());
     } catch (const emp::Unsupported &e) {
@@ -84,6 +87,8 @@ int emp_hip_destroy(emp_hip_context *context) {
     return guarded([&] {
         if (!context) return EMP_HIP_OK;
         (void)hipSetDevice(context->dev.device);
+        if (context->stream) (void)hipStreamSynchronize(context->stream);
+        context->workspace.release();
         if (context->stream) (void)hipStreamDestroy(context->stream);
         delete context;
         return EMP_HIP_OK;
@@ -130,6 +135,9 @@ int emp_hip_order(uint32_t width, uint32_t height, uint32_t order, uint32_t *xy)
         if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC)
             return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
         auto o = emp::make_order(width, height, (int)order);
+        if (!emp::is_permutation(o, width, height))
+            return fail(EMP_HIP_UNSUPPORTED, "the spiral order is not defined for a " + std::to_string(width) + "x" +
+                                                 std::to_string(height) + " board (the walk revisits cells)");
         for (size_t i = 0; i < o.size(); ++i) {
             xy[2 * i] = o[i].first;
             xy[2 * i + 1] = o[i].second;
@@ -192,7 +200,7 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
         cfg.solution_capacity = opt.max_solutions;
         std::vector<emp::EnumOutcome> out;
         emp::EnumStats st;
-        emp::enumerate_on_device(context->dev, context->stream, list, cfg, out, st);
+        emp::enumerate_on_device(context->dev, context->stream, context->workspace, list, cfg, out, st);
         for (uint32_t i = 0; i < n_puzzles; ++i) {
             emp_hip_enum_result &r = results[i];
             const uint32_t P = list[i]->size();

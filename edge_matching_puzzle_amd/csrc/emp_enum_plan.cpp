@@ -21,6 +21,9 @@ EnumTable build_enum_table(const Puzzle &pz, int order) {
     h.H = pz.height();
     h.ncol = pz.max_colour() + 1;
     t.order = make_order(pz.width(), pz.height(), order);
+    if (!is_permutation(t.order, pz.width(), pz.height()))
+        throw std::domain_error("the spiral order is not defined for a " + std::to_string(pz.width()) + "x" +
+                                std::to_string(pz.height()) + " board (the walk revisits cells when height >= width + 2)");
     uint32_t n_kind[3];
     for (int k = 0; k < 3; ++k) {
         n_kind[k] = (uint32_t)pz.pieces_of(k).size();

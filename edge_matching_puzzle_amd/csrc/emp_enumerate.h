@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,6 +22,29 @@ struct OutOfMemory : std::runtime_error {
 };
 struct Unsupported : std::runtime_error {
     using std::runtime_error::runtime_error;
+};
+
+// Grow-only cache of named device allocations, owned by a context so that
+// repeated searches allocate nothing once warm.
+class Workspace {
+  public:
+    Workspace() = default;
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    ~Workspace();
+    template <typename T>
+    T *get(const std::string &name, size_t count) {
+        return static_cast<T *>(raw(name, (count ? count : 1) * sizeof(T)));
+    }
+    void release();
+
+  private:
+    void *raw(const std::string &name, size_t bytes);
+    struct Block {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Block> blocks_;
 };
 
 struct DeviceInfo {
@@ -56,7 +80,7 @@ struct EnumStats {
     uint32_t max_words = 0, lds_tables = 0, blocks = 0, threads = 0;
 };
 
-void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
+void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
                          std::vector<EnumOutcome> &out, EnumStats &stats);
 

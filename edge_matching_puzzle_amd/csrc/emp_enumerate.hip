@@ -429,31 +429,6 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
             throw DeviceError(std::string(#call) + ": " + hipGetErrorString(err_));       \
     } while (0)
 
-template <typename T>
-struct DeviceBuffer {
-    T *ptr = nullptr;
-    size_t count = 0;
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer &) = delete;
-    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
-    ~DeviceBuffer() { release(); }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        count = 0;
-    }
-    void allocate(size_t n) {
-        release();
-        if (n == 0) n = 1;
-        hipError_t e = hipMalloc(&ptr, n * sizeof(T));
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            throw OutOfMemory("hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
-        }
-        count = n;
-    }
-};
-
 // Fill kernel used instead of hipMemsetAsync so passes can be graph-captured.
 __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -504,7 +479,31 @@ int round_words(uint32_t w) {
 
 }  // namespace
 
-void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
+Workspace::~Workspace() { release(); }
+
+void Workspace::release() {
+    for (auto &kv : blocks_)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    blocks_.clear();
+}
+
+void *Workspace::raw(const std::string &name, size_t bytes) {
+    Block &b = blocks_[name];
+    if (b.ptr && b.bytes >= bytes) return b.ptr;
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+        b.ptr = nullptr;
+        (void)hipGetLastError();
+        throw OutOfMemory("hipMalloc of " + std::to_string(want) + " bytes failed");
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
+void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
                          std::vector<EnumOutcome> &out, EnumStats &stats) {
     const uint32_t n = (uint32_t)puzzles.size();
@@ -552,34 +551,26 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
     stats.threads = threads;
 
     // Device buffers.
-    DeviceBuffer<uint64_t> d_tables;
-    DeviceBuffer<uint32_t> d_table_off;
-    d_tables.allocate(total_words);
-    d_table_off.allocate(n);
-    EMP_CHECK(hipMemcpyAsync(d_tables.ptr, blob.data(), blob.size() * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_table_off.ptr, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
+    uint64_t *d_tables = ws.get<uint64_t>("tables", total_words);
+    uint32_t *d_table_off = ws.get<uint32_t>("table_off", n);
+    EMP_CHECK(hipMemcpyAsync(d_tables, blob.data(), blob.size() * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_table_off, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
     const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 8, (uint64_t)n * 4, 1u << 16});
     const uint32_t cap = (uint32_t)std::min<uint64_t>(cap64, 1u << 26);
-    DeviceBuffer<uint64_t> d_hdr[2];
-    DeviceBuffer<uint16_t> d_stack[2];
-    for (int q = 0; q < 2; ++q) {
-        d_hdr[q].allocate(cap);
-        d_stack[q].allocate((size_t)cap * stride);
-    }
-    DeviceBuffer<unsigned long long> d_nodes, d_sols, d_first, d_last, d_limit;
-    DeviceBuffer<unsigned int> d_trunc, d_counters;
-    d_nodes.allocate(n);
-    d_sols.allocate(n);
-    d_first.allocate(n);
-    d_last.allocate(n);
-    d_limit.allocate(n);
-    d_trunc.allocate(n);
-    d_counters.allocate(8);   // [0] in_head [1] out_count [2..4] flags
+    uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
+    uint16_t *d_stack[2] = {ws.get<uint16_t>("stack0", (size_t)cap * stride),
+                            ws.get<uint16_t>("stack1", (size_t)cap * stride)};
+    unsigned long long *d_nodes = ws.get<unsigned long long>("nodes", n);
+    unsigned long long *d_sols = ws.get<unsigned long long>("sols", n);
+    unsigned long long *d_first = ws.get<unsigned long long>("first", n);
+    unsigned long long *d_last = ws.get<unsigned long long>("last", n);
+    unsigned long long *d_limit = ws.get<unsigned long long>("limit", n);
+    unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
+    unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
     const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
-    DeviceBuffer<uint16_t> d_sol_stack;
-    d_sol_stack.allocate((size_t)n * std::max(sol_per_puzzle, 1u) * stride);
+    uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
 
     // Root items: index 0 holds its first candidate (one node each).
     std::vector<uint64_t> h_hdr(n);
@@ -589,35 +580,35 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
         h_hdr[i] = (uint64_t)i | (1ull << 32) | (0ull << 48);
         h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
     }
-    EMP_CHECK(hipMemcpyAsync(d_hdr[0].ptr, h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_stack[0].ptr, h_stack.data(), h_stack.size() * 2, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_nodes.ptr, h_nodes.data(), n * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_sols.ptr, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_first.ptr, h_first.data(), n * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_last.ptr, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_limit.ptr, h_limit.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_hdr[0], h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_stack[0], h_stack.data(), h_stack.size() * 2, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_nodes, h_nodes.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_sols, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_first, h_first.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_last, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_limit, h_limit.data(), n * 8, hipMemcpyHostToDevice, stream));
     std::vector<unsigned int> h_zero32(std::max<uint32_t>(n, 8), 0);
-    EMP_CHECK(hipMemcpyAsync(d_trunc.ptr, h_zero32.data(), n * 4, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_counters.ptr, h_zero32.data(), 8 * 4, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_trunc, h_zero32.data(), n * 4, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_counters, h_zero32.data(), 8 * 4, hipMemcpyHostToDevice, stream));
 
     DfsArgs a{};
-    a.tables = d_tables.ptr;
-    a.table_off = d_table_off.ptr;
+    a.tables = d_tables;
+    a.table_off = d_table_off;
     a.lds_table_words = lds_table ? max_table : 0;
     a.avail_words = max_avail;
     a.stride = stride;
     a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 32;
     a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 18);
-    a.nodes = d_nodes.ptr;
-    a.solutions = d_sols.ptr;
-    a.first_tick = d_first.ptr;
-    a.last_tick = d_last.ptr;
-    a.node_limit = d_limit.ptr;
-    a.truncated = d_trunc.ptr;
-    a.in_head = d_counters.ptr + 0;
-    a.out_count = d_counters.ptr + 1;
-    a.flags = d_counters.ptr + 2;
-    a.sol_stack = d_sol_stack.ptr;
+    a.nodes = d_nodes;
+    a.solutions = d_sols;
+    a.first_tick = d_first;
+    a.last_tick = d_last;
+    a.node_limit = d_limit;
+    a.truncated = d_trunc;
+    a.in_head = d_counters + 0;
+    a.out_count = d_counters + 1;
+    a.flags = d_counters + 2;
+    a.sol_stack = d_sol_stack;
     a.sol_per_puzzle = sol_per_puzzle;
 
     hipEvent_t ev0, ev1;
@@ -630,29 +621,27 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
 
     // Counters saved before each expansion so that an expansion whose
     // children do not fit can be undone and its items searched instead.
-    DeviceBuffer<unsigned long long> d_nodes_save, d_sols_save;
-    DeviceBuffer<unsigned int> d_counters_save;
-    d_nodes_save.allocate(n);
-    d_sols_save.allocate(n);
-    d_counters_save.allocate(8);
+    unsigned long long *d_nodes_save = ws.get<unsigned long long>("nodes_save", n);
+    unsigned long long *d_sols_save = ws.get<unsigned long long>("sols_save", n);
+    unsigned int *d_counters_save = ws.get<unsigned int>("counters_save", 8);
 
     int cur = 0;
     uint64_t n_items = n;
     uint32_t expansions = 0;
     const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 64;
     while (n_items) {
-        a.in_hdr = d_hdr[cur].ptr;
-        a.in_stack = d_stack[cur].ptr;
+        a.in_hdr = d_hdr[cur];
+        a.in_stack = d_stack[cur];
         a.n_in = (uint32_t)n_items;
-        a.out_hdr = d_hdr[1 - cur].ptr;
-        a.out_stack = d_stack[1 - cur].ptr;
+        a.out_hdr = d_hdr[1 - cur];
+        a.out_stack = d_stack[1 - cur];
         a.out_cap = cap;
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters.ptr, 5u, 0u);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 5u, 0u);
         const bool expand = n_items < target && expansions < max_expansions;
         if (expand) {
-            EMP_CHECK(hipMemcpyAsync(d_nodes_save.ptr, d_nodes.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
-            EMP_CHECK(hipMemcpyAsync(d_sols_save.ptr, d_sols.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
-            EMP_CHECK(hipMemcpyAsync(d_counters_save.ptr, d_counters.ptr, 8 * 4, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_counters_save, d_counters, 8 * 4, hipMemcpyDeviceToDevice, stream));
         }
         EMP_CHECK(hipEventRecord(ev0, stream));
         if (expand) dispatch_expand(maxw, a, stream);
@@ -660,7 +649,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
         EMP_CHECK(hipGetLastError());
         EMP_CHECK(hipEventRecord(ev1, stream));
         unsigned int counters[8];
-        EMP_CHECK(hipMemcpyAsync(counters, d_counters.ptr, sizeof(counters), hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
         EMP_CHECK(hipStreamSynchronize(stream));
         float ms = 0;
         EMP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -671,9 +660,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
             if (counters[2]) {
                 // Children did not fit: undo the expansion's counts (the
                 // input items are intact) and search the items instead.
-                EMP_CHECK(hipMemcpyAsync(d_nodes.ptr, d_nodes_save.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
-                EMP_CHECK(hipMemcpyAsync(d_sols.ptr, d_sols_save.ptr, n * 8, hipMemcpyDeviceToDevice, stream));
-                EMP_CHECK(hipMemcpyAsync(d_counters.ptr, d_counters_save.ptr, 8 * 4, hipMemcpyDeviceToDevice, stream));
+                EMP_CHECK(hipMemcpyAsync(d_nodes, d_nodes_save, n * 8, hipMemcpyDeviceToDevice, stream));
+                EMP_CHECK(hipMemcpyAsync(d_sols, d_sols_save, n * 8, hipMemcpyDeviceToDevice, stream));
+                EMP_CHECK(hipMemcpyAsync(d_counters, d_counters_save, 8 * 4, hipMemcpyDeviceToDevice, stream));
                 expansions = max_expansions;
                 continue;
             }
@@ -690,15 +679,15 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream,
     // Results.
     std::vector<unsigned long long> r_nodes(n), r_sols(n), r_first(n), r_last(n);
     std::vector<unsigned int> r_trunc(n);
-    EMP_CHECK(hipMemcpyAsync(r_nodes.data(), d_nodes.ptr, n * 8, hipMemcpyDeviceToHost, stream));
-    EMP_CHECK(hipMemcpyAsync(r_sols.data(), d_sols.ptr, n * 8, hipMemcpyDeviceToHost, stream));
-    EMP_CHECK(hipMemcpyAsync(r_first.data(), d_first.ptr, n * 8, hipMemcpyDeviceToHost, stream));
-    EMP_CHECK(hipMemcpyAsync(r_last.data(), d_last.ptr, n * 8, hipMemcpyDeviceToHost, stream));
-    EMP_CHECK(hipMemcpyAsync(r_trunc.data(), d_trunc.ptr, n * 4, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_nodes.data(), d_nodes, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_sols.data(), d_sols, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_first.data(), d_first, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_last.data(), d_last, n * 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(r_trunc.data(), d_trunc, n * 4, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipStreamSynchronize(stream));
     std::vector<uint16_t> r_stack((size_t)n * sol_per_puzzle * stride);
     if (!r_stack.empty()) {
-        EMP_CHECK(hipMemcpyAsync(r_stack.data(), d_sol_stack.ptr, r_stack.size() * 2, hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipMemcpyAsync(r_stack.data(), d_sol_stack, r_stack.size() * 2, hipMemcpyDeviceToHost, stream));
         EMP_CHECK(hipStreamSynchronize(stream));
     }
     const double tick_ms = 1e-5;   // wall_clock64 runs at 100 MHz on gfx9

@@ -93,6 +93,18 @@ std::vector<std::pair<uint32_t, uint32_t>> make_order(uint32_t width, uint32_t h
     return out;
 }
 
+bool is_permutation(const std::vector<std::pair<uint32_t, uint32_t>> &order, uint32_t width, uint32_t height) {
+    if (order.size() != (size_t)width * height) return false;
+    std::vector<uint8_t> seen(width * height, 0);
+    for (const auto &xy : order) {
+        if (xy.first >= width || xy.second >= height) return false;
+        uint8_t &s = seen[xy.second * width + xy.first];
+        if (s) return false;
+        s = 1;
+    }
+    return true;
+}
+
 std::string situation_string(uint32_t nb_cells, const uint16_t *board) {
     const unsigned w = (unsigned)std::to_string(nb_cells).size() + 1;
     static const char kSide[4] = {'N', 'E', 'S', 'W'};

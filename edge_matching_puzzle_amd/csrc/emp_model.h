@@ -63,6 +63,10 @@ class Puzzle {
 // Strategy order: element i is the (x, y) of the i-th visited cell.
 std::vector<std::pair<uint32_t, uint32_t>> make_order(uint32_t width, uint32_t height, int order);
 
+// True when the order visits every cell exactly once.  The reference's spiral
+// walk does not when height >= width + 2 (it revisits cells).
+bool is_permutation(const std::vector<std::pair<uint32_t, uint32_t>> &order, uint32_t width, uint32_t height);
+
 // Row-major board cell value: ((id - 1) << 2) | orientation, 0xFFFF empty.
 std::string situation_string(uint32_t nb_cells, const uint16_t *board);
 

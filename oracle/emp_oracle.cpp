@@ -387,6 +387,19 @@ struct Enumerator {
     }
 };
 
+// The reference's spiral walk revisits cells and never reaches the last rows
+// when height >= width + 2; such an order is refused.
+bool is_permutation(const Model &m, int order) {
+    auto o = make_order(m.W, m.H, order);
+    std::vector<char> seen(m.P, 0);
+    for (auto &xy : o) {
+        uint32_t cell = xy.second * m.W + xy.first;
+        if (seen[cell]) return false;
+        seen[cell] = 1;
+    }
+    return true;
+}
+
 int copy_order(const Model &m, int order, uint32_t *xy) {
     auto o = make_order(m.W, m.H, order);
     for (size_t i = 0; i < o.size(); ++i) { xy[2 * i] = o[i].first; xy[2 * i + 1] = o[i].second; }
@@ -763,6 +776,7 @@ int oracle_enumerate(uint32_t width, uint32_t height, const uint32_t *colours,
     Model m;
     if (order != ORACLE_ORDER_SPIRAL && order != ORACLE_ORDER_BASIC) return ORACLE_BAD_ARGUMENT;
     if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order)) return ORACLE_BAD_ARGUMENT;
     Enumerator e(m, order);
     Enumerator::Result r;
     e.run(r, node_limit, boards ? max_solutions : 0, trace != nullptr);
@@ -789,6 +803,7 @@ int oracle_results_bin(uint32_t width, uint32_t height, const uint32_t *colours,
                        uint64_t *size) {
     Model m;
     if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order)) return ORACLE_BAD_ARGUMENT;
     Enumerator e(m, order);
     Enumerator::Result r;
     e.run(r, 0, ~0ull, false);

@@ -75,14 +75,23 @@ def test_forced_splits_and_tiny_budget(gpu_context):
     _check_same(r, o.enumerate_(w, h, c))
 
 
-@pytest.mark.parametrize("w,h,B,I,seed", [(3, 4, 2, 2, 2), (4, 6, 2, 3, 1), (6, 4, 3, 3, 5), (4, 4, 2, 2, 1),
-                                          (4, 4, 1, 2, 3), (5, 5, 2, 2, 4)])
-def test_identical_and_self_similar_pieces(gpu_context, w, h, B, I, seed):
+@pytest.mark.parametrize("w,h,B,I,seed,order", [(3, 4, 2, 2, 2, "spiral"), (4, 6, 2, 3, 1, "basic"),
+                                                (6, 4, 3, 3, 5, "spiral"), (4, 4, 2, 2, 1, "spiral"),
+                                                (4, 4, 1, 2, 3, "spiral"), (5, 5, 2, 2, 4, "spiral"),
+                                                (5, 5, 2, 2, 4, "basic")])
+def test_identical_and_self_similar_pieces(gpu_context, w, h, B, I, seed, order):
     """Few colours: many identical and half-similar pieces, rectangles too."""
     c, _ = emp.generate(seed, w, h, B, I)
-    print("case: identical", w, h, B, I, seed, flush=True)
-    (r,), _ = gpu_context.enumerate([(w, h, c)], max_solutions=1 << 14)
-    _check_same(r, o.enumerate_(w, h, c, max_solutions=1 << 14))
+    print("case: identical", w, h, B, I, seed, order, flush=True)
+    (r,), _ = gpu_context.enumerate([(w, h, c)], max_solutions=1 << 14, order=order)
+    _check_same(r, o.enumerate_(w, h, c, max_solutions=1 << 14, order=o.SPIRAL if order == "spiral" else o.BASIC))
+
+
+def test_tall_board_spiral_is_refused(gpu_context):
+    c, _ = emp.generate(1, 4, 6, 2, 3)
+    with pytest.raises(emp.EmpError) as e:
+        gpu_context.enumerate([(4, 6, c)])
+    assert e.value.status == emp.UNSUPPORTED
 
 
 def test_solution_capacity_and_counts(gpu_context):
