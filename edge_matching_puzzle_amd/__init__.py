@@ -35,7 +35,7 @@ class EnumOptions(ctypes.Structure):
                 ("waves_per_block", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
                 ("step_budget", ctypes.c_uint64), ("target_items", ctypes.c_uint64),
                 ("node_limit", ctypes.c_uint64), ("max_solutions", ctypes.c_uint64),
-                ("use_graph", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("use_graph", ctypes.c_uint32), ("queue_items", ctypes.c_uint32)]
 
 
 class EnumResult(ctypes.Structure):
@@ -171,7 +171,7 @@ class Context:
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
-                  node_limit=0):
+                  node_limit=0, queue_items=0):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         Returns (results, stats): one dict per puzzle with n_nodes, n_solutions,
@@ -194,6 +194,7 @@ class Context:
         opt.waves_per_cu, opt.waves_per_block = waves_per_cu, waves_per_block
         opt.steps_per_round, opt.step_budget = steps_per_round, step_budget
         opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
+        opt.queue_items = queue_items
         res = (EnumResult * n)()
         st = EnumStats()
         _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),

@@ -196,6 +196,7 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
         cfg.step_budget = opt.step_budget;
         cfg.target_items = opt.target_items;
         cfg.node_limit = opt.node_limit;
+        cfg.queue_items = opt.queue_items;
         // Device-side store of up to max_solutions boards per puzzle.
         cfg.solution_capacity = opt.max_solutions;
         std::vector<emp::EnumOutcome> out;

@@ -60,10 +60,12 @@ struct EnumConfig {
     uint32_t waves_per_block = 0;
     uint32_t steps_per_round = 0;
     uint64_t step_budget = 0;
+    uint64_t drain_budget = 0;
     uint64_t target_items = 0;
     uint64_t node_limit = 0;
     uint64_t solution_capacity = 1 << 16;
     uint32_t max_expansions = 0;
+    uint32_t queue_items = 0;       // capacity of each work queue, 0 = sized from the grid
 };
 
 struct EnumOutcome {

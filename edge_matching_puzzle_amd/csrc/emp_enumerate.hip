@@ -48,7 +48,8 @@ struct DfsArgs {
     uint32_t avail_words;          // max avail words over puzzles (per lane)
     uint32_t stride;               // u16 per item stack (max P, even)
     uint32_t steps_per_round;
-    uint64_t step_budget;
+    uint64_t step_budget;          // steps before a lane splits its item
+    uint64_t drain_budget;         // same, once the input queue is exhausted
     // input queue
     const uint64_t *in_hdr;        // puzzle | kmin << 32 | (last + 1) << 48
     const uint16_t *in_stack;      // [n_in][stride]
@@ -59,7 +60,7 @@ struct DfsArgs {
     uint16_t *out_stack;
     uint32_t out_cap;
     unsigned int *out_count;
-    unsigned int *flags;           // [0] output overflow, [1] splits, [2] items
+    unsigned int *flags;           // [0] output overflow, [1] splits, [2] items, [3] drain
     // per puzzle
     unsigned long long *nodes;
     unsigned long long *solutions;
@@ -254,8 +255,9 @@ __global__ void expand_kernel(DfsArgs a) {
         }
         return;
     }
+    if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const uint32_t base = atomicAdd(a.out_count, n);
-    if (base + n > a.out_cap) {
+    if ((uint64_t)base + n > a.out_cap) {
         atomicOr(&a.flags[0], 1u);
         return;
     }
@@ -297,6 +299,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
 
     bool active = false;
     bool queue_empty = false;
+    bool may_split = false;   // at most one failed reservation per item
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0;
     int last = -1;
     uint64_t steps = 0, nodes = 0;
@@ -318,7 +321,12 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(a.in_head, n_idle);
             base = __shfl(base, leader);
-            if (base + n_idle >= a.n_in) queue_empty = true;
+            if (base + n_idle >= a.n_in) {
+                queue_empty = true;
+                // Every item is taken: from now on busy lanes split early so
+                // the pass ends soon and the rest is shared out next pass.
+                if (lane == leader) atomicOr(&a.flags[3], 1u);
+            }
             if (!active) {
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
                 const uint32_t idx = base + rank;
@@ -338,6 +346,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                         k = kmin;
                         steps = 0;
                         nodes = 0;
+                        may_split = true;
                         const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
                         for (int K = 0; K < 3; ++K)
                             for (uint32_t w = 0, nw = tv.words(K); w < nw; ++w) A(K, (int)w) = tv.b[tv.off_valid(K) + w];
@@ -361,6 +370,8 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
         }
         if (!active) continue;
 
+        const bool drain = __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        const uint64_t budget = drain ? a.drain_budget : a.step_budget;
         for (uint32_t r = 0; r < a.steps_per_round; ++r) {
             uint64_t c[MAXW];
             int K;
@@ -397,12 +408,16 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                     last = -1;
                 }
             }
-            if (steps >= a.step_budget) {
+            if (steps >= budget && may_split) {
                 // Split: what remains is, for each index j in [kmin, k], the
                 // candidates above the current choice at j (above `last` at k).
+                // Reservations stop once the output queue has overflowed, so
+                // out_count stays below out_cap + lanes * P (no wrap-around).
                 const uint32_t n = k - kmin + 1;
-                const uint32_t base = atomicAdd(a.out_count, n);
-                if (base + n <= a.out_cap) {
+                uint32_t base = 0xFFFFFFFFu;
+                if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    base = atomicAdd(a.out_count, n);
+                if (base != 0xFFFFFFFFu && (uint64_t)base + n <= a.out_cap) {
                     for (uint32_t j = kmin; j <= k; ++j) {
                         const uint32_t o = base + (j - kmin);
                         uint16_t *dst = a.out_stack + (size_t)o * a.stride;
@@ -413,10 +428,13 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                     finish();
                     break;
                 }
-                // No room: mark the reserved slots dead and keep searching.
-                for (uint32_t o = base; o < base + n && o < a.out_cap; ++o) a.out_hdr[o] = make_hdr(0, kDeadItem, -1);
+                // No room: mark what was reserved inside the queue dead and
+                // finish this item here without further split attempts.
+                if (base != 0xFFFFFFFFu)
+                    for (uint64_t o = base; o < (uint64_t)base + n && o < a.out_cap; ++o)
+                        a.out_hdr[o] = make_hdr(0, kDeadItem, -1);
                 atomicOr(&a.flags[0], 1u);
-                steps = 0;
+                may_split = false;
             }
         }
     }
@@ -557,8 +575,11 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     EMP_CHECK(hipMemcpyAsync(d_table_off, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
-    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 8, (uint64_t)n * 4, 1u << 16});
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(cap64, 1u << 26);
+    // Room for every lane to split once per pass (a split emits at most P
+    // items); beyond that lanes finish their items without splitting.
+    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * (max_P + 2), (uint64_t)n * 4, 1u << 16});
+    const uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
+                                         : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
     uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
     uint16_t *d_stack[2] = {ws.get<uint16_t>("stack0", (size_t)cap * stride),
                             ws.get<uint16_t>("stack1", (size_t)cap * stride)};
@@ -598,7 +619,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.avail_words = max_avail;
     a.stride = stride;
     a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 32;
-    a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 18);
+    a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 16);
+    a.drain_budget = std::min<uint64_t>(a.step_budget, cfg.drain_budget ? cfg.drain_budget : 512);
     a.nodes = d_nodes;
     a.solutions = d_sols;
     a.first_tick = d_first;
@@ -628,7 +650,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     int cur = 0;
     uint64_t n_items = n;
     uint32_t expansions = 0;
-    const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 64;
+    const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 1024;
     while (n_items) {
         a.in_hdr = d_hdr[cur];
         a.in_stack = d_stack[cur];
@@ -636,7 +658,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         a.out_hdr = d_hdr[1 - cur];
         a.out_stack = d_stack[1 - cur];
         a.out_cap = cap;
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 5u, 0u);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
         const bool expand = n_items < target && expansions < max_expansions;
         if (expand) {
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));

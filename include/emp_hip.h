@@ -108,7 +108,8 @@ typedef struct {
                                   this many nodes were counted, 0 = none      */
     uint64_t max_solutions;    /* capacity of each boards[i] in solutions       */
     uint32_t use_graph;        /* capture each pass in a hipGraph               */
-    uint32_t reserved;
+    uint32_t queue_items;      /* work queue capacity in items, 0 = sized from
+                                  the grid (a full queue only stops splitting) */
 } emp_hip_enum_options;
 
 typedef struct {

@@ -347,11 +347,34 @@ struct Enumerator {
         std::string trace;
     };
 
-    // emp_strategy::specific_run (include/emp_strategy.h:332-415).
-    void run(Result &r, uint64_t node_limit, uint64_t keep, bool want_trace) {
+    // Places the given transitions at indices 0..len-1 as the DFS would
+    // (emp_strategy::set_initial_state, include/emp_strategy.h:418-467);
+    // false when one of them is not a candidate.
+    bool preset(const uint16_t *prefix, uint32_t len) {
+        for (uint32_t i = 0; i < len; ++i) {
+            compute_available(i);
+            uint32_t t = prefix[i];
+            if (t >= 64 * nw || !(avail_trans[(size_t)i * nw + t / 64] >> (t % 64) & 1)) return false;
+            placed[i] = (int32_t)t;
+            select(i, t);
+        }
+        return true;
+    }
+
+    // emp_strategy::specific_run (include/emp_strategy.h:332-415).  With
+    // start = 0 this is the reference's loop: index 0 takes its first
+    // candidate and the search ends when index 1 has no transition left.
+    // With start = L >= 1 (indices below L preset) it enumerates the subtree
+    // of that prefix and ends when index L has no transition left.  With
+    // stop_depth = D > 0 placements at index D-1 are recorded as prefixes and
+    // not descended.
+    void run(Result &r, uint64_t node_limit, uint64_t keep, bool want_trace,
+             uint32_t start = 0, uint32_t stop_depth = 0,
+             std::vector<uint16_t> *prefixes = nullptr) {
         r.depth_nodes.assign(P, 0);
         r.depth_dead.assign(P, 0);
-        uint32_t index = 0, max = 0;
+        uint32_t index = start, max = 0;
+        const uint32_t floor = start ? start : 1;
         compute_available(index);
         bool go_on = true;
         while (go_on) {
@@ -373,6 +396,11 @@ struct Enumerator {
                         r.boards.push_back(board(index));
                         r.solution_nodes.push_back(r.nodes);
                     }
+                } else if (stop_depth && index + 1 == stop_depth) {
+                    if (prefixes)
+                        for (uint32_t j = 0; j <= index; ++j) prefixes->push_back((uint16_t)placed[j]);
+                    if (node_limit && r.nodes >= node_limit) { r.limited = true; return; }
+                    continue;
                 }
                 ++index;
                 compute_available(index);
@@ -380,7 +408,7 @@ struct Enumerator {
                 if (node_limit && r.nodes >= node_limit) { r.limited = true; return; }
             } else {
                 if (index < P) placed[index] = -1;
-                go_on = index > 1;
+                go_on = index > floor;
                 --index;
             }
         }
@@ -794,6 +822,42 @@ int oracle_enumerate(uint32_t width, uint32_t height, const uint32_t *colours,
         trace[n] = 0;
     }
     return r.limited ? ORACLE_LIMIT_REACHED : ORACLE_OK;
+}
+
+int oracle_enumerate_prefixes(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                              uint32_t depth, uint16_t *prefixes, uint64_t capacity,
+                              uint64_t *n_prefixes, uint64_t *n_nodes, uint64_t *n_solutions) {
+    Model m;
+    if (order != ORACLE_ORDER_SPIRAL && order != ORACLE_ORDER_BASIC) return ORACLE_BAD_ARGUMENT;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order) || depth < 2 || depth >= m.P) return ORACLE_BAD_ARGUMENT;
+    Enumerator e(m, order);
+    Enumerator::Result r;
+    std::vector<uint16_t> pre;
+    e.run(r, 0, 0, false, 0, depth, &pre);
+    uint64_t n = pre.size() / depth;
+    if (n_prefixes) *n_prefixes = n;
+    if (n_nodes) *n_nodes = r.nodes;
+    if (n_solutions) *n_solutions = r.solutions;
+    if (prefixes) std::copy(pre.begin(), pre.begin() + std::min<uint64_t>(n, capacity) * depth, prefixes);
+    return n > capacity ? ORACLE_LIMIT_REACHED : ORACLE_OK;
+}
+
+int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                             uint32_t prefix_len, const uint16_t *prefix, uint64_t max_solutions,
+                             uint16_t *boards, uint64_t *n_solutions, uint64_t *n_nodes) {
+    Model m;
+    if (order != ORACLE_ORDER_SPIRAL && order != ORACLE_ORDER_BASIC) return ORACLE_BAD_ARGUMENT;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order) || prefix_len < 1 || prefix_len >= m.P || !prefix) return ORACLE_BAD_ARGUMENT;
+    Enumerator e(m, order);
+    if (!e.preset(prefix, prefix_len)) return ORACLE_BAD_ARGUMENT;
+    Enumerator::Result r;
+    e.run(r, 0, boards ? max_solutions : 0, false, prefix_len);
+    for (size_t s = 0; s < r.boards.size(); ++s) std::copy(r.boards[s].begin(), r.boards[s].end(), boards + s * m.P);
+    if (n_solutions) *n_solutions = r.solutions;
+    if (n_nodes) *n_nodes = r.nodes;
+    return ORACLE_OK;
 }
 
 // results.bin v1 (include/emp_situation_binary_dumper.h:72-126, records

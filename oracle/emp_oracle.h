@@ -58,6 +58,26 @@ int oracle_enumerate(uint32_t width, uint32_t height, const uint32_t *colours,
                      char *trace, uint64_t trace_capacity);
 
 /*
+ * The same DFS cut at a depth: every placement at strategy index depth-1 is
+ * recorded as a prefix (depth transitions, the oracle's kind_index << 2 | o
+ * numbering) and not descended.  n_nodes / n_solutions count the placements
+ * and solutions above the cut (solutions only when depth reaches P).
+ */
+int oracle_enumerate_prefixes(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                              uint32_t depth, uint16_t *prefixes, uint64_t capacity,
+                              uint64_t *n_prefixes, uint64_t *n_nodes, uint64_t *n_solutions);
+
+/*
+ * The DFS below one prefix (transitions at indices 0..prefix_len-1, as
+ * returned by oracle_enumerate_prefixes); ends when index prefix_len has no
+ * transition left.  n_nodes excludes the prefix's own placements.  The union
+ * over all prefixes of one depth is the whole enumeration.
+ */
+int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                             uint32_t prefix_len, const uint16_t *prefix, uint64_t max_solutions,
+                             uint16_t *boards, uint64_t *n_solutions, uint64_t *n_nodes);
+
+/*
  * results.bin v1 image of an enumeration (include/emp_situation_binary_dumper.h
  * :72-126): header, strategy positions, one record per solution, total nodes.
  * Writes at most capacity bytes into out and returns the full size in *size.

@@ -30,6 +30,10 @@ def lib():
         _lib.oracle_results_bin.restype = i32
         _lib.oracle_order.argtypes = [u32, u32, i32, p]
         _lib.oracle_order.restype = i32
+        _lib.oracle_enumerate_prefixes.argtypes = [u32, u32, p, i32, u32, p, u64, p, p, p]
+        _lib.oracle_enumerate_prefixes.restype = i32
+        _lib.oracle_enumerate_subtree.argtypes = [u32, u32, p, i32, u32, p, u64, p, p, p]
+        _lib.oracle_enumerate_subtree.restype = i32
         _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
         _lib.oracle_glutton_max.restype = i32
     return _lib
@@ -74,6 +78,32 @@ def enumerate_(w, h, colours, order=SPIRAL, node_limit=0, max_solutions=1 << 16,
         "depth_nodes": dn, "depth_dead": dd,
         "trace": buf.value.decode() if trace else None,
     }
+
+
+def prefixes(w, h, colours, depth, order=SPIRAL, capacity=1 << 22):
+    """Placements at index depth-1 of the DFS; returns (prefixes [n, depth],
+    nodes above the cut, solutions above the cut)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    out = np.zeros((capacity, depth), dtype=np.uint16)
+    n, nodes, sols = (np.zeros(1, np.uint64) for _ in range(3))
+    st = lib().oracle_enumerate_prefixes(w, h, _ptr(colours), order, depth, _ptr(out), capacity,
+                                         _ptr(n), _ptr(nodes), _ptr(sols))
+    if st != OK:
+        raise ValueError("oracle_enumerate_prefixes status %d" % st)
+    return out[:int(n[0])].copy(), int(nodes[0]), int(sols[0])
+
+
+def subtree(w, h, colours, prefix, order=SPIRAL, max_solutions=1 << 12):
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    prefix = np.ascontiguousarray(prefix, dtype=np.uint16)
+    boards = np.zeros((max_solutions, w * h), dtype=np.uint16)
+    sols, nodes = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+    st = lib().oracle_enumerate_subtree(w, h, _ptr(colours), order, prefix.size, _ptr(prefix),
+                                        max_solutions, _ptr(boards), _ptr(sols), _ptr(nodes))
+    if st != OK:
+        raise ValueError("oracle_enumerate_subtree status %d" % st)
+    n = int(sols[0])
+    return {"n_nodes": int(nodes[0]), "n_solutions": n, "boards": boards[:min(n, max_solutions)].copy()}
 
 
 def results_bin(w, h, colours, order=SPIRAL):

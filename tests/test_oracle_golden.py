@@ -126,3 +126,22 @@ def test_node_limit_reports_partial():
     w, h, c = o.parse_puzzle(os.path.join(os.path.dirname(__file__), "golden", "puzzles", "5_5.txt"))
     r = o.enumerate_(w, h, c, node_limit=1000)
     assert r["status"] == o.LIMIT_REACHED and r["n_nodes"] == 1000
+
+
+@pytest.mark.parametrize("w,B,I,seed,depth", [(5, 3, 4, 1, 6), (5, 2, 2, 4, 4), (6, 3, 6, 1, 8)])
+def test_prefix_subtrees_union_is_the_enumeration(w, B, I, seed, depth):
+    """The cut used to build the 8x8 golden in parallel: nodes above the cut
+    plus every prefix subtree equal the uncut enumeration, solution for
+    solution."""
+    import edge_matching_puzzle_amd as emp
+    c, _ = emp.generate(seed, w, w, B, I)
+    full = o.enumerate_(w, w, c, max_solutions=1 << 18)
+    pre, nodes, sols = o.prefixes(w, w, c, depth)
+    boards = []
+    for p in pre:
+        r = o.subtree(w, w, c, p, max_solutions=1 << 18)
+        nodes += r["n_nodes"]
+        sols += r["n_solutions"]
+        boards += [tuple(b) for b in r["boards"].tolist()]
+    assert nodes == full["n_nodes"] and sols == full["n_solutions"]
+    assert sorted(boards) == sorted(tuple(b) for b in full["boards"].tolist())
