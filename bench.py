@@ -1,7 +1,7 @@
 """Benchmark of the HIP search path on one MI355X.
 
 Workload (one "step"): full exhaustive enumeration of one 8x8 synthetic
-puzzle (64 pieces, 4 border colours, 8 interior colours, generator seed 1),
+puzzle (64 pieces, 4 border colours, 8 interior colours, generator seed 7),
 corner kind-index 0 fixed at (0,0), spiral order -- the plain backtracker
 configuration the metric is quoted on.  The puzzle tables are built and
 uploaded once before timing; each step runs the whole enumeration on the GPU.
@@ -32,7 +32,7 @@ LDS_PEAK_GBS = 256 * 128 * 2.4          # GB/s: 256 CUs x 128 B/clk x 2.4 GHz
 VALU_PEAK_GOPS = 256 * 4 * 16 * 2.4     # 32-bit integer lane-ops/ns: 4 SIMD x 16 lanes per CU
 
 
-def puzzle_c1(seed=1):
+def puzzle_c1(seed=7):
     import edge_matching_puzzle_amd as emp
     colours, _ = emp.generate(seed, 8, 8, 4, 8)
     return 8, 8, colours
@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-nodes", type=int, default=600_000_000,
                     help="nodes of the CPU baseline sample (about 15 s on one core)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -89,8 +89,9 @@ def main():
     import edge_matching_puzzle_amd as emp
     w, h, colours = puzzle_c1(args.seed)
     ctx = emp.Context(0)
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         ctx.enumerate([(w, h, colours)], max_solutions=64)
+        print("warmup step %d done" % i, file=sys.stderr, flush=True)
     nodes = 0
     kernel_ms = 0.0
     launches = 0
@@ -102,6 +103,7 @@ def main():
         kernel_ms += st["dfs_kernel_ms"]
         launches += st["dfs_launches"]
         results.append((r, st))
+        print("step done: %d nodes, %.1f ms kernel" % (r["n_nodes"], st["dfs_kernel_ms"]), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     r, st = results[-1]
     if not r["complete"]:

@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -74,13 +76,20 @@ struct DfsArgs {
     uint32_t sol_per_puzzle;
 };
 
-__device__ __forceinline__ uint64_t make_hdr(uint32_t puzzle, uint32_t kmin, int last) {
-    return (uint64_t)puzzle | ((uint64_t)kmin << 32) | ((uint64_t)(uint32_t)(last + 1) << 48);
+// Item header: puzzle (bits 0-19) | floor (20-30) | level (31-41) |
+// last + 1 (42-53).  The item's stack row holds the transitions of indices
+// 0..level-1; the search resumes at index `level` above transition `last`
+// and ends when index `floor` has no candidate left.  A fresh subtree has
+// level == floor.
+__device__ __forceinline__ uint64_t make_hdr(uint32_t puzzle, uint32_t floor, uint32_t level, int last) {
+    return (uint64_t)puzzle | ((uint64_t)floor << 20) | ((uint64_t)level << 31) |
+           ((uint64_t)(uint32_t)(last + 1) << 42);
 }
-__device__ __forceinline__ uint32_t hdr_puzzle(uint64_t h) { return (uint32_t)h; }
-__device__ __forceinline__ uint32_t hdr_kmin(uint64_t h) { return (uint32_t)(h >> 32) & 0xFFFF; }
-__device__ __forceinline__ int hdr_last(uint64_t h) { return (int)((uint32_t)(h >> 48) & 0xFFFF) - 1; }
-constexpr uint32_t kDeadItem = 0xFFFF;  // kmin of a slot reserved but never written
+__device__ __forceinline__ uint32_t hdr_puzzle(uint64_t h) { return (uint32_t)h & 0xFFFFF; }
+__device__ __forceinline__ uint32_t hdr_floor(uint64_t h) { return (uint32_t)(h >> 20) & 0x7FF; }
+__device__ __forceinline__ uint32_t hdr_level(uint64_t h) { return (uint32_t)(h >> 31) & 0x7FF; }
+__device__ __forceinline__ int hdr_last(uint64_t h) { return (int)((uint32_t)(h >> 42) & 0xFFF) - 1; }
+constexpr uint32_t kDeadItem = 0x7FF;   // floor of a slot reserved but never written
 
 // Read-only view of one table blob.  Per-kind fields are kept as three
 // scalars and picked with selects, so a runtime kind never indexes a private
@@ -199,10 +208,23 @@ __global__ void expand_kernel(DfsArgs a) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.n_in) return;
     const uint64_t hdr = a.in_hdr[idx];
-    const uint32_t kmin = hdr_kmin(hdr);
+    const uint32_t kmin = hdr_floor(hdr);
     if (kmin == kDeadItem) return;
     const uint32_t puzzle = hdr_puzzle(hdr);
     const int last = hdr_last(hdr);
+    if (hdr_level(hdr) != kmin) {
+        // A resumed search is passed through unchanged.
+        if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+        const uint32_t o = atomicAdd(a.out_count, 1u);
+        if (o >= a.out_cap) {
+            atomicOr(&a.flags[0], 1u);
+            return;
+        }
+        const uint32_t lv = hdr_level(hdr);
+        for (uint32_t j = 0; j < lv; ++j) a.out_stack[(size_t)o * a.stride + j] = a.in_stack[(size_t)idx * a.stride + j];
+        a.out_hdr[o] = hdr;
+        return;
+    }
     TableView tv;
     tv.load(a.tables + a.table_off[puzzle]);
     const uint16_t *stack = a.in_stack + (size_t)idx * a.stride;
@@ -270,7 +292,7 @@ __global__ void expand_kernel(DfsArgs a) {
             uint16_t *dst = a.out_stack + (size_t)o * a.stride;
             for (uint32_t j = 0; j < kmin; ++j) dst[j] = stack[j];
             dst[kmin] = (uint16_t)t;
-            a.out_hdr[o] = make_hdr(puzzle, kmin + 1, -1);
+            a.out_hdr[o] = make_hdr(puzzle, kmin + 1, kmin + 1, -1);
             ++o;
         }
     }
@@ -332,7 +354,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                 const uint32_t idx = base + rank;
                 if (idx < a.n_in) {
                     const uint64_t hdr = a.in_hdr[idx];
-                    kmin = hdr_kmin(hdr);
+                    kmin = hdr_floor(hdr);
                     puzzle = hdr_puzzle(hdr);
                     bool take = kmin != kDeadItem;
                     if (take && a.node_limit[puzzle] && a.nodes[puzzle] >= a.node_limit[puzzle]) {
@@ -343,14 +365,14 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                         if (!LDS_TABLE) tv.load(a.tables + a.table_off[puzzle]);
                         P = tv.P;
                         last = hdr_last(hdr);
-                        k = kmin;
+                        k = hdr_level(hdr);
                         steps = 0;
                         nodes = 0;
                         may_split = true;
                         const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
                         for (int K = 0; K < 3; ++K)
                             for (uint32_t w = 0, nw = tv.words(K); w < nw; ++w) A(K, (int)w) = tv.b[tv.off_valid(K) + w];
-                        for (uint32_t j = 0; j < kmin; ++j) {
+                        for (uint32_t j = 0; j < k; ++j) {
                             const uint32_t t = src[j];
                             S(j) = (uint16_t)t;
                             const int K = (int)(tv.pos(j) & 3);
@@ -409,20 +431,24 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                 }
             }
             if (steps >= budget && may_split) {
-                // Split: what remains is, for each index j in [kmin, k], the
-                // candidates above the current choice at j (above `last` at k).
-                // Reservations stop once the output queue has overflowed, so
-                // out_count stays below out_cap + lanes * P (no wrap-around).
-                const uint32_t n = k - kmin + 1;
+                // Split in two: (a) the candidates above the current choice at
+                // the floor index -- usually the bulk of what remains -- and
+                // (b) the search in progress below that choice, resumed at
+                // index k above `last` with the floor one index deeper.  At the
+                // floor itself the item is simply requeued.  Each item splits at
+                // most once per pass, so a pass emits at most twice its input.
+                const uint32_t n = k > kmin ? 2u : 1u;
                 uint32_t base = 0xFFFFFFFFu;
                 if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
                     base = atomicAdd(a.out_count, n);
                 if (base != 0xFFFFFFFFu && (uint64_t)base + n <= a.out_cap) {
-                    for (uint32_t j = kmin; j <= k; ++j) {
-                        const uint32_t o = base + (j - kmin);
-                        uint16_t *dst = a.out_stack + (size_t)o * a.stride;
-                        for (uint32_t q = 0; q < j; ++q) dst[q] = S(q);
-                        a.out_hdr[o] = make_hdr(puzzle, j, j < k ? (int)S(j) : last);
+                    uint16_t *da = a.out_stack + (size_t)base * a.stride;
+                    for (uint32_t q = 0; q < kmin; ++q) da[q] = S(q);
+                    a.out_hdr[base] = make_hdr(puzzle, kmin, kmin, k > kmin ? (int)S(kmin) : last);
+                    if (n == 2) {
+                        uint16_t *db = a.out_stack + (size_t)(base + 1) * a.stride;
+                        for (uint32_t q = 0; q < k; ++q) db[q] = S(q);
+                        a.out_hdr[base + 1] = make_hdr(puzzle, kmin + 1, k, last);
                     }
                     atomicAdd(&a.flags[1], 1u);
                     finish();
@@ -432,7 +458,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                 // finish this item here without further split attempts.
                 if (base != 0xFFFFFFFFu)
                     for (uint64_t o = base; o < (uint64_t)base + n && o < a.out_cap; ++o)
-                        a.out_hdr[o] = make_hdr(0, kDeadItem, -1);
+                        a.out_hdr[o] = make_hdr(0, kDeadItem, kDeadItem, -1);
                 atomicOr(&a.flags[0], 1u);
                 may_split = false;
             }
@@ -575,9 +601,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     EMP_CHECK(hipMemcpyAsync(d_table_off, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
-    // Room for every lane to split once per pass (a split emits at most P
-    // items); beyond that lanes finish their items without splitting.
-    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * (max_P + 2), (uint64_t)n * 4, 1u << 16});
+    // A pass emits at most two items per input item; a full queue only makes
+    // lanes finish their items without splitting.
+    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n * 4, 1u << 16});
     const uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
                                          : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
     uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
@@ -598,7 +624,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     std::vector<uint16_t> h_stack((size_t)n * stride, 0);
     std::vector<unsigned long long> h_nodes(n, 1), h_zero(n, 0), h_first(n, ~0ull), h_limit(n, cfg.node_limit);
     for (uint32_t i = 0; i < n; ++i) {
-        h_hdr[i] = (uint64_t)i | (1ull << 32) | (0ull << 48);
+        h_hdr[i] = (uint64_t)i | (1ull << 20) | (1ull << 31);   // floor = level = 1, last = -1
         h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
     }
     EMP_CHECK(hipMemcpyAsync(d_hdr[0], h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
@@ -647,6 +673,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned long long *d_sols_save = ws.get<unsigned long long>("sols_save", n);
     unsigned int *d_counters_save = ws.get<unsigned int>("counters_save", 8);
 
+    const char *env = std::getenv("EMP_HIP_VERBOSE");
+    const bool verbose = env && *env && *env != '0';
     int cur = 0;
     uint64_t n_items = n;
     uint32_t expansions = 0;
@@ -696,6 +724,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
         n_items = std::min<uint64_t>(counters[1], cap);
         cur = 1 - cur;
+        if (verbose)
+            std::fprintf(stderr, "emp_hip: %s in=%u out=%u splits=%u overflow=%u %.3f ms\n",
+                         expand ? "expand" : "dfs", a.n_in, counters[1], counters[3], counters[2], ms);
     }
 
     // Results.
