@@ -301,7 +301,7 @@ __global__ void expand_kernel(DfsArgs a) {
 // ---------------------------------------------------------------------------
 // Persistent DFS.
 // ---------------------------------------------------------------------------
-template <int MAXW, bool LDS_TABLE>
+template <int MAXW, bool LDS_TABLE, typename STK>
 __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
     extern __shared__ uint64_t smem[];
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -312,9 +312,11 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
         for (uint32_t i = threadIdx.x; i < a.lds_table_words; i += blockDim.x) smem[i] = src[i];
         __syncthreads();
     }
-    const uint32_t per_wave = a.avail_words * kWave + (a.stride * kWave) / 4;
+    // Per wave: available-transition words then the stack of transitions,
+    // u8 when every kind has at most 256 transitions (boards up to 10x10).
+    const uint32_t per_wave = a.avail_words * kWave + (a.stride * kWave * (uint32_t)sizeof(STK)) / 8;
     uint64_t *avail = smem + a.lds_table_words + (size_t)wave * per_wave;
-    uint16_t *stk = reinterpret_cast<uint16_t *>(avail + a.avail_words * kWave);
+    STK *stk = reinterpret_cast<STK *>(avail + a.avail_words * kWave);
 
     TableView tv;
     if (LDS_TABLE) tv.load(smem);
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
     uint64_t steps = 0, nodes = 0;
 
     auto A = [&](int K, int w) -> uint64_t & { return avail[(tv.abase(K) + (uint32_t)w) * kWave + lane]; };
-    auto S = [&](uint32_t j) -> uint16_t & { return stk[j * kWave + lane]; };
+    auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
         atomicMax(&a.last_tick[puzzle], (unsigned long long)wall_clock64());
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                             for (uint32_t w = 0, nw = tv.words(K); w < nw; ++w) A(K, (int)w) = tv.b[tv.off_valid(K) + w];
                         for (uint32_t j = 0; j < k; ++j) {
                             const uint32_t t = src[j];
-                            S(j) = (uint16_t)t;
+                            S(j) = (STK)t;
                             const int K = (int)(tv.pos(j) & 3);
                             const uint32_t bb = t & ~3u;
                             A(K, (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
@@ -414,7 +416,7 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                 A(Kb, (int)wv) |= (0xFull << (bb & 63)) & tv.b[tv.off_valid(Kb) + wv];
                 last = (int)tb;
             } else {
-                S(k) = (uint16_t)t;
+                S(k) = (STK)t;
                 ++nodes;
                 if (k == P - 1) {
                     const uint64_t s = atomicAdd(&a.solutions[puzzle], 1ull);
@@ -486,10 +488,16 @@ void launch_expand(const DfsArgs &a, hipStream_t s) {
 }
 
 template <int MAXW>
-void launch_dfs(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes,
+void launch_dfs(const DfsArgs &a, bool lds, bool narrow, uint32_t blocks, uint32_t threads, size_t lds_bytes,
                 hipStream_t s) {
-    if (lds) hipLaunchKernelGGL((dfs_kernel<MAXW, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-    else hipLaunchKernelGGL((dfs_kernel<MAXW, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    if (lds && narrow)
+        hipLaunchKernelGGL((dfs_kernel<MAXW, true, uint8_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    else if (lds)
+        hipLaunchKernelGGL((dfs_kernel<MAXW, true, uint16_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    else if (narrow)
+        hipLaunchKernelGGL((dfs_kernel<MAXW, false, uint8_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL((dfs_kernel<MAXW, false, uint16_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
 }
 
 void dispatch_expand(int maxw, const DfsArgs &a, hipStream_t s) {
@@ -502,14 +510,14 @@ void dispatch_expand(int maxw, const DfsArgs &a, hipStream_t s) {
     }
 }
 
-void dispatch_dfs(int maxw, const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads,
+void dispatch_dfs(int maxw, const DfsArgs &a, bool lds, bool narrow, uint32_t blocks, uint32_t threads,
                   size_t lds_bytes, hipStream_t s) {
     switch (maxw) {
-        case 1: launch_dfs<1>(a, lds, blocks, threads, lds_bytes, s); break;
-        case 2: launch_dfs<2>(a, lds, blocks, threads, lds_bytes, s); break;
-        case 4: launch_dfs<4>(a, lds, blocks, threads, lds_bytes, s); break;
-        case 8: launch_dfs<8>(a, lds, blocks, threads, lds_bytes, s); break;
-        default: launch_dfs<16>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 1: launch_dfs<1>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
+        case 2: launch_dfs<2>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
+        case 4: launch_dfs<4>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
+        case 8: launch_dfs<8>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
+        default: launch_dfs<16>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
     }
 }
 
@@ -559,7 +567,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     std::vector<EnumTable> tables;
     tables.reserve(n);
     std::vector<uint32_t> table_off(n);
-    uint32_t total_words = 0, max_words = 1, max_avail = 1, max_P = 0, max_table = 0;
+    uint32_t total_words = 0, max_words = 1, max_avail = 1, max_P = 0, max_table = 0, max_transitions = 0;
     for (uint32_t i = 0; i < n; ++i) {
         tables.push_back(build_enum_table(*puzzles[i], cfg.order));
         EnumTableHeader h = tables.back().header();
@@ -568,6 +576,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
         max_avail = std::max(max_avail, h.avail_words);
         max_P = std::max(max_P, h.P);
+        for (int k = 0; k < 3; ++k) max_transitions = std::max<uint32_t>(max_transitions, 4 * (uint32_t)puzzles[i]->pieces_of(k).size());
         max_table = std::max(max_table, h.total_words);
     }
     if (max_words > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
@@ -579,15 +588,19 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     for (uint32_t i = 0; i < n; ++i)
         std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
 
-    // Launch geometry.
+    // Launch geometry: as many resident waves as LDS and registers allow.
+    // The kernels use about 80-100 VGPRs (5 waves per SIMD, 20 per CU), the
+    // 16-word variant about 150 (3 per SIMD, 12 per CU).
     const bool lds_table = n == 1;
+    const bool narrow = max_transitions <= 256;
     const uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
     const uint32_t threads = wpb * kWave;
-    const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * 2;
+    const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
     const size_t lds_bytes = (lds_table ? (size_t)max_table * 8 : 0) + wpb * per_wave_bytes;
     if (lds_bytes > dev.lds_per_block)
         throw Unsupported("per-block LDS of " + std::to_string(lds_bytes) + " bytes exceeds the device limit");
-    const uint32_t wpc = cfg.waves_per_cu ? cfg.waves_per_cu : 8;
+    const uint32_t resident_blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(dev.lds_per_block / lds_bytes), (maxw >= 16 ? 12u : 20u) / wpb));
+    const uint32_t wpc = cfg.waves_per_cu ? cfg.waves_per_cu : resident_blocks * wpb;
     const uint32_t blocks = std::max(1u, dev.compute_units * wpc / wpb);
     const uint64_t lanes = (uint64_t)blocks * threads;
     stats.lds_tables = lds_table;
@@ -695,7 +708,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
         EMP_CHECK(hipEventRecord(ev0, stream));
         if (expand) dispatch_expand(maxw, a, stream);
-        else dispatch_dfs(maxw, a, lds_table, blocks, threads, lds_bytes, stream);
+        else dispatch_dfs(maxw, a, lds_table, narrow, blocks, threads, lds_bytes, stream);
         EMP_CHECK(hipGetLastError());
         EMP_CHECK(hipEventRecord(ev1, stream));
         unsigned int counters[8];
