@@ -689,6 +689,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     const char *env = std::getenv("EMP_HIP_VERBOSE");
     const bool verbose = env && *env && *env != '0';
     int cur = 0;
+    bool expansion_grows = true;
     uint64_t n_items = n;
     uint32_t expansions = 0;
     const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 1024;
@@ -700,7 +701,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         a.out_stack = d_stack[1 - cur];
         a.out_cap = cap;
         hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
-        const bool expand = n_items < target && expansions < max_expansions;
+        // Expand while the queue is short of the target and expansions still
+        // multiply it (resumed searches pass through an expansion unchanged).
+        const bool expand = n_items < target && expansions < max_expansions && expansion_grows;
         if (expand) {
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
@@ -729,7 +732,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                 expansions = max_expansions;
                 continue;
             }
+            expansion_grows = counters[1] >= n_items + n_items / 4;
         } else {
+            expansion_grows = true;
             stats.dfs_kernel_ms += ms;
             ++stats.dfs_launches;
             stats.splits += counters[3];
