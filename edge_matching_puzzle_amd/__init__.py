@@ -53,6 +53,23 @@ class EnumStats(ctypes.Structure):
                 ("threads", ctypes.c_uint32)]
 
 
+class GluttonOptions(ctypes.Structure):
+    _fields_ = [("push_budget", ctypes.c_uint64), ("waves_per_cu", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class GluttonResult(ctypes.Structure):
+    _fields_ = [("max_depth", ctypes.c_uint32), ("status", ctypes.c_uint32), ("pushes", ctypes.c_uint64)]
+
+
+class GluttonStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("pushes", ctypes.c_uint64), ("evaluations", ctypes.c_uint64),
+                ("waves", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32)]
+
+
+GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)
+
 _lib = None
 
 # Every symbol include/emp_hip.h declares, with (restype, argtypes).
@@ -70,6 +87,8 @@ SIGNATURES = {
     "emp_hip_situation_string": (_I32, [_U32, _U32, _P, ctypes.c_char_p, _U64]),
     "emp_hip_enum_default_options": (None, [_P]),
     "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+    "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
+    "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
 }
 
 
@@ -126,6 +145,24 @@ def validate(width, height, colours):
     _check(lib().emp_hip_validate_puzzle(ctypes.byref(pz)))
 
 
+def pack_move(pos, piece, orientation):
+    return pos | (piece << 8) | (orientation << 16)
+
+
+def spiral_prefixes(width, height, colours, length, count, seed, fixed=()):
+    """Random distinct placements of the first `length` spiral positions,
+    after the `fixed` packed moves; returns [n, len(fixed) + length] uint32."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    fx = np.ascontiguousarray(np.array(list(fixed), dtype=np.uint32))
+    out = np.zeros((count, len(fixed) + length), dtype=np.uint32)
+    n = ctypes.c_uint32()
+    _check(lib().emp_hip_spiral_prefixes(ctypes.byref(pz), length, count, seed,
+                                         _ptr(fx) if len(fixed) else None, len(fixed), _ptr(out),
+                                         ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
 def order(width, height, kind="spiral"):
     xy = np.zeros(2 * width * height, dtype=np.uint32)
     _check(lib().emp_hip_order(width, height, _ORDERS[kind], _ptr(xy)))
@@ -168,6 +205,28 @@ class Context:
         cu, lds, clk = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().emp_hip_device_info(self._ctx, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(clk)))
         return {"compute_units": cu.value, "lds_per_block": lds.value, "clock_khz": clk.value}
+
+    def glutton_max(self, width, height, colours, prefixes=None, push_budget=0, waves_per_cu=0,
+                    n_searches=None):
+        """One glutton_max search per prefix row (packed moves); returns
+        (results, boards [n, P] uint16, stats)."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        if prefixes is None or len(prefixes) == 0 or np.asarray(prefixes).size == 0:
+            n = n_searches or 1
+            pre = np.zeros((n, 0), dtype=np.uint32)
+        else:
+            pre = np.ascontiguousarray(np.asarray(prefixes, dtype=np.uint32))
+            n = pre.shape[0]
+        plen = pre.shape[1] if pre.ndim == 2 else 0
+        res = (GluttonResult * n)()
+        boards = np.zeros((n, width * height), dtype=np.uint16)
+        opt = GluttonOptions(push_budget, waves_per_cu, 0)
+        st = GluttonStats()
+        _check(lib().emp_hip_glutton_max(self._ctx, ctypes.byref(pz), _ptr(pre) if plen else None, plen, n,
+                                         ctypes.byref(opt), ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
+        results = [{"max_depth": r.max_depth, "status": r.status, "pushes": r.pushes} for r in res]
+        return results, boards, {f: getattr(st, f) for f, _ in GluttonStats._fields_}
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,

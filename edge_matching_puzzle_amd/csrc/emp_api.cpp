@@ -11,8 +11,12 @@
 #include <vector>
 
 #include "../../include/emp_hip.h"
+#include "emp_enum_plan.h"
 #include "emp_enumerate.h"
+#include "emp_glutton.h"
 #include "emp_model.h"
+
+#include <unordered_set>
 
 struct emp_hip_context {
     emp::DeviceInfo dev;
@@ -229,6 +233,64 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
             stats->blocks = st.blocks;
             stats->threads = st.threads;
         }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *prefixes,
+                        uint32_t prefix_len, uint32_t n_searches, const emp_hip_glutton_options *options,
+                        emp_hip_glutton_result *results, uint16_t *boards, emp_hip_glutton_stats *stats) {
+    return guarded([&] {
+        if (!context || !puzzle || !results || !boards || !n_searches)
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument or no search");
+        if (prefix_len && !prefixes) return fail(EMP_HIP_BAD_ARGUMENT, "null prefixes");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        if (pz.size() > 256) return fail(EMP_HIP_UNSUPPORTED, "glutton_max handles at most 256 pieces");
+        if (prefix_len >= pz.size()) return fail(EMP_HIP_BAD_ARGUMENT, "prefix as long as the board");
+        emp::GluttonConfig cfg;
+        if (options) {
+            cfg.push_budget = options->push_budget;
+            cfg.waves_per_cu = options->waves_per_cu;
+        }
+        std::vector<uint32_t> pre(prefixes, prefixes + (size_t)prefix_len * n_searches);
+        std::vector<emp::GluttonOutcome> out;
+        emp::GluttonStats st;
+        emp::glutton_on_device(context->dev, context->stream, context->workspace, pz, pre, prefix_len, n_searches,
+                               cfg, out, st);
+        for (uint32_t i = 0; i < n_searches; ++i) {
+            results[i].max_depth = out[i].max_depth;
+            results[i].status = out[i].status;
+            results[i].pushes = out[i].pushes;
+            if (out[i].status > EMP_HIP_GLUTTON_BAD_PREFIX)
+                return fail(EMP_HIP_DEVICE_ERROR, "glutton search " + std::to_string(i) + " stopped on an unplayable best move");
+            std::copy(out[i].board.begin(), out[i].board.end(), boards + (size_t)i * pz.size());
+        }
+        if (stats) {
+            stats->total_ms = st.total_ms;
+            stats->kernel_ms = st.kernel_ms;
+            stats->pushes = st.pushes;
+            stats->evaluations = st.evaluations;
+            stats->waves = st.waves;
+            stats->lds_bytes = st.lds_bytes;
+        }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_spiral_prefixes(const emp_hip_puzzle *puzzle, uint32_t length, uint32_t count, uint64_t seed,
+                            const uint32_t *fixed_moves, uint32_t n_fixed, uint32_t *out, uint32_t *n_out) {
+    return guarded([&] {
+        if (!puzzle || !out || !n_out || (n_fixed && !fixed_moves))
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        if (length == 0 || length >= pz.size()) return fail(EMP_HIP_BAD_ARGUMENT, "bad prefix length");
+        std::vector<uint32_t> rows;
+        uint32_t produced = 0;
+        emp::spiral_prefixes(pz, length, count, seed,
+                             std::vector<uint32_t>(fixed_moves, fixed_moves + n_fixed), rows, produced);
+        std::copy(rows.begin(), rows.end(), out);
+        *n_out = produced;
         return EMP_HIP_OK;
     });
 }

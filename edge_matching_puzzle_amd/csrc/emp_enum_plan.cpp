@@ -3,6 +3,7 @@
 
 #include <cstring>
 #include <stdexcept>
+#include <unordered_set>
 
 namespace emp {
 
@@ -162,6 +163,82 @@ void stack_to_board(const EnumTable &t, const uint16_t *stack, uint16_t *board) 
         uint32_t piece = t.piece_of[k][tr >> 2];
         board[t.order[i].second * h.W + t.order[i].first] = (uint16_t)((piece << 2) | (tr & 3));
     }
+}
+
+void spiral_prefixes(const Puzzle &pz, uint32_t length, uint32_t count, uint64_t seed,
+                     const std::vector<uint32_t> &fixed, std::vector<uint32_t> &rows, uint32_t &produced) {
+    if (pz.size() > 256) throw std::domain_error("prefix moves pack positions and pieces in 8 bits");
+    const EnumTable t = build_enum_table(pz, kSpiral);
+    const EnumTableHeader h = t.header();
+    const uint64_t *b = t.blob.data();
+    std::vector<char> fixed_piece(pz.size(), 0), fixed_cell(pz.size(), 0);
+    for (uint32_t mv : fixed) {
+        if ((mv & 0xFF) >= pz.size() || ((mv >> 8) & 0xFF) >= pz.size()) throw std::invalid_argument("bad fixed move");
+        fixed_cell[mv & 0xFF] = 1;
+        fixed_piece[(mv >> 8) & 0xFF] = 1;
+    }
+    uint64_t state = seed;
+    std::unordered_set<std::string> seen;
+    const uint32_t width = (uint32_t)fixed.size() + length;
+    rows.clear();
+    produced = 0;
+    std::vector<uint16_t> stack(length);
+    const uint64_t attempts = (uint64_t)count * 8 + 64;
+    for (uint64_t attempt = 0; attempt < attempts && produced < count; ++attempt) {
+        // Available transitions per kind, minus the fixed pieces.
+        std::vector<uint64_t> avail[3];
+        for (int k = 0; k < 3; ++k) {
+            avail[k].assign(b + h.off_valid[k], b + h.off_valid[k] + h.words[k]);
+            for (uint32_t ki = 0; ki < t.piece_of[k].size(); ++ki)
+                if (fixed_piece[t.piece_of[k][ki]]) avail[k][(4 * ki) >> 6] &= ~(0xFull << ((4 * ki) & 63));
+        }
+        bool ok = true;
+        for (uint32_t i = 0; i < length && ok; ++i) {
+            const uint64_t rec = b[h.off_pos + i];
+            const int k = (int)(rec & 3);
+            const uint32_t x = t.order[i].first, y = t.order[i].second;
+            if (fixed_cell[y * h.W + x]) { ok = false; break; }
+            std::vector<uint64_t> c(avail[k]);
+            for (unsigned s = 0; s < 4; ++s) {
+                const uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
+                if (code == kUnconstrained) continue;
+                uint32_t col = 0;
+                if (code != kOffBoard) {
+                    const int nk = (int)(rec >> (11 + 11 * s)) & 3;
+                    col = (reinterpret_cast<const uint32_t *>(b + h.off_rot[nk])[stack[code]] >> (8 * ((s + 2) & 3))) & 0xFF;
+                }
+                const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + col) * h.words[k];
+                for (uint32_t w = 0; w < h.words[k]; ++w) c[w] &= m[w];
+            }
+            uint32_t n = 0;
+            for (uint64_t v : c) n += (uint32_t)__builtin_popcountll(v);
+            if (!n) { ok = false; break; }
+            uint32_t pick = (uint32_t)(((splitmix64(state) >> 32) * (uint64_t)n) >> 32);
+            uint32_t tr = 0;
+            for (uint32_t w = 0; w < h.words[k]; ++w) {
+                uint64_t v = c[w];
+                const uint32_t pc = (uint32_t)__builtin_popcountll(v);
+                if (pick >= pc) { pick -= pc; continue; }
+                while (pick--) v &= v - 1;
+                tr = w * 64 + (uint32_t)__builtin_ctzll(v);
+                break;
+            }
+            stack[i] = (uint16_t)tr;
+            avail[k][(tr & ~3u) >> 6] &= ~(0xFull << ((tr & ~3u) & 63));
+        }
+        if (!ok) continue;
+        std::string key(reinterpret_cast<const char *>(stack.data()), length * sizeof(uint16_t));
+        if (!seen.insert(key).second) continue;
+        for (uint32_t mv : fixed) rows.push_back(mv);
+        for (uint32_t i = 0; i < length; ++i) {
+            const int k = (int)(b[h.off_pos + i] & 3);
+            const uint32_t piece = t.piece_of[k][stack[i] >> 2];
+            const uint32_t pos = t.order[i].second * h.W + t.order[i].first;
+            rows.push_back(pos | (piece << 8) | ((uint32_t)(stack[i] & 3) << 16));
+        }
+        ++produced;
+    }
+    (void)width;
 }
 
 }  // namespace emp

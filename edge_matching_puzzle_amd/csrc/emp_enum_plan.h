@@ -60,6 +60,13 @@ EnumTable build_enum_table(const Puzzle &puzzle, int order);
 // Converts a stack of transitions (one per strategy index) to a row-major board.
 void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *board);
 
+// Random spiral prefixes: up to `count` distinct rows, each the fixed moves
+// followed by a placement of strategy indices 0..length-1 in spiral order,
+// drawn by seeded random dives over matching pieces (pieces used by the fixed
+// moves are excluded).  Moves are packed pos | piece << 8 | o << 16.
+void spiral_prefixes(const Puzzle &puzzle, uint32_t length, uint32_t count, uint64_t seed,
+                     const std::vector<uint32_t> &fixed, std::vector<uint32_t> &rows, uint32_t &produced);
+
 }  // namespace emp
 
 #endif

@@ -1,0 +1,695 @@
+// emp_glutton.hip -- glutton_max on gfx950: one wave64 per search.
+//
+// The search is the reference's glutton_max kernel loop
+// (include/CUDA_glutton_max.h:1522-1637): at each level remove every
+// candidate that would leave a free position or an available piece without
+// option (remove_invalid_transitions, :742-954), play the best-scored
+// candidate (get_best_candidate, :1007-1301; apply_best_candidate,
+// :1303-1476), and pop when a level runs out of candidates.  Masks are
+// piece-major (bit 4 * piece + orientation, include/CUDA_piece_position_info2.h)
+// with slot bookkeeping identical to include/CUDA_glutton_max_stack.h:434-522,
+// so the tie-breaks of the reference's known answers are kept.
+//
+// Parallel form, identical results:
+//   * remove_invalid_transitions reaches the greatest set of candidates that
+//     are all valid with respect to each other; validity only shrinks when
+//     masks shrink, so that set does not depend on the order of removals.
+//     Each lane evaluates candidates against the live masks and clears
+//     invalid ones at once (LDS atomic AND); rounds repeat until a round
+//     removes nothing.
+//   * the score of get_best_candidate for candidate (slot i, piece p, o) with
+//     colour-constrained free neighbours j_l and results r_l is
+//        total = 2 (PC - T(p) - pc_i + cnt_i(p) - sum_l (pc_jl - cnt_jl(p)))
+//              + 2 sum_l |r_l|
+//     where pc_j is the popcount of slot j, cnt_j(q) the orientations of piece
+//     q in slot j, T(q) = sum_j cnt_j(q) and PC = sum_j pc_j (every mask bit
+//     belongs to an available piece).  min / max run over the per-slot
+//     popcounts after the move and the per-piece option counts
+//     T(q) - cnt_i(q) - sum_l (cnt_jl(q) - cnt_{r_l}(q)).
+//     The best candidate is the highest (total, (max << 16) + min), first in
+//     (slot, bit) order on ties; the last candidate at the last level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "emp_enumerate.h"
+#include "emp_glutton.h"
+
+namespace emp {
+
+namespace {
+
+constexpr int kWave = 64;
+
+struct GlArgs {
+    uint32_t P, W, S, nw, max_colour;
+    const uint64_t *cm;          // [((c - 1) * 4 + side) * nw + w]
+    const uint32_t *piece_col;   // [P] N | E << 8 | S << 16 | W << 24
+    const uint64_t *init_masks;  // [P][nw] initial capability per position
+    const uint32_t *prefixes;    // [n_items][prefix_len] pos | piece << 8 | o << 16
+    uint32_t n_items, prefix_len;
+    uint64_t push_budget;
+    uint64_t *stacks;            // per wave: S(S+1)/2 * nw words
+    uint32_t *max_played;        // per wave: S words
+    unsigned int *next_item;
+    uint32_t *out_max_depth;     // [n_items]
+    uint64_t *out_pushes;        // [n_items]
+    uint32_t *out_status;        // [n_items]
+    uint16_t *out_board;         // [n_items][P]
+    unsigned long long *evals;   // candidates evaluated
+};
+
+// Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
+struct GlLayout {
+    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, played, avail, scratch, total;
+    __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw) {
+        GlLayout l;
+        uint32_t o = 0;
+        auto take = [&](uint32_t bytes) { uint32_t r = o; o += (bytes + 7) & ~7u; return r; };
+        l.masks = take(S * nw * 8);
+        l.pos_of_slot = take(S * 2);
+        l.slot_of_pos = take(P * 2);
+        l.pc = take(S * 2);
+        l.sp = take(S * 2);
+        l.T = take(256 * 2);
+        l.played = take(S * 4);
+        l.avail = take(4 * 8);
+        l.scratch = take(64);
+        l.total = o;
+        return l;
+    }
+};
+
+__device__ __forceinline__ uint32_t nib(uint64_t v, uint32_t q) {
+    return (uint32_t)__popcll((v >> (4 * (q & 15))) & 0xFull);
+}
+
+struct Wave {
+    const GlArgs &a;
+    uint64_t *M;
+    uint16_t *pos_of_slot, *slot_of_pos, *pc, *sp, *T;
+    uint32_t *played;
+    uint64_t *avail;
+    uint32_t *scratch;
+    uint64_t *stack;
+    uint32_t *max_played;
+    uint32_t lane;
+    uint32_t level, max, base;
+    int off[4];
+
+    __device__ Wave(const GlArgs &args, unsigned char *lds, uint32_t wave_id) : a(args) {
+        GlLayout L = GlLayout::make(a.S, a.P, a.nw);
+        M = reinterpret_cast<uint64_t *>(lds + L.masks);
+        pos_of_slot = reinterpret_cast<uint16_t *>(lds + L.pos_of_slot);
+        slot_of_pos = reinterpret_cast<uint16_t *>(lds + L.slot_of_pos);
+        pc = reinterpret_cast<uint16_t *>(lds + L.pc);
+        sp = reinterpret_cast<uint16_t *>(lds + L.sp);
+        T = reinterpret_cast<uint16_t *>(lds + L.T);
+        played = reinterpret_cast<uint32_t *>(lds + L.played);
+        avail = reinterpret_cast<uint64_t *>(lds + L.avail);
+        scratch = reinterpret_cast<uint32_t *>(lds + L.scratch);
+        stack = a.stacks + (size_t)wave_id * ((size_t)a.S * (a.S + 1) / 2) * a.nw;
+        max_played = a.max_played + (size_t)wave_id * a.S;
+        lane = threadIdx.x;
+        off[0] = -(int)a.W;
+        off[1] = 1;
+        off[2] = (int)a.W;
+        off[3] = -1;
+    }
+
+    __device__ uint32_t n() const { return a.S - level; }
+    __device__ void count(uint64_t evaluated) {
+        for (int o = 32; o > 0; o >>= 1) evaluated += __shfl_xor(evaluated, o);
+        if (lane == 0 && evaluated) atomicAdd(a.evals, (unsigned long long)evaluated);
+    }
+    __device__ bool is_free(uint32_t pos) const { return slot_of_pos[pos] < n(); }
+    __device__ bool piece_avail(uint32_t q) const { return (avail[q >> 6] >> (q & 63)) & 1; }
+    __device__ size_t level_offset(uint32_t l) const {
+        return ((size_t)a.S * (a.S + 1) - (size_t)(a.S - l) * (a.S - l + 1)) / 2 * a.nw;
+    }
+
+    __device__ void init() {
+        for (uint32_t k = lane; k < a.S * a.nw; k += kWave) M[k] = a.init_masks[k];
+        for (uint32_t j = lane; j < a.S; j += kWave) {
+            pos_of_slot[j] = (uint16_t)j;
+            played[j] = 0xFFFFFFFFu;
+        }
+        for (uint32_t q = lane; q < a.P; q += kWave) slot_of_pos[q] = (uint16_t)q;
+        if (lane < 4) {
+            uint64_t v = 0;
+            for (uint32_t b = 0; b < 64; ++b)
+                if (lane * 64 + b < a.P) v |= 1ull << b;
+            avail[lane] = v;
+        }
+        level = max = base = 0;
+        __syncthreads();
+    }
+
+    // Per-slot popcount, single-piece slot marker and per-piece totals;
+    // returns false when a slot is empty.  Sets PC.
+    __device__ bool stats(uint32_t &PC) {
+        const uint32_t nn = n();
+        uint32_t local = 0;
+        bool empty = false;
+        for (uint32_t j = lane; j < nn; j += kWave) {
+            uint32_t cnt = 0, pieces = 0, first = 0xFFFF;
+            for (uint32_t w = 0; w < a.nw; ++w) {
+                const uint64_t v = M[j * a.nw + w];
+                cnt += (uint32_t)__popcll(v);
+                const uint64_t nz = (v | (v >> 1) | (v >> 2) | (v >> 3)) & 0x1111111111111111ull;
+                if (nz) {
+                    if (first == 0xFFFF) first = w * 16 + (uint32_t)__builtin_ctzll(nz) / 4;
+                    pieces += (uint32_t)__popcll(nz);
+                }
+            }
+            pc[j] = (uint16_t)cnt;
+            sp[j] = pieces == 1 ? (uint16_t)first : (uint16_t)0xFFFF;
+            local += cnt;
+            empty |= cnt == 0;
+        }
+        for (uint32_t q = lane; q < a.P; q += kWave) {
+            uint32_t t = 0;
+            const uint32_t w = q >> 4;
+            for (uint32_t j = 0; j < nn; ++j) t += nib(M[j * a.nw + w], q);
+            T[q] = (uint16_t)t;
+        }
+        for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
+        PC = local;
+        const bool any_empty = __ballot(empty) != 0;
+        __syncthreads();
+        return !any_empty;
+    }
+
+    // Evaluates candidate (slot i, piece p, orientation o) against the live
+    // masks.  Returns validity; with score set also total and min / max.
+    __device__ bool evaluate(uint32_t i, uint32_t p, uint32_t o, bool score, uint32_t PC,
+                             uint32_t &total, uint32_t &minmax) {
+        const uint32_t nn = n();
+        const uint32_t pos = pos_of_slot[i];
+        const uint32_t pcol = a.piece_col[p];
+        const uint32_t pw = p >> 4;
+        const uint64_t keep_p = ~(0xFull << (4 * (p & 15)));
+        uint32_t nb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        const uint64_t *cmp[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint32_t rsum = 0, nb_pc_minus = 0;
+        uint32_t mn = 0xFFFFFFFFu, mx = 0;
+        // Colour-constrained free neighbours (apply_color_constraints :528-612).
+        for (uint32_t l = 0; l < 4; ++l) {
+            const uint32_t col = (pcol >> (8 * ((l + o) & 3))) & 0xFF;
+            if (!col) continue;
+            const uint32_t q = (uint32_t)((int)pos + off[l]);
+            if (q >= a.P || !is_free(q)) continue;   // a coloured side never faces the frame
+            const uint32_t j = slot_of_pos[q];
+            const uint64_t *c = a.cm + ((size_t)(col - 1) * 4 + l) * a.nw;
+            uint32_t rp = 0;
+            for (uint32_t w = 0; w < a.nw; ++w) {
+                uint64_t r = M[j * a.nw + w] & c[w];
+                if (w == pw) r &= keep_p;
+                rp += (uint32_t)__popcll(r);
+            }
+            if (!rp) return false;
+            nb[l] = j;
+            cmp[l] = c;
+            rsum += rp;
+            nb_pc_minus += pc[j] - nib(M[j * a.nw + pw], p);
+            if (score) {
+                mn = rp < mn ? rp : mn;
+                mx = rp > mx ? rp : mx;
+            }
+        }
+        // Other free slots: invalid when piece p is all they hold.
+        for (uint32_t j = 0; j < nn; ++j) {
+            if (j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3]) continue;
+            if (sp[j] == p) return false;
+            if (score) {
+                const uint32_t v = pc[j] - nib(M[j * a.nw + pw], p);
+                mn = v < mn ? v : mn;
+                mx = v > mx ? v : mx;
+            }
+        }
+        // Option counts of the other available pieces.
+        uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
+        for (uint32_t w = 0; w < a.nw; ++w) {
+            const uint64_t mi = M[i * a.nw + w];
+            uint64_t mj[4], rj[4];
+            for (uint32_t l = 0; l < 4; ++l) {
+                if (nb[l] == 0xFFFFFFFFu) { mj[l] = 0; rj[l] = 0; continue; }
+                mj[l] = M[nb[l] * a.nw + w];
+                rj[l] = mj[l] & cmp[l][w];
+            }
+            for (uint32_t b = 0; b < 16; ++b) {
+                const uint32_t q = w * 16 + b;
+                if (q >= a.P) break;
+                if (q == p || !piece_avail(q)) continue;
+                uint32_t d = nib(mi, q);
+                for (uint32_t l = 0; l < 4; ++l) d += nib(mj[l], q) - nib(rj[l], q);
+                const uint32_t c = (uint32_t)T[q] - d;
+                if (!c) return false;
+                pmn = c < pmn ? c : pmn;
+                pmx = c > pmx ? c : pmx;
+            }
+        }
+        if (score) {
+            mn = pmn < mn ? pmn : mn;
+            mx = pmx > mx ? pmx : mx;
+            const uint32_t cnt_ip = nib(M[i * a.nw + pw], p);
+            total = 2 * (PC - T[p] - pc[i] + cnt_ip - nb_pc_minus) + 2 * rsum;
+            minmax = (mx << 16) + mn;
+        }
+        return true;
+    }
+
+    // remove_invalid_transitions: false when a slot runs empty.
+    __device__ bool fixpoint() {
+        uint64_t evaluated = 0;
+        while (true) {
+            uint32_t PC;
+            if (!stats(PC)) return false;
+            bool removed = false;
+            const uint32_t pairs = n() * a.nw;
+            for (uint32_t k = lane; k < pairs; k += kWave) {
+                uint64_t v = M[k];
+                const uint32_t j = k / a.nw, w = k % a.nw;
+                while (v) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(v);
+                    v &= v - 1;
+                    const uint32_t bit = w * 64 + b;
+                    uint32_t t, mm;
+                    ++evaluated;
+                    if (!evaluate(j, bit >> 2, bit & 3, false, PC, t, mm)) {
+                        atomicAnd(reinterpret_cast<unsigned long long *>(&M[k]), ~(1ull << b));
+                        removed = true;
+                    }
+                }
+            }
+            __syncthreads();
+            if (!__ballot(removed)) break;
+        }
+        count(evaluated);
+        uint32_t PC;
+        return stats(PC);
+    }
+
+    // get_best_candidate: returns (slot << 16) | bit, or 0xFFFFFFFF.
+    __device__ uint32_t best() {
+        uint32_t PC;
+        stats(PC);
+        const bool last_level = level == a.S - 1;
+        uint64_t best_key = 0;
+        uint32_t best_idx = 0xFFFFFFFFu;
+        bool have = false;
+        uint64_t evaluated = 0;
+        const uint32_t pairs = n() * a.nw;
+        for (uint32_t k = lane; k < pairs; k += kWave) {
+            uint64_t v = M[k];
+            const uint32_t j = k / a.nw, w = k % a.nw;
+            while (v) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(v);
+                v &= v - 1;
+                const uint32_t bit = w * 64 + b;
+                uint32_t total = 0, mm = 0;
+                ++evaluated;
+                if (!evaluate(j, bit >> 2, bit & 3, true, PC, total, mm)) continue;
+                const uint64_t key = last_level ? 0 : (((uint64_t)total << 32) | mm);
+                const uint32_t idx = (j << 16) | bit;
+                // higher key wins; on equal keys the earlier candidate, except
+                // at the last level where the later one wins
+                const bool better = !have || key > best_key ||
+                                    (key == best_key && (last_level ? idx > best_idx : idx < best_idx));
+                if (better) {
+                    best_key = key;
+                    best_idx = idx;
+                    have = true;
+                }
+            }
+        }
+        count(evaluated);
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t ok = __shfl_xor(best_key, o);
+            const uint32_t oi = __shfl_xor(best_idx, o);
+            const bool oh = __shfl_xor((int)have, o) != 0;
+            const bool better = oh && (!have || ok > best_key ||
+                                       (ok == best_key && (last_level ? oi > best_idx : oi < best_idx)));
+            if (better) {
+                best_key = ok;
+                best_idx = oi;
+                have = true;
+            }
+        }
+        return have ? best_idx : 0xFFFFFFFFu;
+    }
+
+    // CUDA_glutton_max_stack::push (:434-495), bookkeeping by lane 0.
+    __device__ void push(uint32_t slot, uint32_t pos, uint32_t piece, uint32_t o) {
+        if (lane == 0) {
+            played[level] = (o << 16) | (piece << 8) | pos;
+            avail[piece >> 6] &= ~(1ull << (piece & 63));
+        }
+        __syncthreads();
+        const uint32_t nl = level + 1;
+        if (nl > max) {
+            for (uint32_t k = lane; k < nl; k += kWave) max_played[k] = played[k];
+            max = nl;
+        }
+        if (lane == 0) {
+            const uint32_t last = a.S - nl;
+            const uint32_t last_pos = pos_of_slot[last];
+            pos_of_slot[slot] = (uint16_t)slot;
+            const uint16_t t = pos_of_slot[last];
+            pos_of_slot[last] = pos_of_slot[slot];
+            pos_of_slot[slot] = t;
+            const uint16_t u = slot_of_pos[last_pos];
+            slot_of_pos[last_pos] = slot_of_pos[pos];
+            slot_of_pos[pos] = u;
+        }
+        level = nl;
+        __syncthreads();
+    }
+
+    // CUDA_glutton_max_stack::pop (:497-522), then reload the level's masks.
+    __device__ uint32_t pop() {
+        uint32_t slot = 0;
+        if (lane == 0) {
+            const uint32_t last = a.S - level;
+            slot = pos_of_slot[last];
+            const uint32_t pl = played[level - 1];
+            const uint32_t pos = pl & 0xFF, piece = (pl >> 8) & 0xFF;
+            avail[piece >> 6] |= 1ull << (piece & 63);
+            pos_of_slot[last] = (uint16_t)pos;
+            const uint32_t other = pos_of_slot[slot];
+            const uint16_t t = pos_of_slot[last];
+            pos_of_slot[last] = pos_of_slot[slot];
+            pos_of_slot[slot] = t;
+            const uint16_t u = slot_of_pos[other];
+            slot_of_pos[other] = slot_of_pos[pos];
+            slot_of_pos[pos] = u;
+        }
+        slot = __shfl(slot, 0);
+        level -= 1;
+        __syncthreads();
+        const uint64_t *src = stack + level_offset(level);
+        for (uint32_t k = lane; k < n() * a.nw; k += kWave) M[k] = src[k];
+        __syncthreads();
+        return slot;
+    }
+
+    // apply_best_candidate (:1303-1476).  False where the reference stops
+    // (a position emptied by the move).
+    __device__ bool apply(uint32_t slot, uint32_t bit) {
+        const uint32_t piece = bit >> 2, o = bit & 3;
+        const uint32_t nn = n();
+        if (lane == 0) M[slot * a.nw + (bit >> 6)] &= ~(1ull << (bit & 63));
+        __syncthreads();
+        // Save this level (with the played bit cleared) for a later pop.
+        uint64_t *dst = stack + level_offset(level);
+        for (uint32_t k = lane; k < nn * a.nw; k += kWave) dst[k] = M[k];
+        const uint32_t pos = pos_of_slot[slot];
+        const uint32_t pw = piece >> 4;
+        const uint64_t keep_p = ~(0xFull << (4 * (piece & 15)));
+        bool empty = false;
+        if (level + 1 < a.S) {
+            // Next level in place: slot `slot` takes the last slot's masks.
+            for (uint32_t k = lane; k < (nn - 1) * a.nw; k += kWave) {
+                const uint32_t j = k / a.nw, w = k % a.nw;
+                uint64_t v = (j == slot) ? M[(nn - 1) * a.nw + w] : M[k];
+                if (w == pw) v &= keep_p;
+                M[k] = v;
+            }
+            __syncthreads();
+            for (uint32_t j = lane; j + 1 < nn; j += kWave) {
+                uint64_t any = 0;
+                for (uint32_t w = 0; w < a.nw; ++w) any |= M[j * a.nw + w];
+                empty |= any == 0;
+            }
+        }
+        const bool bad = __ballot(empty) != 0;
+        __syncthreads();
+        if (bad) return false;
+        push(slot, pos, piece, o);
+        const uint32_t pcol = a.piece_col[piece];
+        bool emptied = false;
+        for (uint32_t l = 0; l < 4; ++l) {
+            const uint32_t col = (pcol >> (8 * ((l + o) & 3))) & 0xFF;
+            if (!col) continue;
+            const uint32_t q = (uint32_t)((int)pos + off[l]);
+            if (q >= a.P || !is_free(q)) continue;
+            const uint32_t j = slot_of_pos[q];
+            const uint64_t *c = a.cm + ((size_t)(col - 1) * 4 + l) * a.nw;
+            uint64_t any = 0;
+            for (uint32_t w = lane; w < a.nw; w += kWave) {
+                const uint64_t v = M[j * a.nw + w] & c[w];
+                M[j * a.nw + w] = v;
+                any |= v;
+            }
+            emptied |= __ballot(any != 0) == 0;
+            __syncthreads();
+        }
+        return !emptied;
+    }
+
+    __device__ bool slot_empty(uint32_t slot) {
+        uint64_t any = 0;
+        for (uint32_t w = lane; w < a.nw; w += kWave) any |= M[slot * a.nw + w];
+        return __ballot(any != 0) == 0;
+    }
+};
+
+// Status codes of one search.
+constexpr uint32_t kSolved = 0, kExhausted = 1, kLimit = 2, kBadPrefix = 3, kInternal = 4;
+
+__global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    Wave wv(a, lds, blockIdx.x);
+    while (true) {
+        uint32_t item = 0;
+        if (wv.lane == 0) item = atomicAdd(a.next_item, 1u);
+        item = __shfl(item, 0);
+        if (item >= a.n_items) break;
+        wv.init();
+        uint32_t status = kExhausted;
+        bool ok = true;
+        for (uint32_t m = 0; m < a.prefix_len && ok; ++m) {
+            const uint32_t mv = a.prefixes[(size_t)item * a.prefix_len + m];
+            const uint32_t pos = mv & 0xFF, piece = (mv >> 8) & 0xFF, o = (mv >> 16) & 3;
+            ok = pos < a.P && piece < a.P && wv.is_free(pos) && wv.piece_avail(piece);
+            if (ok) {
+                const uint32_t slot = wv.slot_of_pos[pos];
+                const uint32_t bit = 4 * piece + o;
+                ok = (wv.M[slot * a.nw + (bit >> 6)] >> (bit & 63)) & 1;
+                if (ok) ok = wv.apply(slot, bit);
+            }
+        }
+        uint64_t pushes = 0;
+        if (!ok) status = kBadPrefix;
+        else {
+            wv.base = wv.level;
+            bool new_level = true;
+            uint32_t pop_slot = 0;
+            status = kSolved;
+            while (wv.level < a.S) {
+                if (!new_level && wv.slot_empty(pop_slot)) {
+                    if (wv.level == wv.base) { status = kExhausted; break; }
+                    pop_slot = wv.pop();
+                    new_level = false;
+                    continue;
+                }
+                if (!wv.fixpoint()) {
+                    if (wv.level == wv.base) { status = kExhausted; break; }
+                    pop_slot = wv.pop();
+                    new_level = false;
+                    continue;
+                }
+                const uint32_t b = wv.best();
+                if (b == 0xFFFFFFFFu || !wv.apply(b >> 16, b & 0xFFFF)) { status = kInternal; break; }
+                ++pushes;
+                new_level = true;
+                if (a.push_budget && pushes >= a.push_budget && wv.level < a.S) { status = kLimit; break; }
+            }
+        }
+        // Result: the full board when solved, else the deepest situation.
+        const bool full = status == kSolved && wv.level == a.S;
+        uint16_t *board = a.out_board + (size_t)item * a.P;
+        for (uint32_t q = wv.lane; q < a.P; q += kWave) board[q] = 0xFFFF;
+        __syncthreads();
+        const uint32_t cnt = full ? a.S : wv.max;
+        for (uint32_t k = wv.lane; k < cnt; k += kWave) {
+            const uint32_t pl = full ? wv.played[k] : wv.max_played[k];
+            board[pl & 0xFF] = (uint16_t)((((pl >> 8) & 0xFF) << 2) | ((pl >> 16) & 3));
+        }
+        if (wv.lane == 0) {
+            a.out_max_depth[item] = wv.max;
+            a.out_pushes[item] = pushes;
+            a.out_status[item] = status;
+        }
+        __syncthreads();
+    }
+}
+
+#define GL_CHECK(call)                                                                    \
+    do {                                                                                  \
+        hipError_t err_ = (call);                                                         \
+        if (err_ != hipSuccess)                                                           \
+            throw DeviceError(std::string(#call) + ": " + hipGetErrorString(err_));       \
+    } while (0)
+
+}  // namespace
+
+GluttonTables build_glutton_tables(const Puzzle &pz) {
+    GluttonTables t;
+    t.P = pz.size();
+    t.W = pz.width();
+    t.nw = (4 * t.P + 63) / 64;
+    t.max_colour = pz.max_colour();
+    t.cm.assign((size_t)t.max_colour * 4 * t.nw, 0);
+    t.piece_col.assign(t.P, 0);
+    t.init_masks.assign((size_t)t.P * t.nw, 0);
+    auto set_bit = [&](uint64_t *m, uint32_t piece, uint32_t o) {
+        const uint32_t b = 4 * piece + o;
+        m[b >> 6] |= 1ull << (b & 63);
+    };
+    for (uint32_t p = 0; p < t.P; ++p) {
+        const Piece &pc = pz.piece(p);
+        t.piece_col[p] = pc.colour[0] | pc.colour[1] << 8 | pc.colour[2] << 16 | pc.colour[3] << 24;
+        // Colour masks (include/CUDA_common_glutton.h:109-146): entry (c, side)
+        // holds (piece, o) whose side opposite to `side` shows c.
+        for (uint32_t o = 0; o < 4; ++o)
+            for (uint32_t side = 0; side < 4; ++side) {
+                const uint32_t c = shown(pc, (side + 2) & 3, o);
+                if (c) set_bit(&t.cm[((size_t)(c - 1) * 4 + side) * t.nw], p, o);
+            }
+    }
+    // Initial capability (include/CUDA_common_glutton.h:148-211).
+    const uint32_t W = pz.width(), H = pz.height();
+    for (uint32_t pos = 0; pos < t.P; ++pos) {
+        const uint32_t x = pos % W, y = pos / W;
+        uint64_t *m = &t.init_masks[(size_t)pos * t.nw];
+        const int kind = pz.position_kind(x, y);
+        if (kind == kCorner) {
+            // corners (0,0),(W-1,0),(W-1,H-1),(0,H-1): frame sides (N,W),(E,N),(S,E),(W,S)
+            const unsigned ci = (x == 0 && y == 0) ? 0 : (y == 0 ? 1 : (x != 0 ? 2 : 3));
+            const unsigned b1 = ci, b2 = (ci + 3) & 3;
+            for (uint32_t idx : pz.pieces_of(kCorner))
+                for (unsigned o = 0; o < 4; ++o)
+                    if (shown(pz.piece(idx), b1, o) == 0 && shown(pz.piece(idx), b2, o) == 0) {
+                        set_bit(m, idx, o);
+                        break;
+                    }
+        } else if (kind == kBorder) {
+            const unsigned side = y == 0 ? 0 : (x == W - 1 ? 1 : (y == H - 1 ? 2 : 3));
+            for (uint32_t idx : pz.pieces_of(kBorder)) {
+                unsigned zero = 0;
+                while (pz.piece(idx).colour[zero]) ++zero;
+                set_bit(m, idx, (zero + 4 - side) & 3);
+            }
+        } else {
+            for (uint32_t idx : pz.pieces_of(kCenter))
+                for (unsigned o = 0; o < 4; ++o) set_bit(m, idx, o);
+        }
+    }
+    return t;
+}
+
+void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &pz,
+                       const std::vector<uint32_t> &prefixes, uint32_t prefix_len, uint32_t n_items,
+                       const GluttonConfig &cfg, std::vector<GluttonOutcome> &out, GluttonStats &stats) {
+    if (pz.size() > 256) throw Unsupported("glutton_max handles at most 256 pieces");
+    if ((uint64_t)prefixes.size() != (uint64_t)prefix_len * n_items)
+        throw std::invalid_argument("prefix array size does not match n_items * prefix_len");
+    auto wall0 = std::chrono::steady_clock::now();
+    GluttonTables t = build_glutton_tables(pz);
+    const uint32_t S = t.P;
+    const GlLayout L = GlLayout::make(S, t.P, t.nw);
+    if (L.total > dev.lds_per_block) throw Unsupported("glutton LDS layout exceeds the device limit");
+    const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu
+                                                   : std::max<uint32_t>(1, std::min<uint32_t>(8, (uint32_t)(160 * 1024 / L.total)));
+    const uint32_t waves = std::max<uint32_t>(1, std::min<uint32_t>(dev.compute_units * waves_per_cu, std::max<uint32_t>(n_items, 1)));
+    const size_t tri = (size_t)S * (S + 1) / 2 * t.nw;
+
+    uint64_t *d_cm = ws.get<uint64_t>("gl_cm", t.cm.size());
+    uint32_t *d_pcol = ws.get<uint32_t>("gl_pcol", t.piece_col.size());
+    uint64_t *d_init = ws.get<uint64_t>("gl_init", t.init_masks.size());
+    uint32_t *d_pre = ws.get<uint32_t>("gl_prefixes", std::max<size_t>(prefixes.size(), 1));
+    uint64_t *d_stacks = ws.get<uint64_t>("gl_stacks", tri * waves);
+    uint32_t *d_maxp = ws.get<uint32_t>("gl_max_played", (size_t)S * waves);
+    unsigned int *d_next = ws.get<unsigned int>("gl_next", 2);
+    uint32_t *d_depth = ws.get<uint32_t>("gl_depth", n_items);
+    uint64_t *d_pushes = ws.get<uint64_t>("gl_pushes", n_items);
+    uint32_t *d_status = ws.get<uint32_t>("gl_status", n_items);
+    uint16_t *d_board = ws.get<uint16_t>("gl_board", (size_t)n_items * t.P);
+    unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 1);
+    GL_CHECK(hipMemcpyAsync(d_cm, t.cm.data(), t.cm.size() * 8, hipMemcpyHostToDevice, stream));
+    GL_CHECK(hipMemcpyAsync(d_pcol, t.piece_col.data(), t.piece_col.size() * 4, hipMemcpyHostToDevice, stream));
+    GL_CHECK(hipMemcpyAsync(d_init, t.init_masks.data(), t.init_masks.size() * 8, hipMemcpyHostToDevice, stream));
+    if (!prefixes.empty())
+        GL_CHECK(hipMemcpyAsync(d_pre, prefixes.data(), prefixes.size() * 4, hipMemcpyHostToDevice, stream));
+    const unsigned int zero2[2] = {0, 0};
+    const unsigned long long zero64 = 0;
+    GL_CHECK(hipMemcpyAsync(d_next, zero2, sizeof(zero2), hipMemcpyHostToDevice, stream));
+    GL_CHECK(hipMemcpyAsync(d_evals, &zero64, 8, hipMemcpyHostToDevice, stream));
+
+    GlArgs a{};
+    a.P = t.P;
+    a.W = t.W;
+    a.S = S;
+    a.nw = t.nw;
+    a.max_colour = t.max_colour;
+    a.cm = d_cm;
+    a.piece_col = d_pcol;
+    a.init_masks = d_init;
+    a.prefixes = d_pre;
+    a.n_items = n_items;
+    a.prefix_len = prefix_len;
+    a.push_budget = cfg.push_budget;
+    a.stacks = d_stacks;
+    a.max_played = d_maxp;
+    a.next_item = d_next;
+    a.out_max_depth = d_depth;
+    a.out_pushes = d_pushes;
+    a.out_status = d_status;
+    a.out_board = d_board;
+    a.evals = d_evals;
+
+    hipEvent_t ev0, ev1;
+    GL_CHECK(hipEventCreate(&ev0));
+    GL_CHECK(hipEventCreate(&ev1));
+    GL_CHECK(hipEventRecord(ev0, stream));
+    hipLaunchKernelGGL(glutton_kernel, dim3(waves), dim3(kWave), L.total, stream, a);
+    GL_CHECK(hipGetLastError());
+    GL_CHECK(hipEventRecord(ev1, stream));
+    GL_CHECK(hipStreamSynchronize(stream));
+    float ms = 0;
+    GL_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+
+    std::vector<uint32_t> depth(n_items), status(n_items);
+    std::vector<uint64_t> pushes(n_items);
+    std::vector<uint16_t> boards((size_t)n_items * t.P);
+    unsigned long long evals = 0;
+    GL_CHECK(hipMemcpyAsync(depth.data(), d_depth, n_items * 4, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipMemcpyAsync(status.data(), d_status, n_items * 4, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipMemcpyAsync(pushes.data(), d_pushes, n_items * 8, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipMemcpyAsync(boards.data(), d_board, boards.size() * 2, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipMemcpyAsync(&evals, d_evals, 8, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipStreamSynchronize(stream));
+    out.assign(n_items, GluttonOutcome{});
+    stats = GluttonStats{};
+    for (uint32_t i = 0; i < n_items; ++i) {
+        out[i].max_depth = depth[i];
+        out[i].pushes = pushes[i];
+        out[i].status = status[i];
+        out[i].board.assign(boards.begin() + (size_t)i * t.P, boards.begin() + (size_t)(i + 1) * t.P);
+        stats.pushes += pushes[i];
+    }
+    stats.kernel_ms = ms;
+    stats.evaluations = evals;
+    stats.waves = waves;
+    stats.lds_bytes = L.total;
+    stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+}  // namespace emp

@@ -146,6 +146,61 @@ int emp_hip_enumerate(emp_hip_context *context,
                       uint16_t *const *boards,
                       emp_hip_enum_stats *stats);
 
+/* ------------------------------------------------------------------------ *
+ * glutton_max (include/CUDA_glutton_max.h:1522-1637): best-first search
+ * that removes every move leaving a free position or an available piece
+ * without option, plays the best-scored move, and backtracks when a level
+ * runs dry.  One search per prefix; each prefix (prefix_len moves packed as
+ * position | piece_index << 8 | orientation << 16, position row-major, piece
+ * index = id - 1) is played before the search, which never undoes it.
+ * Replaces: launch_CUDA_glutton_max (src/CUDA_glutton_max.cu:38-108), which
+ * runs one search from the empty board (prefix_len = 0, n_searches = 1).
+ * At most 256 pieces (include/CUDA_common_struct_glutton.h:358).
+ * ------------------------------------------------------------------------ */
+#define EMP_HIP_GLUTTON_SOLVED     0  /* stack full: boards[i] is a solution  */
+#define EMP_HIP_GLUTTON_EXHAUSTED  1  /* no move left above the prefix        */
+#define EMP_HIP_GLUTTON_BUDGET     2  /* stopped after push_budget pushes     */
+#define EMP_HIP_GLUTTON_BAD_PREFIX 3  /* a prefix move is not playable        */
+
+typedef struct {
+    uint64_t push_budget;      /* pushes per search, 0 = unlimited          */
+    uint32_t waves_per_cu;     /* 0 = as many as LDS allows                 */
+    uint32_t reserved;
+} emp_hip_glutton_options;
+
+typedef struct {
+    uint32_t max_depth;        /* deepest level: pieces on the board        */
+    uint32_t status;           /* EMP_HIP_GLUTTON_*                         */
+    uint64_t pushes;           /* pushes of the search (prefix excluded)    */
+} emp_hip_glutton_result;
+
+typedef struct {
+    double   total_ms;
+    double   kernel_ms;        /* HIP events around the glutton kernel      */
+    uint64_t pushes;
+    uint64_t evaluations;      /* candidate evaluations                      */
+    uint32_t waves;
+    uint32_t lds_bytes;        /* LDS per wave                               */
+} emp_hip_glutton_stats;
+
+/* boards: [n_searches][P], the solution or the deepest situation reached
+ * (the stack's max played info).  stats may be NULL. */
+int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle,
+                        const uint32_t *prefixes, uint32_t prefix_len, uint32_t n_searches,
+                        const emp_hip_glutton_options *options,
+                        emp_hip_glutton_result *results, uint16_t *boards,
+                        emp_hip_glutton_stats *stats);
+
+/* Seeded prefixes for glutton searches: each row is the n_fixed given moves
+ * followed by one placement of each of the first `length` positions of the
+ * spiral order, drawn by random dives that only place matching pieces
+ * (distinct rows).  out: [count][n_fixed + length] packed moves; *n_out rows
+ * were produced.  No counterpart in the reference (its glutton_wide level
+ * expansion, include/CUDA_glutton_situations.h:713-757, enumerates instead). */
+int emp_hip_spiral_prefixes(const emp_hip_puzzle *puzzle, uint32_t length, uint32_t count,
+                            uint64_t seed, const uint32_t *fixed_moves, uint32_t n_fixed,
+                            uint32_t *out, uint32_t *n_out);
+
 #ifdef __cplusplus
 }
 #endif

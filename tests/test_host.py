@@ -110,3 +110,29 @@ def test_null_context_is_an_argument_error():
     with pytest.raises(emp.EmpError) as e:
         emp._check(emp.lib().emp_hip_enumerate(None, None, 0, None, None, None, None))
     assert e.value.status == emp.BAD_ARGUMENT
+
+
+def test_spiral_prefixes_are_valid_and_distinct():
+    w = h = 8
+    c, planted = emp.generate(5, w, h, 4, 8)
+    hint = emp.pack_move(36, int(planted[36]) >> 2, int(planted[36]) & 3)
+    rows = emp.spiral_prefixes(w, h, c, 10, 50, 9, fixed=[hint])
+    assert len(rows) == 50 and len({tuple(r) for r in rows}) == 50
+    order = emp.order(w, h)
+    for row in rows:
+        assert row[0] == hint
+        cells = {}
+        for k, m in enumerate(row[1:]):
+            pos, piece, ori = int(m) & 0xFF, (int(m) >> 8) & 0xFF, (int(m) >> 16) & 3
+            assert pos == order[k][1] * w + order[k][0]
+            cells[pos] = [int(c[piece][(s + ori) % 4]) for s in range(4)]
+        pieces = [(int(m) >> 8) & 0xFF for m in row]
+        assert len(set(pieces)) == len(pieces)
+        for pos, look in cells.items():
+            x, y = pos % w, pos // w
+            assert (look[0] == 0) == (y == 0) and (look[3] == 0) == (x == 0)
+            assert (look[1] == 0) == (x == w - 1) and (look[2] == 0) == (y == h - 1)
+            if pos + 1 in cells and x + 1 < w:
+                assert look[1] == cells[pos + 1][3]
+            if pos + w in cells:
+                assert look[2] == cells[pos + w][0]
