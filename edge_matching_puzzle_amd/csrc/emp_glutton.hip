@@ -471,8 +471,8 @@ __global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
         if (item >= a.n_items) break;
         wv.init();
         uint32_t status = kExhausted;
-        bool ok = true;
-        for (uint32_t m = 0; m < a.prefix_len && ok; ++m) {
+        bool ok = true, dead = false;
+        for (uint32_t m = 0; m < a.prefix_len && ok && !dead; ++m) {
             const uint32_t mv = a.prefixes[(size_t)item * a.prefix_len + m];
             const uint32_t pos = mv & 0xFF, piece = (mv >> 8) & 0xFF, o = (mv >> 16) & 3;
             ok = pos < a.P && piece < a.P && wv.is_free(pos) && wv.piece_avail(piece);
@@ -480,11 +480,14 @@ __global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
                 const uint32_t slot = wv.slot_of_pos[pos];
                 const uint32_t bit = 4 * piece + o;
                 ok = (wv.M[slot * a.nw + (bit >> 6)] >> (bit & 63)) & 1;
-                if (ok) ok = wv.apply(slot, bit);
+                // A playable move that leaves a free cell without option ends
+                // the search at the depth reached.
+                if (ok) dead = !wv.apply(slot, bit);
             }
         }
         uint64_t pushes = 0;
         if (!ok) status = kBadPrefix;
+        else if (dead) status = kExhausted;
         else {
             wv.base = wv.level;
             bool new_level = true;

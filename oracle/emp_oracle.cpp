@@ -744,14 +744,17 @@ struct Glutton {
     // Prefix seeding: each move is applied like a chosen candidate (the
     // reference's prepare_stack skips the colour constraints of pieces
     // already on the board, include/CUDA_glutton_max.h:136-152).
-    bool seed(uint32_t pos, uint32_t piece, uint32_t o) {
-        if (pos >= P || piece >= P || o > 3 || !is_free(pos) || !avail[piece]) return false;
+    // Returns 0 when played, 1 when the move is not playable (occupied cell,
+    // used piece, or not a candidate of the cell), 2 when it was played but
+    // left a free cell without option (the search then ends at once).
+    int seed(uint32_t pos, uint32_t piece, uint32_t o) {
+        if (pos >= P || piece >= P || o > 3 || !is_free(pos) || !avail[piece]) return 1;
         uint32_t slot = slot_of_pos[pos];
         uint32_t bit = 4 * piece + o;
-        if (!(cur(slot)[bit / 64] >> (bit % 64) & 1)) return false;
-        if (!apply(slot, bit)) return false;
+        if (!(cur(slot)[bit / 64] >> (bit % 64) & 1)) return 1;
+        if (!apply(slot, bit)) return 2;
         base = level;
-        return true;
+        return 0;
     }
 
     // kernel (include/CUDA_glutton_max.h:1522-1637)
@@ -904,11 +907,14 @@ int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
     if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
     if (m.P > 256) return ORACLE_BAD_ARGUMENT;   // include/CUDA_common_struct_glutton.h:358
     Glutton g(m);
-    for (uint32_t i = 0; i < n_prefix; ++i)
-        if (!g.seed(prefix_moves[3 * i], prefix_moves[3 * i + 1], prefix_moves[3 * i + 2]))
-            return ORACLE_BAD_ARGUMENT;
+    bool dead = false;
+    for (uint32_t i = 0; i < n_prefix && !dead; ++i) {
+        int r = g.seed(prefix_moves[3 * i], prefix_moves[3 * i + 1], prefix_moves[3 * i + 2]);
+        if (r == 1) return ORACLE_BAD_ARGUMENT;
+        dead = r == 2;
+    }
     uint64_t pushes = 0;
-    int status = g.run(push_limit, pushes);
+    int status = dead ? ORACLE_OK : g.run(push_limit, pushes);
     bool full = g.level == g.S;
     if (board) {
         std::fill(board, board + m.P, 0xFFFF);

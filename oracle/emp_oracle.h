@@ -96,7 +96,9 @@ int oracle_order(uint32_t width, uint32_t height, int order, uint32_t *xy);
  *   prefix_moves    n_prefix moves (position, piece_index, orientation),
  *                   played through the same move application as the search
  *                   itself before the search starts; the search never pops
- *                   below them.
+ *                   below them.  An unplayable move is ORACLE_BAD_ARGUMENT; a
+ *                   move that leaves a free cell without option ends the
+ *                   search there (not solved, no push).
  *   push_limit      stop after that many pushes (0 = unlimited).
  *   board           [P] final board: the full solution when *solved, else the
  *                   deepest situation seen (the stack's max played info).
