@@ -65,7 +65,7 @@ struct GlArgs {
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
 struct GlLayout {
-    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, played, avail, scratch, total;
+    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, played, avail, scratch, total;
     __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw) {
         GlLayout l;
         uint32_t o = 0;
@@ -76,6 +76,8 @@ struct GlLayout {
         l.pc = take(S * 2);
         l.sp = take(S * 2);
         l.T = take(256 * 2);
+        l.single = take(256 * 4);
+        l.crit = take(256 * 2);
         l.played = take(S * 4);
         l.avail = take(4 * 8);
         l.scratch = take(64);
@@ -91,7 +93,8 @@ __device__ __forceinline__ uint32_t nib(uint64_t v, uint32_t q) {
 struct Wave {
     const GlArgs &a;
     uint64_t *M;
-    uint16_t *pos_of_slot, *slot_of_pos, *pc, *sp, *T;
+    uint16_t *pos_of_slot, *slot_of_pos, *pc, *sp, *T, *crit;
+    uint32_t *single;   // per piece: free slots holding only that piece
     uint32_t *played;
     uint64_t *avail;
     uint32_t *scratch;
@@ -109,6 +112,8 @@ struct Wave {
         pc = reinterpret_cast<uint16_t *>(lds + L.pc);
         sp = reinterpret_cast<uint16_t *>(lds + L.sp);
         T = reinterpret_cast<uint16_t *>(lds + L.T);
+        single = reinterpret_cast<uint32_t *>(lds + L.single);
+        crit = reinterpret_cast<uint16_t *>(lds + L.crit);
         played = reinterpret_cast<uint32_t *>(lds + L.played);
         avail = reinterpret_cast<uint64_t *>(lds + L.avail);
         scratch = reinterpret_cast<uint32_t *>(lds + L.scratch);
@@ -155,6 +160,8 @@ struct Wave {
         const uint32_t nn = n();
         uint32_t local = 0;
         bool empty = false;
+        for (uint32_t q = lane; q < 256; q += kWave) single[q] = 0;
+        __syncthreads();
         for (uint32_t j = lane; j < nn; j += kWave) {
             uint32_t cnt = 0, pieces = 0, first = 0xFFFF;
             for (uint32_t w = 0; w < a.nw; ++w) {
@@ -168,15 +175,28 @@ struct Wave {
             }
             pc[j] = (uint16_t)cnt;
             sp[j] = pieces == 1 ? (uint16_t)first : (uint16_t)0xFFFF;
+            if (pieces == 1) atomicAdd(&single[first], 1u);
             local += cnt;
             empty |= cnt == 0;
         }
-        for (uint32_t q = lane; q < a.P; q += kWave) {
+        // Per-piece totals, and the list of available pieces with at most 20
+        // options: a move removes at most 4 options from each of 5 cells, so
+        // only those pieces can be left without option.
+        uint32_t n_crit = 0;
+        for (uint32_t base = 0; base < a.P; base += kWave) {
+            const uint32_t q = base + lane;
             uint32_t t = 0;
-            const uint32_t w = q >> 4;
-            for (uint32_t j = 0; j < nn; ++j) t += nib(M[j * a.nw + w], q);
-            T[q] = (uint16_t)t;
+            if (q < a.P) {
+                const uint32_t w = q >> 4;
+                for (uint32_t j = 0; j < nn; ++j) t += nib(M[j * a.nw + w], q);
+                T[q] = (uint16_t)t;
+            }
+            const bool c = q < a.P && t <= 20 && piece_avail(q);
+            const uint64_t m = __ballot(c);
+            if (c) crit[n_crit + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = (uint16_t)q;
+            n_crit += (uint32_t)__popcll(m);
         }
+        if (lane == 0) scratch[0] = n_crit;
         for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
         PC = local;
         const bool any_empty = __ballot(empty) != 0;
@@ -221,15 +241,34 @@ struct Wave {
                 mx = rp > mx ? rp : mx;
             }
         }
-        // Other free slots: invalid when piece p is all they hold.
-        for (uint32_t j = 0; j < nn; ++j) {
-            if (j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3]) continue;
-            if (sp[j] == p) return false;
-            if (score) {
+        // Other free slots: invalid when piece p is all one of them holds.
+        uint32_t own_single = sp[i] == p;
+        for (uint32_t l = 0; l < 4; ++l) own_single += nb[l] != 0xFFFFFFFFu && sp[nb[l]] == p;
+        if (single[p] > own_single) return false;
+        if (score) {
+            for (uint32_t j = 0; j < nn; ++j) {
+                if (j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3]) continue;
                 const uint32_t v = pc[j] - nib(M[j * a.nw + pw], p);
                 mn = v < mn ? v : mn;
                 mx = v > mx ? v : mx;
             }
+        }
+        if (!score) {
+            // Only pieces with few options can be left without one.
+            const uint32_t nc = scratch[0];
+            for (uint32_t k = 0; k < nc; ++k) {
+                const uint32_t q = crit[k];
+                if (q == p) continue;
+                const uint32_t w = q >> 4;
+                uint32_t d = nib(M[i * a.nw + w], q);
+                for (uint32_t l = 0; l < 4; ++l) {
+                    if (nb[l] == 0xFFFFFFFFu) continue;
+                    const uint64_t mj = M[nb[l] * a.nw + w];
+                    d += nib(mj, q) - nib(mj & cmp[l][w], q);
+                }
+                if ((uint32_t)T[q] == d) return false;
+            }
+            return true;
         }
         // Option counts of the other available pieces.
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
