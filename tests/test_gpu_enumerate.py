@@ -153,3 +153,18 @@ def test_batch_of_uneven_sizes(gpu_context):
     res, _ = gpu_context.enumerate(puzzles, max_solutions=4096)
     for (w, h, c), r in zip(puzzles, res):
         _check_same(r, o.enumerate_(w, h, c, max_solutions=4096))
+
+
+def test_c1_full_size_against_golden(gpu_context, golden_dir):
+    """The bench workload at full size (8x8, 4 border / 8 interior colours,
+    seed 7: 3.7e11 nodes, about 15 s on the GPU) against the oracle's result
+    from tests/golden/make_c1_golden.py (1.6e3 s on 7 CPU cores)."""
+    import json
+    g = json.load(open(os.path.join(golden_dir, "c1_8x8_seed7.json")))
+    c, _ = emp.generate(7, 8, 8, 4, 8)
+    print("case: C1 full size", flush=True)
+    (r,), st = gpu_context.enumerate([(8, 8, c)], max_solutions=64)
+    assert r["complete"]
+    assert r["n_nodes"] == g["n_nodes"] == 366518800244
+    assert r["n_solutions"] == g["n_solutions"] == 2
+    assert sorted(emp.situation_string(8, 8, b) for b in r["boards"]) == g["solutions"]
