@@ -87,6 +87,7 @@ SIGNATURES = {
     "emp_hip_situation_string": (_I32, [_U32, _U32, _P, ctypes.c_char_p, _U64]),
     "emp_hip_enum_default_options": (None, [_P]),
     "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+    "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
 }
@@ -227,6 +228,17 @@ class Context:
                                          ctypes.byref(opt), ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
         results = [{"max_depth": r.max_depth, "status": r.status, "pushes": r.pushes} for r in res]
         return results, boards, {f: getattr(st, f) for f, _ in GluttonStats._fields_}
+
+    def expand(self, width, height, colours, depth, order="spiral", capacity=1 << 20):
+        """Situations of the enumeration with `depth` pieces placed; returns
+        (boards [n, P] uint16, n_situations, nodes_above, solutions_above)."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        boards = np.zeros((capacity, width * height), dtype=np.uint16)
+        n, nodes, sols = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().emp_hip_expand(self._ctx, ctypes.byref(pz), _ORDERS[order], depth, capacity, _ptr(boards),
+                                    ctypes.byref(n), ctypes.byref(nodes), ctypes.byref(sols)))
+        return boards[:min(n.value, capacity)].copy(), n.value, nodes.value, sols.value
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,

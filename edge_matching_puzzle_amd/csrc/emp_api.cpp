@@ -237,6 +237,27 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
     });
 }
 
+int emp_hip_expand(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint32_t order, uint32_t depth,
+                   uint64_t capacity, uint16_t *boards, uint64_t *n_situations, uint64_t *n_nodes,
+                   uint64_t *n_solutions) {
+    return guarded([&] {
+        if (!context || !puzzle || !n_situations || (capacity && !boards))
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC) return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        std::vector<uint16_t> out;
+        uint64_t items = 0, nodes = 0, sols = 0;
+        emp::expand_on_device(context->dev, context->stream, context->workspace, pz, (int)order, depth, capacity, out,
+                              items, nodes, sols);
+        if (capacity) std::copy(out.begin(), out.end(), boards);
+        *n_situations = items;
+        if (n_nodes) *n_nodes = nodes;
+        if (n_solutions) *n_solutions = sols;
+        return EMP_HIP_OK;
+    });
+}
+
 int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *prefixes,
                         uint32_t prefix_len, uint32_t n_searches, const emp_hip_glutton_options *options,
                         emp_hip_glutton_result *results, uint16_t *boards, emp_hip_glutton_stats *stats) {

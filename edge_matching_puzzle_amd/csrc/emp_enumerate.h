@@ -86,6 +86,14 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
                          std::vector<EnumOutcome> &out, EnumStats &stats);
 
+// Breadth expansion (glutton_wide's play_from applied level by level,
+// include/CUDA_glutton_situations.h:713-757): every situation the DFS visits
+// with `depth` pieces placed, as row-major boards (up to `capacity`).  nodes
+// and solutions count the placements of indices 0..depth-1.
+void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &puzzle, int order,
+                      uint32_t depth, uint64_t capacity, std::vector<uint16_t> &boards, uint64_t &n_items,
+                      uint64_t &nodes, uint64_t &solutions);
+
 }  // namespace emp
 
 #endif

@@ -780,4 +780,87 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }
 
+void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &pz, int order,
+                      uint32_t depth, uint64_t capacity, std::vector<uint16_t> &boards, uint64_t &n_items,
+                      uint64_t &nodes, uint64_t &solutions) {
+    (void)dev;
+    const EnumTable t = build_enum_table(pz, order);
+    const EnumTableHeader h = t.header();
+    if (depth < 1 || depth > h.P) throw std::invalid_argument("expansion depth must be in 1..P");
+    uint32_t max_words = 1;
+    for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
+    const int maxw = round_words(max_words);
+    const uint32_t stride = (h.P + 3) & ~3u;
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(capacity, 1), 1u << 25);
+    uint64_t *d_tables = ws.get<uint64_t>("x_tables", h.total_words);
+    uint32_t *d_off = ws.get<uint32_t>("x_off", 1);
+    uint64_t *d_hdr[2] = {ws.get<uint64_t>("x_hdr0", cap), ws.get<uint64_t>("x_hdr1", cap)};
+    uint16_t *d_stack[2] = {ws.get<uint16_t>("x_stack0", (size_t)cap * stride),
+                            ws.get<uint16_t>("x_stack1", (size_t)cap * stride)};
+    unsigned long long *d_nodes = ws.get<unsigned long long>("x_nodes", 1);
+    unsigned long long *d_sols = ws.get<unsigned long long>("x_sols", 1);
+    unsigned int *d_counters = ws.get<unsigned int>("x_counters", 8);
+    uint16_t *d_sol_stack = ws.get<uint16_t>("x_sol_stack", stride);
+    const uint32_t zero_off = 0;
+    const unsigned long long one = 1, zero = 0;
+    const uint64_t root_hdr = (1ull << 20) | (1ull << 31);
+    std::vector<uint16_t> root(stride, 0);
+    root[0] = (uint16_t)h.root;
+    EMP_CHECK(hipMemcpyAsync(d_tables, t.blob.data(), t.blob.size() * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_off, &zero_off, 4, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_hdr[0], &root_hdr, 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_stack[0], root.data(), stride * 2, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_nodes, &one, 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_sols, &zero, 8, hipMemcpyHostToDevice, stream));
+    DfsArgs a{};
+    a.tables = d_tables;
+    a.table_off = d_off;
+    a.stride = stride;
+    a.nodes = d_nodes;
+    a.solutions = d_sols;
+    a.out_count = d_counters + 1;
+    a.flags = d_counters + 2;
+    a.sol_stack = d_sol_stack;
+    a.sol_per_puzzle = 0;
+    int cur = 0;
+    uint64_t items = 1;
+    for (uint32_t level = 1; level < depth && items; ++level) {
+        a.in_hdr = d_hdr[cur];
+        a.in_stack = d_stack[cur];
+        a.n_in = (uint32_t)items;
+        a.out_hdr = d_hdr[1 - cur];
+        a.out_stack = d_stack[1 - cur];
+        a.out_cap = cap;
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
+        dispatch_expand(maxw, a, stream);
+        EMP_CHECK(hipGetLastError());
+        unsigned int counters[8];
+        EMP_CHECK(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+        if (counters[2]) throw OutOfMemory("expansion exceeds the capacity of " + std::to_string(cap) + " situations");
+        items = counters[1];
+        cur = 1 - cur;
+    }
+    unsigned long long r_nodes = 0, r_sols = 0;
+    EMP_CHECK(hipMemcpyAsync(&r_nodes, d_nodes, 8, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipMemcpyAsync(&r_sols, d_sols, 8, hipMemcpyDeviceToHost, stream));
+    std::vector<uint16_t> stacks((size_t)items * stride);
+    if (items) EMP_CHECK(hipMemcpyAsync(stacks.data(), d_stack[cur], stacks.size() * 2, hipMemcpyDeviceToHost, stream));
+    EMP_CHECK(hipStreamSynchronize(stream));
+    nodes = r_nodes;
+    solutions = r_sols;
+    n_items = items;
+    const uint64_t keep = std::min<uint64_t>(items, capacity);
+    boards.assign((size_t)keep * h.P, 0xFFFF);
+    for (uint64_t i = 0; i < keep; ++i) {
+        const uint16_t *st = &stacks[(size_t)i * stride];
+        uint16_t *bd = &boards[(size_t)i * h.P];
+        for (uint32_t k = 0; k < depth && k < h.P; ++k) {
+            const int kind = (int)(t.blob[h.off_pos + k] & 3);
+            bd[t.order[k].second * h.W + t.order[k].first] =
+                (uint16_t)(((uint32_t)t.piece_of[kind][st[k] >> 2] << 2) | (st[k] & 3));
+        }
+    }
+}
+
 }  // namespace emp

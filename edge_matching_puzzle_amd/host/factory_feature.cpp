@@ -1,0 +1,24 @@
+#include "factory_feature.h"
+
+#include "feature_CUDA_backtracker.h"
+#include "feature_CUDA_glutton_max.h"
+#include "feature_CUDA_glutton_wide.h"
+
+namespace edge_matching_puzzle {
+
+std::unique_ptr<feature_if> factory_feature::create_feature(const std::string &p_name,
+                                                            const puzzle_description &p_puzzle,
+                                                            const std::string &p_strategy,
+                                                            const std::string &p_hint) {
+    if (p_name == "CUDA_backtracker")
+        return std::unique_ptr<feature_if>(new feature_CUDA_backtracker(p_puzzle, p_strategy));
+    if (p_name == "CUDA_glutton_max") return std::unique_ptr<feature_if>(new feature_CUDA_glutton_max(p_puzzle));
+    if (p_name == "CUDA_glutton_wide") {
+        unsigned int depth = 1;
+        if (!p_hint.empty()) depth = (unsigned int)std::stoul(p_hint);
+        return std::unique_ptr<feature_if>(new feature_CUDA_glutton_wide(p_puzzle, depth));
+    }
+    throw logic_exception("Unsupported feature \"" + p_name + "\"", __LINE__, __FILE__);
+}
+
+}  // namespace edge_matching_puzzle

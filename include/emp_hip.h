@@ -146,6 +146,20 @@ int emp_hip_enumerate(emp_hip_context *context,
                       uint16_t *const *boards,
                       emp_hip_enum_stats *stats);
 
+/*
+ * Breadth expansion: every situation the enumeration visits with `depth`
+ * pieces placed (indices 0..depth-1 of the order), level by level on the GPU.
+ * boards: [capacity][P] row-major (cells beyond the depth empty); *n_situations
+ * is the full count (EMP_HIP_OUT_OF_MEMORY when it exceeds 2^25);
+ * *n_nodes / *n_solutions count the placements above the depth.
+ * Replaces: launch_CUDA_glutton_wide (src/CUDA_glutton_wide.cu:27-37), whose
+ * CPU form expands the (0,0) corner's candidates (src/CUDA_glutton_wide.cpp
+ * :29-67) with play_from (include/CUDA_glutton_situations.h:713-757).
+ */
+int emp_hip_expand(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint32_t order,
+                   uint32_t depth, uint64_t capacity, uint16_t *boards, uint64_t *n_situations,
+                   uint64_t *n_nodes, uint64_t *n_solutions);
+
 /* ------------------------------------------------------------------------ *
  * glutton_max (include/CUDA_glutton_max.h:1522-1637): best-first search
  * that removes every move leaving a free position or an available piece

@@ -1,0 +1,107 @@
+"""The C++ feature layer and command line (edge_matching_puzzle_amd/host)
+against the reference's own CLI test expectations (tests/golden/expected_stdout.json,
+collected from the reference's test.info files)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "edge_matching_puzzle_amd", "edge_matching_puzzle")
+PUZZLES = os.path.join(ROOT, "tests", "golden", "puzzles")
+
+
+def run(*args, timeout=120):
+    p = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=timeout)
+    return p.returncode, p.stdout
+
+
+def test_unknown_feature_is_an_error():
+    rc, out = run(os.path.join(PUZZLES, "3_3.txt"), "ressources", "no_such_feature")
+    assert rc == 255 and out.startswith('ERROR : Unsupported feature "no_such_feature"')
+
+
+def test_refused_puzzle_is_an_error(tmp_path):
+    bad = tmp_path / "bad.txt"
+    text = open(os.path.join(PUZZLES, "3_3.txt")).read().replace("5,4,4,3,3", "5,4,4,3,9")
+    bad.write_text(text)
+    rc, out = run(str(bad), "ressources", "CUDA_glutton_max")
+    assert rc == 255 and "multiple of 2" in out
+
+
+def test_missing_piece_is_an_error(tmp_path):
+    bad = tmp_path / "short.txt"
+    bad.write_text("\n".join(open(os.path.join(PUZZLES, "3_3.txt")).read().splitlines()[:-1]) + "\n")
+    rc, out = run(str(bad), "ressources", "CUDA_glutton_max")
+    assert rc == 255 and "ERROR" in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["3_3", "4_4", "5_5"])
+def test_glutton_max_feature_stdout(expected_stdout, name):
+    """Every expected_stdout_string of tests/CUDA_glutton_max_<name>/test.info."""
+    print("case: CLI CUDA_glutton_max", name, flush=True)
+    rc, out = run(os.path.join(PUZZLES, name + ".txt"), "ressources", "CUDA_glutton_max", "--enable_gui=false")
+    assert rc == 0, out
+    for s in expected_stdout["CUDA_glutton_max_" + name]["strings"]:
+        assert s in out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["3_3", "4_4", "5_5"])
+def test_backtracker_feature_lists_reference_solution(expected_stdout, name):
+    """The first solution the reference's CUDA backtracker prints
+    (tests_CUDA/backtracker_<name>/test.info, basic order) is among the
+    solutions the enumeration lists; totals match the oracle in basic order."""
+    import oracle_bindings as o
+    print("case: CLI CUDA_backtracker", name, flush=True)
+    rc, out = run(os.path.join(PUZZLES, name + ".txt"), "ressources", "CUDA_backtracker", "--enable_gui=false")
+    assert rc == 0, out
+    assert expected_stdout["backtracker_" + name]["strings"][-1] in out.splitlines()
+    w, h, c = o.parse_puzzle(os.path.join(PUZZLES, name + ".txt"))
+    ref = o.enumerate_(w, h, c, order=o.BASIC)
+    assert "Total situations explored : %d" % ref["n_nodes"] in out
+    assert "Nb solutions : %d" % ref["n_solutions"] in out
+
+
+@pytest.mark.gpu
+def test_glutton_wide_feature_levels():
+    import oracle_bindings as o
+    w, h, c = o.parse_puzzle(os.path.join(PUZZLES, "4_4.txt"))
+    rc, out = run(os.path.join(PUZZLES, "4_4.txt"), "ressources", "CUDA_glutton_wide", "--hint=5")
+    assert rc == 0, out
+    for depth in range(2, 6):
+        pre, _, _ = o.prefixes(w, h, c, depth)
+        assert "Level %d : %d situations" % (depth, len(pre)) in out
+
+
+def _kind_lists(c):
+    kinds = {0: [], 1: [], 2: []}
+    for p in range(len(c)):
+        kinds[int(sum(1 for v in c[p] if v == 0))].append(p)
+    return kinds
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,B,I,seed,depth", [(5, 3, 4, 1, 6), (6, 3, 6, 2, 9), (8, 4, 8, 7, 10)])
+def test_expand_matches_oracle_prefix_cut(gpu_context, w, B, I, seed, depth):
+    """GPU breadth expansion = the oracle's DFS cut at the same depth."""
+    import edge_matching_puzzle_amd as emp
+    import oracle_bindings as o
+    c, _ = emp.generate(seed, w, w, B, I)
+    print("case: expand", w, depth, flush=True)
+    boards, n, nodes, sols = gpu_context.expand(w, w, c, depth)
+    pre, onodes, osols = o.prefixes(w, w, c, depth)
+    assert n == len(pre) and nodes == onodes and sols == osols
+    order = o.order(w, w)
+    kinds = _kind_lists(c)
+    ref = set()
+    for row in pre:
+        b = [0xFFFF] * (w * w)
+        for k, t in enumerate(row):
+            x, y = order[k]
+            kind = 2 if (x in (0, w - 1) and y in (0, w - 1)) else (1 if (x in (0, w - 1) or y in (0, w - 1)) else 0)
+            b[y * w + x] = (kinds[kind][int(t) >> 2] << 2) | (int(t) & 3)
+        ref.add(tuple(b))
+    assert {tuple(int(v) for v in b) for b in boards} == ref
