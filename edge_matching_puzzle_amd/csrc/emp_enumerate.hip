@@ -791,7 +791,8 @@ void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, 
     for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
     const int maxw = round_words(max_words);
     const uint32_t stride = (h.P + 3) & ~3u;
-    const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(capacity, 1), 1u << 25);
+    // Device queues hold whole levels; `capacity` only bounds the boards returned.
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(capacity, 1u << 22), 1u << 25);
     uint64_t *d_tables = ws.get<uint64_t>("x_tables", h.total_words);
     uint32_t *d_off = ws.get<uint32_t>("x_off", 1);
     uint64_t *d_hdr[2] = {ws.get<uint64_t>("x_hdr0", cap), ws.get<uint64_t>("x_hdr1", cap)};
