@@ -87,6 +87,7 @@ SIGNATURES = {
     "emp_hip_situation_string": (_I32, [_U32, _U32, _P, ctypes.c_char_p, _U64]),
     "emp_hip_enum_default_options": (None, [_P]),
     "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+    "emp_hip_backtrack_trace": (_I32, [_P, _P, _U32, _U64, _P, _P, _P, _U64, _P]),
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
@@ -228,6 +229,20 @@ class Context:
                                          ctypes.byref(opt), ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
         results = [{"max_depth": r.max_depth, "status": r.status, "pushes": r.pushes} for r in res]
         return results, boards, {f: getattr(st, f) for f, _ in GluttonStats._fields_}
+
+    def backtrack_trace(self, width, height, colours, order="basic", max_selections=0, trace_capacity=1 << 16):
+        """The reference's CUDA backtracker: first solution and the selection
+        trace as (lane, variable, step) rows."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        board = np.zeros(width * height, dtype=np.uint16)
+        trace = np.zeros((trace_capacity, 3), dtype=np.uint32)
+        found, n = ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().emp_hip_backtrack_trace(self._ctx, ctypes.byref(pz), _ORDERS[order], max_selections,
+                                             _ptr(board), ctypes.byref(found), _ptr(trace), trace_capacity,
+                                             ctypes.byref(n)))
+        return {"found": bool(found.value), "board": board, "trace": trace[:min(n.value, trace_capacity)].copy(),
+                "trace_length": n.value}
 
     def expand(self, width, height, colours, depth, order="spiral", capacity=1 << 20):
         """Situations of the enumeration with `depth` pieces placed; returns

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/emp_hip.h"
+#include "emp_backtrack.h"
 #include "emp_enum_plan.h"
 #include "emp_enumerate.h"
 #include "emp_glutton.h"
@@ -233,6 +234,27 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
             stats->blocks = st.blocks;
             stats->threads = st.threads;
         }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_backtrack_trace(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint32_t order,
+                            uint64_t max_selections, uint16_t *board, uint32_t *found, uint32_t *trace,
+                            uint64_t trace_capacity, uint64_t *trace_length) {
+    return guarded([&] {
+        if (!context || !puzzle || !board || !found || (trace_capacity && !trace))
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC) return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        emp::BacktrackOutcome out;
+        out.trace_capacity = trace_capacity;
+        out.max_selections = max_selections;
+        emp::backtrack_on_device(context->dev, context->stream, context->workspace, pz, (int)order, out);
+        std::copy(out.board.begin(), out.board.end(), board);
+        *found = out.found ? 1 : 0;
+        if (trace_capacity) std::copy(out.trace.begin(), out.trace.end(), trace);
+        if (trace_length) *trace_length = out.trace_length;
         return EMP_HIP_OK;
     });
 }

@@ -12,6 +12,8 @@ std::unique_ptr<feature_if> factory_feature::create_feature(const std::string &p
                                                             const std::string &p_hint) {
     if (p_name == "CUDA_backtracker")
         return std::unique_ptr<feature_if>(new feature_CUDA_backtracker(p_puzzle, p_strategy));
+    if (p_name == "CUDA_enumerate")
+        return std::unique_ptr<feature_if>(new feature_CUDA_enumerate(p_puzzle, p_strategy));
     if (p_name == "CUDA_glutton_max") return std::unique_ptr<feature_if>(new feature_CUDA_glutton_max(p_puzzle));
     if (p_name == "CUDA_glutton_wide") {
         unsigned int depth = 1;

@@ -15,6 +15,25 @@ feature_CUDA_backtracker::feature_CUDA_backtracker(const puzzle_description &p_p
 void feature_CUDA_backtracker::run() {
     device_session session;
     emp_hip_puzzle puzzle = m_puzzle.view();
+    const uint32_t n = m_puzzle.width * m_puzzle.height;
+    std::vector<uint16_t> board(n);
+    const uint64_t capacity = 1 << 20;
+    std::vector<uint32_t> trace(3 * capacity);
+    uint32_t found = 0;
+    uint64_t length = 0;
+    std::cout << "Launch kernels" << std::endl;
+    EMP_HIP_CALL(emp_hip_backtrack_trace(session.get(), &puzzle, m_order, 0, board.data(), &found, trace.data(),
+                                         capacity, &length));
+    for (uint64_t i = 0; i < length && i < capacity; ++i)
+        std::cout << "Thread " << trace[3 * i] << " : selected variable " << trace[3 * i + 1] << " at step "
+                  << trace[3 * i + 2] << std::endl;
+    if (found) std::cout << situation_string(m_puzzle, board.data()) << std::endl;
+    else std::cout << "No solution" << std::endl;
+}
+
+void feature_CUDA_enumerate::run() {
+    device_session session;
+    emp_hip_puzzle puzzle = m_puzzle.view();
     emp_hip_enum_options options;
     emp_hip_enum_default_options(&options);
     options.order = m_order;

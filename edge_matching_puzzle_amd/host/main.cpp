@@ -2,8 +2,8 @@
 //   edge_matching_puzzle <definition.txt> <ressources_dir> <feature_name>
 //                        [--strategy=basic|spiral] [--hint=<depth>]
 //                        [--enable_gui=false] [--dump_file=] [--initial_situation=]
-// CUDA_backtracker uses the basic order unless --strategy is given
-// (src/main.cpp:116-119).  Errors print "ERROR : <what> at <file>:<line>"
+// CUDA_backtracker (and CUDA_enumerate) use the basic order unless
+// --strategy is given (src/main.cpp:116-119).  Errors print "ERROR : <what> at <file>:<line>"
 // and return -1 (src/main.cpp:137-146).
 #include <iostream>
 #include <string>
@@ -30,7 +30,7 @@ int main(int argc, char **argv) {
             } else throw logic_exception("Unknown option \"" + a + "\"", __LINE__, __FILE__);
         }
         const std::string feature = argv[3];
-        if (strategy.empty()) strategy = feature == "CUDA_backtracker" ? "basic" : "spiral";
+        if (strategy.empty()) strategy = feature == "CUDA_backtracker" || feature == "CUDA_enumerate" ? "basic" : "spiral";
         puzzle_description puzzle = parse_puzzle_file(argv[1]);
         emp_hip_puzzle view = puzzle.view();
         EMP_HIP_CALL(emp_hip_validate_puzzle(&view));

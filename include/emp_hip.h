@@ -147,6 +147,21 @@ int emp_hip_enumerate(emp_hip_context *context,
                       emp_hip_enum_stats *stats);
 
 /*
+ * The reference's CUDA backtracker search (kernel_and_info,
+ * src/CUDA_backtracker.cu:68-178): forward checking to the first solution,
+ * candidates of a position in orientation-major order (lowest orientation,
+ * then lowest piece), with the selection trace it prints ("Thread <lane> :
+ * selected variable <variable> at step <step>") as (lane, variable, step)
+ * triples in trace[trace_capacity][3].  The reference is only consistent in
+ * the basic order (its transition table is keyed by row-major position);
+ * both orders are accepted here.  max_selections = 0: unlimited.
+ * Replaces: launch_CUDA_backtracker (src/CUDA_backtracker.cu:235-269).
+ */
+int emp_hip_backtrack_trace(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint32_t order,
+                            uint64_t max_selections, uint16_t *board, uint32_t *found,
+                            uint32_t *trace, uint64_t trace_capacity, uint64_t *trace_length);
+
+/*
  * Breadth expansion: every situation the enumeration visits with `depth`
  * pieces placed (indices 0..depth-1 of the order), level by level on the GPU.
  * boards: [capacity][P] row-major (cells beyond the depth empty); *n_situations

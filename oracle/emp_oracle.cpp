@@ -786,6 +786,124 @@ struct Glutton {
     }
 };
 
+// ---------------------------------------------------------------------------
+// CUDA_backtracker (src/CUDA_backtracker.cu:68-178) with its inputs:
+// variables of emp_variable_generator::create_variables
+// (include/emp_variable_generator.h:336-531: per position in strategy order,
+// every kept (piece, orientation) of the position's kind matching the frame),
+// level-0 capability of system_equation_for_CUDA::prepare_initial
+// (include/system_equation_for_CUDA.h:159-173) and the transitions of
+// prepare_transitions (:70-134): a variable clears every variable of its
+// position and of its piece, and the colour-incompatible variables of its
+// four neighbours.  Capabilities are orientation-major
+// (include/CUDA_piece_position_info.h:67-88): bit 256 * o + piece, so the
+// first set bit is the lowest orientation, then the lowest piece.  The
+// position index of a capability is the strategy index; the kernel looks a
+// transition up by step * 1024 + 256 * o + piece while the table is keyed by
+// the row-major position, which agree for the basic order only.
+struct Backtracker {
+    const Model &m;
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    uint32_t P;
+    static constexpr unsigned kWords = 16;   // 1024 bits
+    std::vector<uint64_t> caps;              // [(P + 1) stack levels][P positions][kWords]
+    std::vector<uint32_t> var_of_step;
+
+    Backtracker(const Model &model, int ord) : m(model), P(model.P) {
+        order = make_order(m.W, m.H, ord);
+    }
+
+    static void set(uint64_t *c, uint32_t piece, uint32_t o) {
+        uint32_t b = 256 * o + piece;
+        c[b >> 6] |= 1ull << (b & 63);
+    }
+    uint64_t *cap(uint32_t level, uint32_t pos) { return &caps[((size_t)level * P + pos) * kWords]; }
+
+    // Does (piece, o) at cell (x, y) match the frame, and is it kept?
+    bool frame_ok(const Piece &p, unsigned o, uint32_t x, uint32_t y) const {
+        if (!Puzzle_orientation_kept(p, o)) return false;
+        const bool off[4] = {y == 0, x + 1 == m.W, y + 1 == m.H, x == 0};
+        for (unsigned s = 0; s < 4; ++s)
+            if ((shown(p, s, o) == 0) != off[s]) return false;
+        return true;
+    }
+    static bool Puzzle_orientation_kept(const Piece &p, unsigned o) { return (o & ((1u << p.sim) - 1u)) == 0; }
+
+    int kind_at(uint32_t x, uint32_t y) const {
+        bool xb = x == 0 || x == m.W - 1, yb = y == 0 || y == m.H - 1;
+        return (xb && yb) ? CORNER : ((xb || yb) ? BORDER : CENTER);
+    }
+
+    // Transition of variable (piece, o) at strategy index s applied to the
+    // capability of a later index j: same piece cleared, and at a neighbour
+    // the variables whose facing colour differs.
+    void apply_transition(uint32_t s, uint32_t piece, uint32_t o, uint32_t j, uint64_t *c) const {
+        for (unsigned oo = 0; oo < 4; ++oo) {
+            uint32_t b = 256 * oo + piece;
+            c[b >> 6] &= ~(1ull << (b & 63));
+        }
+        const uint32_t xs = order[s].first, ys = order[s].second, xj = order[j].first, yj = order[j].second;
+        int side = -1;   // side of the variable facing cell j
+        if (xj == xs && yj + 1 == ys) side = 0;
+        else if (xj == xs + 1 && yj == ys) side = 1;
+        else if (xj == xs && yj == ys + 1) side = 2;
+        else if (xj + 1 == xs && yj == ys) side = 3;
+        if (side < 0) return;
+        const uint32_t col = shown(m.pieces[piece], (unsigned)side, o);
+        for (uint32_t q = 0; q < P; ++q)
+            for (unsigned oo = 0; oo < 4; ++oo) {
+                uint32_t b = 256 * oo + q;
+                if (!(c[b >> 6] >> (b & 63) & 1)) continue;
+                if (q == piece) continue;
+                if (shown(m.pieces[q], ((unsigned)side + 2) & 3, oo) != col) c[b >> 6] &= ~(1ull << (b & 63));
+            }
+    }
+
+    // Returns true when a solution was found; trace gets (lane, variable, step).
+    bool run(std::vector<uint32_t> &trace) {
+        caps.assign((size_t)(P + 1) * P * kWords, 0);
+        for (uint32_t i = 0; i < P; ++i) {
+            const uint32_t x = order[i].first, y = order[i].second;
+            const int k = kind_at(x, y);
+            for (uint32_t idx : m.by_kind[k])
+                for (unsigned o = 0; o < 4; ++o)
+                    if (frame_ok(m.pieces[idx], o, x, y)) set(cap(0, i), idx, o);
+        }
+        var_of_step.assign(P, 0);
+        uint32_t step = 0;
+        while (step < P) {
+            uint64_t *avail = cap(step, step);
+            int bit = -1;
+            for (unsigned w = 0; w < kWords && bit < 0; ++w)
+                if (avail[w]) bit = (int)(w * 64 + __builtin_ctzll(avail[w]));
+            if (bit < 0) {
+                if (step == 0) return false;
+                --step;
+                continue;
+            }
+            const uint32_t o = (uint32_t)bit / 256, piece = (uint32_t)bit % 256;
+            const uint32_t lane = (uint32_t)bit / 32, variable = (step << 10) | (uint32_t)bit;
+            trace.push_back(lane);
+            trace.push_back(variable);
+            trace.push_back(step);
+            avail[bit >> 6] &= ~(1ull << (bit & 63));
+            var_of_step[step] = (uint32_t)bit;
+            if (step == P - 1) return true;
+            bool valid = true;
+            for (uint32_t j = step + 1; j < P && valid; ++j) {
+                uint64_t *dst = cap(step + 1, j);
+                std::copy(cap(step, j), cap(step, j) + kWords, dst);
+                apply_transition(step, piece, o, j, dst);
+                uint64_t any = 0;
+                for (unsigned w = 0; w < kWords; ++w) any |= dst[w];
+                valid = any != 0;
+            }
+            if (valid) ++step;
+        }
+        return true;
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -895,6 +1013,28 @@ int oracle_results_bin(uint32_t width, uint32_t height, const uint32_t *colours,
     put64(r.nodes);
     *size = img.size();
     if (out) std::memcpy(out, img.data(), std::min<uint64_t>(capacity, img.size()));
+    return ORACLE_OK;
+}
+
+int oracle_backtracker(uint32_t width, uint32_t height, const uint32_t *colours, int order, uint16_t *board,
+                       uint32_t *found, uint32_t *trace, uint64_t trace_capacity, uint64_t *trace_length) {
+    Model m;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (m.P > 256 || !is_permutation(m, order)) return ORACLE_BAD_ARGUMENT;
+    Backtracker b(m, order);
+    std::vector<uint32_t> tr;
+    bool ok = b.run(tr);
+    if (found) *found = ok ? 1 : 0;
+    if (board) {
+        std::fill(board, board + m.P, 0xFFFF);
+        if (ok)
+            for (uint32_t i = 0; i < m.P; ++i) {
+                const uint32_t bit = b.var_of_step[i];
+                board[b.order[i].second * width + b.order[i].first] = (uint16_t)(((bit % 256) << 2) | (bit / 256));
+            }
+    }
+    if (trace_length) *trace_length = tr.size() / 3;
+    if (trace) std::copy(tr.begin(), tr.begin() + std::min<uint64_t>(tr.size(), 3 * trace_capacity), trace);
     return ORACLE_OK;
 }
 

@@ -113,6 +113,17 @@ int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
                        uint32_t *max_depth, uint64_t *n_pushes,
                        uint32_t *solved);
 
+/*
+ * CUDA_backtracker restatement (src/CUDA_backtracker.cu:68-178 with the
+ * variables and transitions of include/emp_variable_generator.h:336-621 and
+ * include/system_equation_for_CUDA.h:70-173): forward-checking search for
+ * the first solution, candidates in orientation-major order.  trace receives
+ * (lane, variable, step) triples as the kernel prints them
+ * ("Thread <lane> : selected variable <variable> at step <step>").
+ */
+int oracle_backtracker(uint32_t width, uint32_t height, const uint32_t *colours, int order, uint16_t *board,
+                       uint32_t *found, uint32_t *trace, uint64_t trace_capacity, uint64_t *trace_length);
+
 #ifdef __cplusplus
 }
 #endif

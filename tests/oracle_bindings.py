@@ -34,6 +34,8 @@ def lib():
         _lib.oracle_enumerate_prefixes.restype = i32
         _lib.oracle_enumerate_subtree.argtypes = [u32, u32, p, i32, u32, p, u64, p, p, p]
         _lib.oracle_enumerate_subtree.restype = i32
+        _lib.oracle_backtracker.argtypes = [u32, u32, p, i32, p, p, p, u64, p]
+        _lib.oracle_backtracker.restype = i32
         _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
         _lib.oracle_glutton_max.restype = i32
     return _lib
@@ -137,6 +139,20 @@ def glutton_max(w, h, colours, prefix=(), push_limit=0):
         raise ValueError("oracle_glutton_max status %d" % st)
     return {"status": st, "board": board, "max_depth": int(md[0]), "pushes": int(pushes[0]),
             "solved": bool(solved[0])}
+
+
+def backtracker(w, h, colours, order=BASIC, trace_capacity=1 << 20):
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    board = np.zeros(w * h, dtype=np.uint16)
+    found = np.zeros(1, np.uint32)
+    trace = np.zeros((trace_capacity, 3), np.uint32)
+    n = np.zeros(1, np.uint64)
+    st = lib().oracle_backtracker(w, h, _ptr(colours), order, _ptr(board), _ptr(found), _ptr(trace),
+                                  trace_capacity, _ptr(n))
+    if st != OK:
+        raise ValueError("oracle_backtracker status %d" % st)
+    return {"found": bool(found[0]), "board": board, "trace": trace[:min(int(n[0]), trace_capacity)].copy(),
+            "trace_length": int(n[0])}
 
 
 def situation_string(board, P):

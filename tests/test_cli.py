@@ -50,13 +50,25 @@ def test_glutton_max_feature_stdout(expected_stdout, name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["3_3", "4_4", "5_5"])
-def test_backtracker_feature_lists_reference_solution(expected_stdout, name):
-    """The first solution the reference's CUDA backtracker prints
-    (tests_CUDA/backtracker_<name>/test.info, basic order) is among the
-    solutions the enumeration lists; totals match the oracle in basic order."""
-    import oracle_bindings as o
+def test_backtracker_feature_stdout(expected_stdout, name):
+    """Every expected_stdout_string of tests_CUDA/backtracker_<name>/test.info
+    (the selection trace and the first solution)."""
     print("case: CLI CUDA_backtracker", name, flush=True)
     rc, out = run(os.path.join(PUZZLES, name + ".txt"), "ressources", "CUDA_backtracker", "--enable_gui=false")
+    assert rc == 0, out
+    lines = out.splitlines()
+    for s in expected_stdout["backtracker_" + name]["strings"]:
+        assert s in lines
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["3_3", "4_4", "5_5"])
+def test_enumerate_feature_totals(expected_stdout, name):
+    """CUDA_enumerate lists the reference backtracker's solution among all
+    solutions; totals equal the oracle's in the basic order."""
+    import oracle_bindings as o
+    print("case: CLI CUDA_enumerate", name, flush=True)
+    rc, out = run(os.path.join(PUZZLES, name + ".txt"), "ressources", "CUDA_enumerate", "--enable_gui=false")
     assert rc == 0, out
     assert expected_stdout["backtracker_" + name]["strings"][-1] in out.splitlines()
     w, h, c = o.parse_puzzle(os.path.join(PUZZLES, name + ".txt"))
