@@ -71,12 +71,122 @@ def cpu_baseline(w, h, colours, nodes):
     return r, dt
 
 
+def _cpu_pool_c2(puzzles, limit, seconds=15.0):
+    """Oracle on the host cores, one instance per worker, bounded sample."""
+    from multiprocessing import Pool
+    cores = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    nodes = 0
+    done = 0
+    with Pool(cores) as pool:
+        for r in pool.imap_unordered(_cpu_one, [(w, h, c.tolist(), limit) for w, h, c in puzzles]):
+            nodes += r
+            done += 1
+            if time.perf_counter() - t > seconds:
+                break
+        pool.terminate()
+    dt = time.perf_counter() - t
+    return {"value": nodes / dt, "unit": "nodes/s", "cores": cores, "kind": "port",
+            "sample": "first %d C2 instances, node limit %d each, one instance per process (%.1f s)" % (done, limit, dt)}
+
+
+def _cpu_one(args):
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bindings as o
+    w, h, c, limit = args
+    return o.enumerate_(w, h, np.array(c, dtype=np.uint32), node_limit=limit, max_solutions=1)["n_nodes"]
+
+
+def bench_c2(args, ctx):
+    """C2: 4096 uneven instances in one launch, per-instance node limit."""
+    from edge_matching_puzzle_amd.workloads import c2_puzzles
+    puzzles = c2_puzzles(args.instances)
+    limit = args.node_limit
+    for i in range(args.warmup):
+        ctx.enumerate(puzzles[:256], max_solutions=4, node_limit=limit)
+        print("warmup step %d done" % i, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    nodes = 0
+    kernel_ms = 0.0
+    for _ in range(args.steps):
+        res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit)
+        nodes += sum(r["n_nodes"] for r in res)
+        kernel_ms += st["dfs_kernel_ms"]
+        print("step done: %d nodes, %.1f ms kernel" % (sum(r["n_nodes"] for r in res), st["dfs_kernel_ms"]),
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    done = [r for r in res if r["complete"]]
+    times = sorted(r["solve_ms"] for r in done)
+    pct = lambda q: times[min(len(times) - 1, int(q * len(times)))] if times else None
+    cpu = None if args.no_cpu else _cpu_pool_c2(puzzles, 20_000_000)
+    return {
+        "metric": METRIC, "value": nodes / elapsed, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "C2 batched enumeration: %d puzzles, side 5..10, border colours 3..5, "
+                               "interior colours 4..12, one launch, node limit %d per instance" % (len(puzzles), limit),
+                   "completed": len(done), "truncated": len(res) - len(done),
+                   "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99),
+                   "nodes_per_step": sum(r["n_nodes"] for r in res)},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
+def bench_c3(args, ctx):
+    """C3: glutton_max on 16x16 from hint + 24 spiral moves, push budget."""
+    from edge_matching_puzzle_amd.workloads import c3_prefixes, c3_puzzle
+    w, h, c, hint = c3_puzzle()
+    rows = c3_prefixes(args.prefixes)
+    for i in range(args.warmup):
+        ctx.glutton_max(w, h, c, prefixes=rows[:256], push_budget=2)
+        print("warmup step %d done" % i, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    pushes = 0
+    for _ in range(args.steps):
+        res, boards, st = ctx.glutton_max(w, h, c, prefixes=rows, push_budget=args.push_budget)
+        pushes += st["pushes"]
+        print("step done: %d pushes, %.1f ms kernel, %.3g evaluations" % (st["pushes"], st["kernel_ms"], st["evaluations"]),
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    depths = [r["max_depth"] for r in res]
+    return {
+        "metric": METRIC, "value": pushes / elapsed, "unit": "glutton nodes (pushes)/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "C3 glutton_max: 16x16, 5 border / 17 interior colours, seed 3, centre hint + 24 "
+                               "spiral moves, %d prefixes, %d pushes each" % (len(rows), args.push_budget),
+                   "max_depth": max(depths), "prefixes_depth_ge_200": sum(1 for d in depths if d >= 200),
+                   "mean_depth": sum(depths) / len(depths), "evaluations_per_s": st["evaluations"] / (st["kernel_ms"] * 1e-3),
+                   "waves": st["waves"], "lds_bytes_per_wave": st["lds_bytes"]},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu else _cpu_c3(w, h, c, rows[0]),
+    }
+
+
+def _cpu_c3(w, h, c, row):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bindings as o
+    moves = [v for m in row for v in ((int(m) & 0xFF), (int(m) >> 8) & 0xFF, (int(m) >> 16) & 3)]
+    t = time.perf_counter()
+    r = o.glutton_max(w, h, c, prefix=moves, push_limit=1)
+    dt = time.perf_counter() - t
+    return {"value": r["pushes"] / dt, "unit": "glutton nodes (pushes)/s", "cores": 1, "kind": "port",
+            "sample": "one prefix, one push (CPU oracle, %.1f s)" % dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--instances", type=int, default=4096, help="C2 instances")
+    ap.add_argument("--node-limit", type=int, default=200_000_000, help="C2 nodes per instance")
+    ap.add_argument("--prefixes", type=int, default=4096, help="C3 prefixes")
+    ap.add_argument("--push-budget", type=int, default=16, help="C3 pushes per prefix")
+    ap.add_argument("--autotune", action="store_true", help="tune the C1 launch before timing")
     ap.add_argument("--cpu-nodes", type=int, default=600_000_000,
                     help="nodes of the CPU baseline sample (about 15 s on one core)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -87,10 +197,25 @@ def main():
         sys.exit(2)
 
     import edge_matching_puzzle_amd as emp
-    w, h, colours = puzzle_c1(args.seed)
     ctx = emp.Context(0)
+    if args.workload != "c1":
+        line = bench_c2(args, ctx) if args.workload == "c2" else bench_c3(args, ctx)
+        ctx.close()
+        print(json.dumps(line), flush=True)
+        return
+    w, h, colours = puzzle_c1(args.seed)
+    tuned = {}
+    if args.autotune:
+        from edge_matching_puzzle_amd import autotune
+        best, history = autotune.tune(ctx, [(w, h, colours)], 10_000_000_000, budget_s=60,
+                                      log=lambda m: print(m, file=sys.stderr, flush=True))
+        lanes = ctx.device_info()["compute_units"] * best["waves_per_cu"] * 64
+        tuned = dict(waves_per_cu=best["waves_per_cu"], waves_per_block=best["waves_per_block"],
+                     steps_per_round=best["steps_per_round"], step_budget=best["step_budget"],
+                     target_items=best["target_per_lane"] * lanes)
+        print("autotune best %s" % json.dumps(best, sort_keys=True), file=sys.stderr, flush=True)
     for i in range(args.warmup):
-        ctx.enumerate([(w, h, colours)], max_solutions=64)
+        ctx.enumerate([(w, h, colours)], max_solutions=64, **tuned)
         print("warmup step %d done" % i, file=sys.stderr, flush=True)
     nodes = 0
     kernel_ms = 0.0
@@ -98,7 +223,7 @@ def main():
     results = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        (r,), st = ctx.enumerate([(w, h, colours)], max_solutions=64)
+        (r,), st = ctx.enumerate([(w, h, colours)], max_solutions=64, **tuned)
         nodes += r["n_nodes"]
         kernel_ms += st["dfs_kernel_ms"]
         launches += st["dfs_launches"]
@@ -140,7 +265,7 @@ def main():
                                "seed %d, corner kind-index 0 fixed, spiral order" % args.seed,
                    "nodes_per_step": r["n_nodes"], "solutions": r["n_solutions"],
                    "dfs_launches_per_step": st["dfs_launches"], "grid": [st["blocks"], st["threads"]],
-                   "lds_tables": bool(st["lds_tables"])},
+                   "lds_tables": bool(st["lds_tables"]), "tuned": tuned or None},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / LDS_PEAK_GBS, "traffic": None,
                      "bytes_per_node": bytes_node, "words_per_mask": M,

@@ -168,3 +168,26 @@ def test_c1_full_size_against_golden(gpu_context, golden_dir):
     assert r["n_nodes"] == g["n_nodes"] == 366518800244
     assert r["n_solutions"] == g["n_solutions"] == 2
     assert sorted(emp.situation_string(8, 8, b) for b in r["boards"]) == g["solutions"]
+
+
+def test_c2_slice_with_node_limit(gpu_context):
+    """The first 48 C2 instances (sides 5..10, B 3..5, I 4..12) in one launch
+    with a per-instance node limit of 2e7: every instance the GPU completes
+    must equal the oracle; the others are flagged incomplete."""
+    from edge_matching_puzzle_amd.workloads import c2_puzzles
+    puzzles = c2_puzzles(48)
+    limit = 20_000_000
+    print("case: C2 slice", flush=True)
+    res, st = gpu_context.enumerate(puzzles, max_solutions=256, node_limit=limit)
+    complete = 0
+    for (w, h, c), r in zip(puzzles, res):
+        if not r["complete"]:
+            continue
+        complete += 1
+        # the oracle must finish with exactly the GPU's node count
+        ref = o.enumerate_(w, h, c, node_limit=r["n_nodes"] + 1, max_solutions=256)
+        assert ref["status"] == o.OK
+        assert r["n_nodes"] == ref["n_nodes"] and r["n_solutions"] == ref["n_solutions"]
+        if ref["n_solutions"] <= 256:
+            assert _sorted_boards(r["boards"]) == _sorted_boards(ref["boards"])
+    assert complete >= 10
