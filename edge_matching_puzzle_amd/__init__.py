@@ -50,7 +50,7 @@ class EnumStats(ctypes.Structure):
                 ("expand_launches", ctypes.c_uint64), ("items_processed", ctypes.c_uint64),
                 ("splits", ctypes.c_uint64), ("max_words", ctypes.c_uint32),
                 ("lds_tables", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
-                ("threads", ctypes.c_uint32)]
+                ("threads", ctypes.c_uint32), ("candidate_sets", ctypes.c_uint64)]
 
 
 class GluttonOptions(ctypes.Structure):

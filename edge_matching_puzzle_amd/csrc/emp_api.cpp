@@ -229,6 +229,7 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
             stats->expand_launches = st.expand_launches;
             stats->items_processed = st.items_processed;
             stats->splits = st.splits;
+            stats->candidate_sets = st.candidate_sets;
             stats->max_words = st.max_words;
             stats->lds_tables = st.lds_tables;
             stats->blocks = st.blocks;

@@ -1,6 +1,7 @@
 // emp_enum_plan.cpp -- builds the enumeration table of a puzzle.
 #include "emp_enum_plan.h"
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <unordered_set>
@@ -13,10 +14,17 @@ EnumTableHeader EnumTable::header() const {
     return h;
 }
 
-EnumTable build_enum_table(const Puzzle &pz, int order) {
+uint32_t natural_row_words(const Puzzle &pz) {
+    uint32_t w = 1;
+    for (int k = 0; k < 3; ++k) w = std::max<uint32_t>(w, (4 * (uint32_t)pz.pieces_of(k).size() + 63) / 64);
+    return w;
+}
+
+EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
     EnumTable t;
     EnumTableHeader h{};
     const uint32_t P = pz.size();
+    if (P >= kOffBoard) throw std::domain_error("enumeration tables index at most 509 positions");
     h.P = P;
     h.W = pz.width();
     h.H = pz.height();
@@ -33,6 +41,8 @@ EnumTable build_enum_table(const Puzzle &pz, int order) {
         for (uint32_t idx : pz.pieces_of(k)) t.piece_of[k].push_back((uint16_t)idx);
     }
     h.avail_words = h.words[0] + h.words[1] + h.words[2];
+    h.row = std::max(row_words, natural_row_words(pz));
+    const uint32_t R = h.row;
 
     // Appearance classes: kept transitions of the same kind showing the same
     // four colours, recorded when they span at least two pieces.  Trying
@@ -57,7 +67,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order) {
         std::vector<char> seen(4 * n_kind[k], 0);
         for (uint32_t a = 0; a < 4 * n_kind[k]; ++a) {
             if (!valid[a] || seen[a]) continue;
-            std::vector<uint64_t> mask(h.words[k], 0);
+            std::vector<uint64_t> mask(R, 0);
             bool several_pieces = false;
             for (uint32_t b = a; b < 4 * n_kind[k]; ++b) {
                 if (!valid[b] || look[b] != look[a]) continue;
@@ -74,13 +84,21 @@ EnumTable build_enum_table(const Puzzle &pz, int order) {
     uint32_t off = kHeaderWords;
     h.off_pos = off;
     off += P;
+    h.off_nbr = off;
+    off += P;
+    h.off_base = off;
+    off += P * R;
+    h.off_ones = off;
+    off += 32;
     for (int k = 0; k < 3; ++k) {
         h.off_cm[k] = off;
-        off += 4 * h.ncol * h.words[k];
+        off += 4 * h.ncol * R;
         h.off_valid[k] = off;
-        off += h.words[k];
+        off += R;
         h.off_dup[k] = off;
-        off += h.ndup[k] * h.words[k];
+        off += h.ndup[k] * R + R;
+        h.off_cls[k] = off;
+        off += (4 * n_kind[k] + 3) / 4;
         h.off_rot[k] = off;
         off += (4 * n_kind[k] + 1) / 2;
     }
@@ -125,27 +143,60 @@ EnumTable build_enum_table(const Puzzle &pz, int order) {
                 valid[tr >> 6] |= 1ull << (tr & 63);
                 for (unsigned s = 0; s < 4; ++s) {
                     uint32_t c = shown(p, s, o);
-                    cm[((size_t)s * h.ncol + c) * h.words[k] + (tr >> 6)] |= 1ull << (tr & 63);
+                    cm[((size_t)s * h.ncol + c) * R + (tr >> 6)] |= 1ull << (tr & 63);
                 }
             }
         }
         for (uint32_t d = 0; d < h.ndup[k]; ++d)
-            for (uint32_t w = 0; w < h.words[k]; ++w) b[h.off_dup[k] + d * h.words[k] + w] = dup[k][d][w];
+            for (uint32_t w = 0; w < R; ++w) b[h.off_dup[k] + d * R + w] = dup[k][d][w];
+        uint16_t *cls = reinterpret_cast<uint16_t *>(b + h.off_cls[k]);
+        for (uint32_t tr = 0; tr < 4 * n_kind[k]; ++tr) cls[tr] = 0xFFFF;
+        for (uint32_t d = 0; d < h.ndup[k]; ++d)
+            for (uint32_t w = 0; w < R; ++w) {
+                b[h.off_dup[k] + h.ndup[k] * R + w] |= dup[k][d][w];
+                for (uint64_t v = dup[k][d][w]; v; v &= v - 1) cls[w * 64 + __builtin_ctzll(v)] = (uint16_t)d;
+            }
+    }
+
+    for (uint32_t w = 0; w < 16; ++w) b[h.off_ones + w] = ~0ull;
+
+    // Base masks (valid transitions matching the frame sides) and neighbour
+    // descriptors.
+    for (uint32_t i = 0; i < P; ++i) {
+        const uint64_t rec = b[h.off_pos + i];
+        const int k = (int)(rec & 3);
+        uint64_t *base = b + h.off_base + (size_t)i * R;
+        for (uint32_t w = 0; w < R; ++w) base[w] = b[h.off_valid[k] + w];
+        uint64_t d = (uint64_t)k << 3;
+        uint32_t n = 0;
+        for (unsigned s = 0; s < 4; ++s) {
+            const uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
+            if (code == kUnconstrained) continue;
+            if (code == kOffBoard) {
+                const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + 0) * R;
+                for (uint32_t w = 0; w < R; ++w) base[w] &= m[w];
+                continue;
+            }
+            const uint32_t nk = (uint32_t)(rec >> (11 + 11 * s)) & 3;
+            d |= (uint64_t)(code | nk << 9 | s << 11) << (kNbrShift + kNbrBits * n);
+            ++n;
+        }
+        b[h.off_nbr + i] = d | n;
     }
 
     // Root: first candidate of index 0, where no neighbour is placed yet.
     {
         uint64_t rec = b[h.off_pos];
         int k = (int)(rec & 3);
-        std::vector<uint64_t> c(b + h.off_valid[k], b + h.off_valid[k] + h.words[k]);
+        std::vector<uint64_t> c(b + h.off_valid[k], b + h.off_valid[k] + R);
         for (unsigned s = 0; s < 4; ++s) {
             uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
             if (code != kOffBoard) continue;
-            const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + 0) * h.words[k];
-            for (uint32_t w = 0; w < h.words[k]; ++w) c[w] &= m[w];
+            const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + 0) * R;
+            for (uint32_t w = 0; w < R; ++w) c[w] &= m[w];
         }
         int root = -1;
-        for (uint32_t w = 0; w < h.words[k] && root < 0; ++w)
+        for (uint32_t w = 0; w < R && root < 0; ++w)
             if (c[w]) root = (int)(w * 64 + __builtin_ctzll(c[w]));
         if (root < 0) throw std::invalid_argument("no piece fits the first position");
         h.root = (uint32_t)root;
@@ -188,7 +239,7 @@ void spiral_prefixes(const Puzzle &pz, uint32_t length, uint32_t count, uint64_t
         // Available transitions per kind, minus the fixed pieces.
         std::vector<uint64_t> avail[3];
         for (int k = 0; k < 3; ++k) {
-            avail[k].assign(b + h.off_valid[k], b + h.off_valid[k] + h.words[k]);
+            avail[k].assign(b + h.off_valid[k], b + h.off_valid[k] + h.row);
             for (uint32_t ki = 0; ki < t.piece_of[k].size(); ++ki)
                 if (fixed_piece[t.piece_of[k][ki]]) avail[k][(4 * ki) >> 6] &= ~(0xFull << ((4 * ki) & 63));
         }
@@ -207,15 +258,15 @@ void spiral_prefixes(const Puzzle &pz, uint32_t length, uint32_t count, uint64_t
                     const int nk = (int)(rec >> (11 + 11 * s)) & 3;
                     col = (reinterpret_cast<const uint32_t *>(b + h.off_rot[nk])[stack[code]] >> (8 * ((s + 2) & 3))) & 0xFF;
                 }
-                const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + col) * h.words[k];
-                for (uint32_t w = 0; w < h.words[k]; ++w) c[w] &= m[w];
+                const uint64_t *m = b + h.off_cm[k] + ((size_t)s * h.ncol + col) * h.row;
+                for (uint32_t w = 0; w < h.row; ++w) c[w] &= m[w];
             }
             uint32_t n = 0;
             for (uint64_t v : c) n += (uint32_t)__builtin_popcountll(v);
             if (!n) { ok = false; break; }
             uint32_t pick = (uint32_t)(((splitmix64(state) >> 32) * (uint64_t)n) >> 32);
             uint32_t tr = 0;
-            for (uint32_t w = 0; w < h.words[k]; ++w) {
+            for (uint32_t w = 0; w < h.row; ++w) {
                 uint64_t v = c[w];
                 const uint32_t pc = (uint32_t)__builtin_popcountll(v);
                 if (pick >= pc) { pick -= pc; continue; }

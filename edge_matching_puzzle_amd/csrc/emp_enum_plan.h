@@ -30,18 +30,32 @@ namespace emp {
 constexpr uint32_t kOffBoard = 0x1FE;
 constexpr uint32_t kUnconstrained = 0x1FF;
 
+// Neighbour descriptor (one u64 per strategy index), the form the kernels use:
+//   bits 0-2            number n of neighbours placed earlier (0..4)
+//   bits 3-4            kind of the position
+//   bits 5+13i .. +12   i-th such neighbour (i < n): strategy index (9 bits),
+//                       its kind << 9, the side of this position facing it << 11
+// Frame sides are folded into the position's base mask instead.
+constexpr uint32_t kNbrShift = 5, kNbrBits = 13;
+
 struct EnumTableHeader {
     uint32_t P, W, H, ncol;      // ncol = max colour + 1 (colour 0 = frame)
-    uint32_t words[3];           // u64 words of a kind's candidate bitfield
+    uint32_t words[3];           // u64 words holding a kind's transitions
     uint32_t ndup[3];            // appearance classes with >= 2 transitions
     uint32_t off_pos;            // u64 offsets into the table blob
-    uint32_t off_cm[3];          // [4 sides][ncol][words[K]]
-    uint32_t off_valid[3];       // [words[K]] orientations kept per piece
-    uint32_t off_dup[3];         // [ndup[K]][words[K]]
+    uint32_t off_cm[3];          // [4 sides][ncol][row]
+    uint32_t off_valid[3];       // [row] orientations kept per piece
+    uint32_t off_dup[3];         // [ndup[K]][row], then the union of the classes [row]
     uint32_t off_rot[3];         // u32 per transition: colours N|E<<8|S<<16|W<<24
     uint32_t root;               // transition placed at index 0
     uint32_t total_words;        // size of the blob in u64
     uint32_t avail_words;        // words[0] + words[1] + words[2]
+    uint32_t row;                // words of every mask row (>= each words[K]; padding is zero)
+    uint32_t off_base;           // [P][row]: valid transitions of the position's kind that match its frame sides
+    uint32_t off_nbr;            // [P] neighbour descriptors
+    uint32_t off_ones;           // 16 all-ones words, then 16 all-zeros words
+    uint32_t off_cls[3];         // u16 per transition: its appearance class, 0xFFFF for none
+    uint32_t reserved;
 };
 static_assert(sizeof(EnumTableHeader) % 8 == 0, "header must be u64 aligned");
 constexpr uint32_t kHeaderWords = sizeof(EnumTableHeader) / 8;
@@ -53,9 +67,12 @@ struct EnumTable {
     EnumTableHeader header() const;
 };
 
-// Builds the table; throws std::invalid_argument when the first position has
-// no candidate.
-EnumTable build_enum_table(const Puzzle &puzzle, int order);
+// u64 words needed by the largest kind's transitions (4 per piece).
+uint32_t natural_row_words(const Puzzle &puzzle);
+
+// Builds the table with mask rows of max(row_words, natural_row_words) words;
+// throws std::invalid_argument when the first position has no candidate.
+EnumTable build_enum_table(const Puzzle &puzzle, int order, uint32_t row_words = 0);
 
 // Converts a stack of transitions (one per strategy index) to a row-major board.
 void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *board);

@@ -80,6 +80,7 @@ struct EnumStats {
     double total_ms = 0, dfs_kernel_ms = 0, expand_kernel_ms = 0;
     uint64_t dfs_launches = 0, expand_launches = 0, items_processed = 0, splits = 0;
     uint32_t max_words = 0, lds_tables = 0, blocks = 0, threads = 0;
+    uint64_t candidate_sets = 0;   // DFS iterations (one candidate set each)
 };
 
 void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,

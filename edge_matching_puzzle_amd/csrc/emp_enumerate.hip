@@ -32,6 +32,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "emp_enum_plan.h"
@@ -74,6 +75,7 @@ struct DfsArgs {
     // goes to slot p * sol_per_puzzle + s while s < sol_per_puzzle
     uint16_t *sol_stack;           // [n_puzzles * sol_per_puzzle][stride]
     uint32_t sol_per_puzzle;
+    unsigned long long *candidate_sets;   // candidate sets computed (DFS iterations)
 };
 
 // Item header: puzzle (bits 0-19) | floor (20-30) | level (31-41) |
@@ -93,15 +95,17 @@ constexpr uint32_t kDeadItem = 0x7FF;   // floor of a slot reserved but never wr
 
 // Read-only view of one table blob.  Per-kind fields are kept as three
 // scalars and picked with selects, so a runtime kind never indexes a private
-// array (which would spill it to scratch memory).
+// array (which would spill it to scratch memory).  Every mask row of the blob
+// has W words (the launch's row width; words past a kind's transitions are 0).
 template <typename T>
 __device__ __forceinline__ T sel3(int k, T a0, T a1, T a2) { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
 
 struct TableView {
     const uint64_t *b;
-    uint32_t P, ncol, off_pos;
-    uint32_t w0, w1, w2, d0, d1, d2;
+    uint32_t P, ncol;
+    uint32_t n0, n1, n2, d0, d1, d2;
     uint32_t cm0, cm1, cm2, va0, va1, va2, du0, du1, du2, ro0, ro1, ro2;
+    uint32_t off_base, off_nbr, off_ones, off_zero, cl0, cl1, cl2;
 
     __device__ __forceinline__ void load(const uint64_t *blob) {
         b = blob;
@@ -109,101 +113,118 @@ struct TableView {
         // field order of EnumTableHeader
         P = h[0];
         ncol = h[3];
-        w0 = h[4]; w1 = h[5]; w2 = h[6];
+        n0 = h[4]; n1 = h[5]; n2 = h[6];
         d0 = h[7]; d1 = h[8]; d2 = h[9];
-        off_pos = h[10];
         cm0 = h[11]; cm1 = h[12]; cm2 = h[13];
         va0 = h[14]; va1 = h[15]; va2 = h[16];
         du0 = h[17]; du1 = h[18]; du2 = h[19];
         ro0 = h[20]; ro1 = h[21]; ro2 = h[22];
+        off_base = h[27];
+        off_nbr = h[28];
+        off_ones = h[29];
+        off_zero = h[29] + 16;
+        cl0 = h[30]; cl1 = h[31]; cl2 = h[32];
     }
-    __device__ __forceinline__ uint32_t words(int k) const { return sel3(k, w0, w1, w2); }
+    // natural words of kind K (per-lane available-transition layout)
+    __device__ __forceinline__ uint32_t nat(int k) const { return sel3(k, n0, n1, n2); }
+    __device__ __forceinline__ uint32_t abase(int k) const { return sel3(k, 0u, n0, n0 + n1); }
     __device__ __forceinline__ uint32_t ndup(int k) const { return sel3(k, d0, d1, d2); }
     __device__ __forceinline__ uint32_t off_cm(int k) const { return sel3(k, cm0, cm1, cm2); }
     __device__ __forceinline__ uint32_t off_valid(int k) const { return sel3(k, va0, va1, va2); }
     __device__ __forceinline__ uint32_t off_dup(int k) const { return sel3(k, du0, du1, du2); }
-    __device__ __forceinline__ uint32_t abase(int k) const { return sel3(k, 0u, w0, w0 + w1); }
-    __device__ __forceinline__ uint64_t pos(uint32_t k) const { return b[off_pos + k]; }
+    __device__ __forceinline__ uint64_t nbr(uint32_t k) const { return b[off_nbr + k]; }
+    __device__ __forceinline__ uint32_t cls(int kind, uint32_t t) const {
+        return reinterpret_cast<const uint16_t *>(b + sel3(kind, cl0, cl1, cl2))[t];
+    }
     __device__ __forceinline__ uint32_t rot(int kind, uint32_t t) const {
         return reinterpret_cast<const uint32_t *>(b + sel3(kind, ro0, ro1, ro2))[t];
     }
 };
 
-static_assert(sizeof(EnumTableHeader) == 26 * 4, "TableView::load reads EnumTableHeader by field index");
+static_assert(sizeof(EnumTableHeader) == 34 * 4, "TableView::load reads EnumTableHeader by field index");
 
-// Candidates of strategy index k: AND of the available pieces of its kind and
-// the colour masks of its constrained sides; in each appearance class only
-// the lowest piece present keeps its transitions; then only transitions above
-// `last`.  The class reduction is applied to the full candidate set, as the
-// reference's identical-piece masks accumulate from the first candidate on.
-// get_choice(j): transition placed at index j; get_avail(K, w): available
-// transition word w of kind K.
-template <int MAXW, typename GetChoice, typename GetAvail>
-__device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, int last,
-                                           GetChoice get_choice, GetAvail get_avail,
-                                           uint64_t (&c)[MAXW], int &kind_out) {
-    const uint64_t rec = tv.pos(k);
-    const int K = (int)(rec & 3);
-    const uint32_t wK = tv.words(K);
-    kind_out = K;
-#pragma unroll
-    for (int w = 0; w < MAXW; ++w) c[w] = (w < (int)wK) ? get_avail(K, w) : 0ull;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
-        if (code == kUnconstrained) continue;
-        uint32_t col = 0;
-        if (code != kOffBoard) {
-            const int nk = (int)(rec >> (11 + 11 * s)) & 3;
-            const uint32_t tn = get_choice(code);
-            col = (tv.rot(nk, tn) >> (8 * ((s + 2) & 3))) & 0xFF;
-        }
-        const uint64_t *cm = tv.b + tv.off_cm(K) + ((uint32_t)s * tv.ncol + col) * wK;
-#pragma unroll
-        for (int w = 0; w < MAXW; ++w)
-            if (w < (int)wK) c[w] &= cm[w];
-    }
-    for (uint32_t d = 0, nd = tv.ndup(K); d < nd; ++d) {
-        const uint64_t *dm = tv.b + tv.off_dup(K) + d * wK;
-        bool seen = false;
-#pragma unroll
-        for (int w = 0; w < MAXW; ++w) {
-            if (w < (int)wK) {
-                const uint64_t m = c[w] & dm[w];
-                if (seen) c[w] &= ~m;
-                else if (m) {
-                    // keep the transitions of the lowest piece of the class
-                    const uint64_t keep = 0xFull << (__builtin_ctzll(m) & ~3);
-                    c[w] &= ~(m & ~keep);
-                    seen = true;
-                }
-            }
-        }
-    }
-    if (last >= 0) {
-        const int from = last + 1;
-        const int fw = from >> 6, fb = from & 63;
-#pragma unroll
-        for (int w = 0; w < MAXW; ++w) {
-            if (w < fw) c[w] = 0;
-            else if (w == fw) c[w] &= ~0ull << fb;
-        }
-    }
-}
+__device__ __forceinline__ int desc_kind(uint64_t d) { return (int)(d >> 3) & 3; }
 
-template <int MAXW>
-__device__ __forceinline__ int first_bit(const uint64_t (&c)[MAXW]) {
+// Candidates of strategy index k (descriptor d): its base mask (valid
+// transitions matching the frame) AND the available transitions of its kind
+// AND one colour mask per neighbour placed earlier; in each appearance class
+// only the lowest piece present keeps its transitions; then only transitions
+// above `last`.  The class reduction is applied to the full candidate set, as
+// the reference's identical-piece masks accumulate from the first candidate
+// on.  get_choice(j): transition placed at index j; get_avail(K, w): available
+// transition word w of kind K (only words below the kind's natural count
+// matter: the base mask is zero above).
+template <int W>
+__device__ __forceinline__ int first_bit(const uint64_t (&c)[W]) {
     int t = -1;
 #pragma unroll
-    for (int w = MAXW - 1; w >= 0; --w)
+    for (int w = W - 1; w >= 0; --w)
         if (c[w]) t = w * 64 + __builtin_ctzll(c[w]);
     return t;
+}
+
+template <int W>
+__device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
+    uint64_t v = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v |= c[w];
+    return v != 0;
+}
+
+template <int W, typename GetChoice, typename GetAvail>
+__device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint64_t d, int last,
+                                           GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W]) {
+    const int K = desc_kind(d);
+    const uint32_t n = (uint32_t)d & 7;
+    const uint64_t *base = tv.b + tv.off_base + (size_t)k * W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c[w] = base[w] & get_avail(K, w);
+    // Neighbour masks.  The loop runs to the wave's largest count; a lane
+    // past its own count ANDs the all-ones row, so the body has no branch.
+    const uint64_t *cmK = tv.b + tv.off_cm(K);
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        const bool on = i < n;
+        if (__ballot(on) == 0) break;
+        const uint32_t e = (uint32_t)(d >> (kNbrShift + kNbrBits * i)) & 0x1FFF;
+        const uint32_t j = on ? (e & 0x1FF) : 0u, nk = (e >> 9) & 3, s = e >> 11;
+        const uint32_t col = (tv.rot((int)nk, get_choice(j)) >> (8 * ((s + 2) & 3))) & 0xFF;
+        const uint64_t *row = on ? cmK + (s * tv.ncol + col) * W : tv.b + tv.off_ones;
+#pragma unroll
+        for (int w = 0; w < W; ++w) c[w] &= row[w];
+    }
+    // Appearance classes present among the candidates, lowest first; most
+    // candidate sets meet none, so the loop usually does not run.
+    const uint64_t *dup = tv.b + tv.off_dup(K);
+    const uint64_t *uni = dup + tv.ndup(K) * W;
+    uint64_t rem[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) rem[w] = c[w] & uni[w];
+    while (__ballot(any_bit<W>(rem)) != 0) {
+        const int b = first_bit<W>(rem);
+        const uint64_t *dm = b >= 0 ? dup + tv.cls(K, (uint32_t)b) * W : tv.b + tv.off_zero;
+        bool seen = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint64_t m = c[w] & dm[w];
+            // keep the transitions of the lowest piece of the class
+            const uint64_t low = m ? (0xFull << (__builtin_ctzll(m) & 60)) : 0ull;
+            c[w] &= ~(m & ~(seen ? 0ull : low));
+            seen |= m != 0;
+            rem[w] &= ~dm[w];
+        }
+    }
+    // Only transitions above `last` (last = -1 keeps all).
+    const int from = last + 1;
+    const int fw = from >> 6, fb = from & 63;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c[w] &= w < fw ? 0ull : (w == fw ? ~0ull << fb : ~0ull);
 }
 
 // ---------------------------------------------------------------------------
 // Prefix expansion: one thread per item.
 // ---------------------------------------------------------------------------
-template <int MAXW>
+template <int W>
 __global__ void expand_kernel(DfsArgs a) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.n_in) return;
@@ -228,41 +249,40 @@ __global__ void expand_kernel(DfsArgs a) {
     TableView tv;
     tv.load(a.tables + a.table_off[puzzle]);
     const uint16_t *stack = a.in_stack + (size_t)idx * a.stride;
-    uint64_t avail[3][MAXW];
+    uint64_t avail[3][W];
 #pragma unroll
     for (int K = 0; K < 3; ++K)
 #pragma unroll
-        for (int w = 0; w < MAXW; ++w)
-            avail[K][w] = (w < (int)tv.words(K)) ? tv.b[tv.off_valid(K) + w] : 0ull;
+        for (int w = 0; w < W; ++w) avail[K][w] = tv.b[tv.off_valid(K) + w];
     for (uint32_t j = 0; j < kmin; ++j) {
-        const int K = (int)(tv.pos(j) & 3);
-        const uint32_t t = stack[j];
-        const uint32_t base = t & ~3u;
+        const int K = desc_kind(tv.nbr(j));
+        const uint32_t bb = (uint32_t)stack[j] & ~3u;
 #pragma unroll
-        for (int w = 0; w < MAXW; ++w)
-            if (w == (int)(base >> 6)) avail[K][w] &= ~(0xFull << (base & 63));
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (q == K && w == (int)(bb >> 6)) avail[q][w] &= ~(0xFull << (bb & 63));
     }
-    uint64_t c[MAXW];
-    int K;
-    candidates<MAXW>(tv, kmin, last, [&](uint32_t j) { return (uint32_t)stack[j]; },
-                     [&](int kk, int w) {
-                         uint64_t v = 0;
+    uint64_t c[W];
+    candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t j) { return (uint32_t)stack[j]; },
+                  [&](int kk, int w) {
+                      uint64_t v = 0;
 #pragma unroll
-                         for (int q = 0; q < 3; ++q)
+                      for (int q = 0; q < 3; ++q)
 #pragma unroll
-                             for (int x = 0; x < MAXW; ++x)
-                                 if (q == kk && x == w) v = avail[q][x];
-                         return v;
-                     },
-                     c, K);
+                          for (int x = 0; x < W; ++x)
+                              if (q == kk && x == w) v = avail[q][x];
+                      return v;
+                  },
+                  c);
     uint32_t n = 0;
 #pragma unroll
-    for (int w = 0; w < MAXW; ++w) n += (uint32_t)__popcll(c[w]);
-    if (n) atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
+    for (int w = 0; w < W; ++w) n += (uint32_t)__popcll(c[w]);
     if (!n) return;
+    atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
     if (kmin == tv.P - 1) {
         uint64_t s = atomicAdd(&a.solutions[puzzle], (unsigned long long)n);
-        for (int w = 0; w < MAXW; ++w) {
+        for (int w = 0; w < W; ++w) {
             uint64_t v = c[w];
             while (v) {
                 const uint32_t t = w * 64 + __builtin_ctzll(v);
@@ -284,7 +304,7 @@ __global__ void expand_kernel(DfsArgs a) {
         return;
     }
     uint32_t o = base;
-    for (int w = 0; w < MAXW; ++w) {
+    for (int w = 0; w < W; ++w) {
         uint64_t v = c[w];
         while (v) {
             const uint32_t t = w * 64 + __builtin_ctzll(v);
@@ -300,9 +320,21 @@ __global__ void expand_kernel(DfsArgs a) {
 
 // ---------------------------------------------------------------------------
 // Persistent DFS.
+//
+// A lane's state is the level k it is at and, in registers, the candidates of
+// level k not tried yet (c, all above `last`); levels below k are placed (the
+// transition stack and the available transitions in LDS).  Each iteration
+// computes one candidate set:
+//   * c empty: level k is exhausted; pop to k - 1 and recompute its
+//     candidates above the transition it held;
+//   * else place the first candidate t of c at level k and compute the
+//     candidates of level k + 1 with t's piece taken; descend only when that
+//     set is not empty -- otherwise the lane stays at level k with c intact,
+//     so a dead-end placement costs one candidate set, not two.
 // ---------------------------------------------------------------------------
-template <int MAXW, bool LDS_TABLE, typename STK>
+template <int W, bool LDS_TABLE>
 __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
+    using STK = typename std::conditional<(W <= 4), uint8_t, uint16_t>::type;
     extern __shared__ uint64_t smem[];
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const int wave = (int)(threadIdx.x / kWave);
@@ -324,14 +356,25 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
     bool active = false;
     bool queue_empty = false;
     bool may_split = false;   // at most one failed reservation per item
+    bool fresh = false;       // c not computed yet for the item's level
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0;
-    int last = -1;
+    int last = -1, curK = 0;
     uint64_t steps = 0, nodes = 0;
+    uint64_t c[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) c[w] = 0;
 
-    auto A = [&](int K, int w) -> uint64_t & { return avail[(tv.abase(K) + (uint32_t)w) * kWave + lane]; };
+    // Available word w of kind K; words past the kind's own are read clamped
+    // (the base mask zeroes them).
+    auto A = [&](int K, int w) -> uint64_t & {
+        const uint32_t nw = tv.nat(K);
+        const uint32_t ww = (uint32_t)w < nw ? (uint32_t)w : nw - 1;
+        return avail[(tv.abase(K) + ww) * kWave + lane];
+    };
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
+        atomicAdd(a.candidate_sets, (unsigned long long)steps);
         atomicMax(&a.last_tick[puzzle], (unsigned long long)wall_clock64());
         active = false;
     };
@@ -371,15 +414,16 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
                         steps = 0;
                         nodes = 0;
                         may_split = true;
+                        fresh = true;
                         const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
                         for (int K = 0; K < 3; ++K)
-                            for (uint32_t w = 0, nw = tv.words(K); w < nw; ++w) A(K, (int)w) = tv.b[tv.off_valid(K) + w];
+                            for (uint32_t w = 0, nw = tv.nat(K); w < nw; ++w)
+                                avail[(tv.abase(K) + w) * kWave + lane] = tv.b[tv.off_valid(K) + w];
                         for (uint32_t j = 0; j < k; ++j) {
                             const uint32_t t = src[j];
                             S(j) = (STK)t;
-                            const int K = (int)(tv.pos(j) & 3);
                             const uint32_t bb = t & ~3u;
-                            A(K, (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
+                            A(desc_kind(tv.nbr(j)), (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
                         }
                         atomicMin(&a.first_tick[puzzle], (unsigned long long)wall_clock64());
                         atomicAdd(&a.flags[2], 1u);
@@ -397,48 +441,75 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
         const bool drain = __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         const uint64_t budget = drain ? a.drain_budget : a.step_budget;
         for (uint32_t r = 0; r < a.steps_per_round; ++r) {
-            uint64_t c[MAXW];
-            int K;
-            candidates<MAXW>(tv, k, last, [&](uint32_t j) { return (uint32_t)S(j); },
-                             [&](int kk, int w) { return A(kk, w); }, c, K);
-            const int t = first_bit<MAXW>(c);
-            ++steps;
-            if (t < 0) {
-                if (k == kmin) {
-                    finish();
-                    break;
-                }
-                --k;
-                const uint32_t tb = S(k);
-                const int Kb = (int)(tv.pos(k) & 3);
-                const uint32_t bb = tb & ~3u;
-                const uint32_t wv = bb >> 6;
-                A(Kb, (int)wv) |= (0xFull << (bb & 63)) & tv.b[tv.off_valid(Kb) + wv];
-                last = (int)tb;
-            } else {
-                S(k) = (STK)t;
-                ++nodes;
-                if (k == P - 1) {
-                    const uint64_t s = atomicAdd(&a.solutions[puzzle], 1ull);
-                    if (s < a.sol_per_puzzle) {
-                        uint16_t *dst = a.sol_stack + ((size_t)puzzle * a.sol_per_puzzle + s) * a.stride;
-                        for (uint32_t j = 0; j < P; ++j) dst[j] = S(j);
-                    }
-                    last = t;
-                } else {
-                    const uint32_t bb = (uint32_t)t & ~3u;
-                    A(K, (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
-                    ++k;
-                    last = -1;
+            // One candidate set per iteration, of level tk above tl:
+            //   fresh item   tk = k,     tl = last
+            //   pop          tk = k - 1, tl = transition held there
+            //   place t      tk = k + 1, tl = -1, t's piece taken
+            //   solution t   tk = k,     tl = t (the rest of level k)
+            // Updates are written unconditionally where the value is inert
+            // for the lanes that do not need them, so the body has no
+            // divergent branch except the lane that finishes its item.
+            const int t = fresh ? -1 : first_bit<W>(c);
+            const bool pop = !fresh && t < 0;
+            if (pop && k == kmin) {
+                finish();
+                break;
+            }
+            const bool plc = t >= 0;
+            const bool sol = plc && k == P - 1;
+            const bool down = plc && !sol;
+            const uint32_t km1 = k - (k > 0);
+            const uint32_t held = S(km1);
+            const uint64_t dprev = tv.nbr(km1);
+            // S(k) is dead for every lane that does not place.
+            S(k) = (STK)t;
+            // pop: level k - 1 gets its piece back (a no-op OR otherwise)
+            {
+                const uint32_t bb = held & ~3u;
+                A(desc_kind(dprev), (int)(bb >> 6)) |= pop ? (0xFull << (bb & 63)) : 0ull;
+            }
+            if (sol) {
+                const uint64_t sidx = atomicAdd(&a.solutions[puzzle], 1ull);
+                if (sidx < a.sol_per_puzzle) {
+                    uint16_t *dst = a.sol_stack + ((size_t)puzzle * a.sol_per_puzzle + sidx) * a.stride;
+                    for (uint32_t j = 0; j < P; ++j) dst[j] = S(j);
                 }
             }
+            nodes += plc;
+            ++steps;
+            const uint32_t tk = pop ? km1 : (down ? k + 1 : k);
+            const int tl = pop ? (int)held : (down ? -1 : (sol ? t : last));
+            const uint32_t xw = down ? ((uint32_t)t >> 6) : 0u;
+            const uint64_t xm = down ? (0xFull << ((uint32_t)t & 60)) : 0ull;
+            const int xK = curK;
+            fresh = false;
+            const uint64_t d = tv.nbr(tk);
+            uint64_t cc[W];
+            candidates<W>(tv, tk, d, tl, [&](uint32_t j) { return (uint32_t)S(j); },
+                          [&](int kk, int w) {
+                              const uint64_t v = A(kk, w);
+                              return (kk == xK && (uint32_t)w == xw) ? (v & ~xm) : v;
+                          },
+                          cc);
+            // A placement whose next level has no candidate is a dead end:
+            // the lane stays at level k with the rest of c.
+            const bool stay = down && !any_bit<W>(cc);
+            const uint64_t tbit = plc ? (1ull << (t & 63)) : 0ull;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                c[w] = stay ? (c[w] & ~((w == (t >> 6)) ? tbit : 0ull)) : cc[w];
+            // commit the take when descending (a no-op AND otherwise)
+            A(xK, (int)xw) &= (down && !stay) ? ~xm : ~0ull;
+            k = stay ? k : tk;
+            last = stay ? t : tl;
+            curK = stay ? curK : desc_kind(d);
             if (steps >= budget && may_split) {
                 // Split in two: (a) the candidates above the current choice at
                 // the floor index -- usually the bulk of what remains -- and
-                // (b) the search in progress below that choice, resumed at
-                // index k above `last` with the floor one index deeper.  At the
-                // floor itself the item is simply requeued.  Each item splits at
-                // most once per pass, so a pass emits at most twice its input.
+                // (b) the search in progress at level k above `last`, with the
+                // floor one index deeper.  At the floor itself the item is
+                // simply requeued.  Each item splits at most once per pass, so
+                // a pass emits at most twice its input.
                 const uint32_t n = k > kmin ? 2u : 1u;
                 uint32_t base = 0xFFFFFFFFu;
                 if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
@@ -481,50 +552,47 @@ __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
     if (i < n) p[i] = v;
 }
 
-template <int MAXW>
+template <int W>
 void launch_expand(const DfsArgs &a, hipStream_t s) {
     const uint32_t threads = 256, blocks = (a.n_in + threads - 1) / threads;
-    if (blocks) hipLaunchKernelGGL(expand_kernel<MAXW>, dim3(blocks), dim3(threads), 0, s, a);
+    if (blocks) hipLaunchKernelGGL(expand_kernel<W>, dim3(blocks), dim3(threads), 0, s, a);
 }
 
-template <int MAXW>
-void launch_dfs(const DfsArgs &a, bool lds, bool narrow, uint32_t blocks, uint32_t threads, size_t lds_bytes,
-                hipStream_t s) {
-    if (lds && narrow)
-        hipLaunchKernelGGL((dfs_kernel<MAXW, true, uint8_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-    else if (lds)
-        hipLaunchKernelGGL((dfs_kernel<MAXW, true, uint16_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-    else if (narrow)
-        hipLaunchKernelGGL((dfs_kernel<MAXW, false, uint8_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL((dfs_kernel<MAXW, false, uint16_t>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+template <int W>
+void launch_dfs(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes, hipStream_t s) {
+    if (lds) hipLaunchKernelGGL((dfs_kernel<W, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    else hipLaunchKernelGGL((dfs_kernel<W, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
 }
 
-void dispatch_expand(int maxw, const DfsArgs &a, hipStream_t s) {
-    switch (maxw) {
+void dispatch_expand(int w, const DfsArgs &a, hipStream_t s) {
+    switch (w) {
         case 1: launch_expand<1>(a, s); break;
         case 2: launch_expand<2>(a, s); break;
+        case 3: launch_expand<3>(a, s); break;
         case 4: launch_expand<4>(a, s); break;
+        case 6: launch_expand<6>(a, s); break;
         case 8: launch_expand<8>(a, s); break;
         default: launch_expand<16>(a, s); break;
     }
 }
 
-void dispatch_dfs(int maxw, const DfsArgs &a, bool lds, bool narrow, uint32_t blocks, uint32_t threads,
-                  size_t lds_bytes, hipStream_t s) {
-    switch (maxw) {
-        case 1: launch_dfs<1>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
-        case 2: launch_dfs<2>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
-        case 4: launch_dfs<4>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
-        case 8: launch_dfs<8>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
-        default: launch_dfs<16>(a, lds, narrow, blocks, threads, lds_bytes, s); break;
+void dispatch_dfs(int w, const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes,
+                  hipStream_t s) {
+    switch (w) {
+        case 1: launch_dfs<1>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 2: launch_dfs<2>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 3: launch_dfs<3>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 4: launch_dfs<4>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 6: launch_dfs<6>(a, lds, blocks, threads, lds_bytes, s); break;
+        case 8: launch_dfs<8>(a, lds, blocks, threads, lds_bytes, s); break;
+        default: launch_dfs<16>(a, lds, blocks, threads, lds_bytes, s); break;
     }
 }
 
-int round_words(uint32_t w) {
-    if (w <= 1) return 1;
-    if (w <= 2) return 2;
-    if (w <= 4) return 4;
+// Row widths with a compiled kernel.
+uint32_t round_words(uint32_t w) {
+    if (w <= 4) return w < 1 ? 1 : w;
+    if (w <= 6) return 6;
     if (w <= 8) return 8;
     return 16;
 }
@@ -563,26 +631,27 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     stats = EnumStats{};
     auto wall0 = std::chrono::steady_clock::now();
 
-    // Tables.
+    // Tables, every mask row padded to the launch's row width.
+    uint32_t natural = 1;
+    for (uint32_t i = 0; i < n; ++i) natural = std::max(natural, natural_row_words(*puzzles[i]));
+    if (natural > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
+    const uint32_t row = round_words(natural);
     std::vector<EnumTable> tables;
     tables.reserve(n);
     std::vector<uint32_t> table_off(n);
-    uint32_t total_words = 0, max_words = 1, max_avail = 1, max_P = 0, max_table = 0, max_transitions = 0;
+    uint32_t total_words = 0, max_avail = 1, max_P = 0, max_table = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        tables.push_back(build_enum_table(*puzzles[i], cfg.order));
+        tables.push_back(build_enum_table(*puzzles[i], cfg.order, row));
         EnumTableHeader h = tables.back().header();
         table_off[i] = total_words;
         total_words += h.total_words;
-        for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
         max_avail = std::max(max_avail, h.avail_words);
         max_P = std::max(max_P, h.P);
-        for (int k = 0; k < 3; ++k) max_transitions = std::max<uint32_t>(max_transitions, 4 * (uint32_t)puzzles[i]->pieces_of(k).size());
         max_table = std::max(max_table, h.total_words);
     }
-    if (max_words > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
-    const int maxw = round_words(max_words);
+    const int maxw = (int)row;
     const uint32_t stride = (max_P + 3) & ~3u;   // u16 per item, multiple of 4
-    stats.max_words = max_words;
+    stats.max_words = natural;
 
     std::vector<uint64_t> blob(total_words);
     for (uint32_t i = 0; i < n; ++i)
@@ -592,7 +661,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // The kernels use about 80-100 VGPRs (5 waves per SIMD, 20 per CU), the
     // 16-word variant about 150 (3 per SIMD, 12 per CU).
     const bool lds_table = n == 1;
-    const bool narrow = max_transitions <= 256;
+    const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
     const uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
     const uint32_t threads = wpb * kWave;
     const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
@@ -629,6 +698,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned long long *d_limit = ws.get<unsigned long long>("limit", n);
     unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
     unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
+    unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
     const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
     uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
 
@@ -671,6 +741,11 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.flags = d_counters + 2;
     a.sol_stack = d_sol_stack;
     a.sol_per_puzzle = sol_per_puzzle;
+    a.candidate_sets = d_csets;
+    {
+        const unsigned long long zero = 0;
+        EMP_CHECK(hipMemcpyAsync(d_csets, &zero, 8, hipMemcpyHostToDevice, stream));
+    }
 
     hipEvent_t ev0, ev1;
     EMP_CHECK(hipEventCreate(&ev0));
@@ -711,7 +786,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
         EMP_CHECK(hipEventRecord(ev0, stream));
         if (expand) dispatch_expand(maxw, a, stream);
-        else dispatch_dfs(maxw, a, lds_table, narrow, blocks, threads, lds_bytes, stream);
+        else dispatch_dfs(maxw, a, lds_table, blocks, threads, lds_bytes, stream);
         EMP_CHECK(hipGetLastError());
         EMP_CHECK(hipEventRecord(ev1, stream));
         unsigned int counters[8];
@@ -755,7 +830,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     EMP_CHECK(hipMemcpyAsync(r_first.data(), d_first, n * 8, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipMemcpyAsync(r_last.data(), d_last, n * 8, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipMemcpyAsync(r_trunc.data(), d_trunc, n * 4, hipMemcpyDeviceToHost, stream));
+    unsigned long long r_csets = 0;
+    EMP_CHECK(hipMemcpyAsync(&r_csets, d_csets, 8, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipStreamSynchronize(stream));
+    stats.candidate_sets = r_csets;
     std::vector<uint16_t> r_stack((size_t)n * sol_per_puzzle * stride);
     if (!r_stack.empty()) {
         EMP_CHECK(hipMemcpyAsync(r_stack.data(), d_sol_stack, r_stack.size() * 2, hipMemcpyDeviceToHost, stream));
@@ -784,12 +862,12 @@ void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, 
                       uint32_t depth, uint64_t capacity, std::vector<uint16_t> &boards, uint64_t &n_items,
                       uint64_t &nodes, uint64_t &solutions) {
     (void)dev;
-    const EnumTable t = build_enum_table(pz, order);
+    const uint32_t natural = natural_row_words(pz);
+    if (natural > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
+    const int maxw = (int)round_words(natural);
+    const EnumTable t = build_enum_table(pz, order, (uint32_t)maxw);
     const EnumTableHeader h = t.header();
     if (depth < 1 || depth > h.P) throw std::invalid_argument("expansion depth must be in 1..P");
-    uint32_t max_words = 1;
-    for (int k = 0; k < 3; ++k) max_words = std::max(max_words, h.words[k]);
-    const int maxw = round_words(max_words);
     const uint32_t stride = (h.P + 3) & ~3u;
     // Device queues hold whole levels; `capacity` only bounds the boards returned.
     const uint32_t cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(capacity, 1u << 22), 1u << 25);

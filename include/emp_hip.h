@@ -133,6 +133,7 @@ typedef struct {
     uint32_t lds_tables;       /* 1: tables held in LDS                        */
     uint32_t blocks;           /* grid of the DFS kernel                       */
     uint32_t threads;          /* block size of the DFS kernel                 */
+    uint64_t candidate_sets;   /* DFS iterations, one candidate set each       */
 } emp_hip_enum_stats;
 
 void emp_hip_enum_default_options(emp_hip_enum_options *options);
