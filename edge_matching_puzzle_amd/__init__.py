@@ -91,7 +91,18 @@ SIGNATURES = {
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
+    "emp_hip_mlp_train": (_I32, [_P, _P, _P, _U64, _P, _P, _P]),
+    "emp_hip_mlp_score": (_I32, [_P, _P, _P, _U64, _P]),
+    "emp_hip_mlp_gradients": (_I32, [_P, _P, _P, _P, _U64, _P, _P]),
 }
+
+MLP_INPUTS = 126
+MLP_PARAMS = 99073
+
+
+class MlpOptions(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("reserved", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
 
 
 def lib():
@@ -207,6 +218,35 @@ class Context:
         cu, lds, clk = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().emp_hip_device_info(self._ctx, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(clk)))
         return {"compute_units": cu.value, "lds_per_block": lds.value, "clock_khz": clk.value}
+
+    def mlp_train(self, features, targets, params, epochs=4, batch=1024, lr=1e-3, seed=1):
+        """Adam on the scorer's MSE; returns (params, loss per epoch)."""
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_INPUTS)
+        y = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        p = np.array(params, dtype=np.float32).reshape(MLP_PARAMS)
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        opt = MlpOptions(epochs, batch, lr, 0, seed)
+        _check(lib().emp_hip_mlp_train(self._ctx, _ptr(x), _ptr(y), x.shape[0], ctypes.byref(opt), _ptr(p),
+                                       _ptr(losses)))
+        return p, losses[:epochs]
+
+    def mlp_score(self, params, features):
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_INPUTS)
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(MLP_PARAMS)
+        out = np.zeros(x.shape[0], dtype=np.float32)
+        _check(lib().emp_hip_mlp_score(self._ctx, _ptr(p), _ptr(x), x.shape[0], _ptr(out)))
+        return out
+
+    def mlp_gradients(self, params, features, targets):
+        """(loss, gradient of every parameter) over one batch of rows."""
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_INPUTS)
+        y = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(MLP_PARAMS)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().emp_hip_mlp_gradients(self._ctx, _ptr(p), _ptr(x), _ptr(y), x.shape[0], _ptr(g),
+                                           ctypes.byref(loss)))
+        return loss.value, g
 
     def glutton_max(self, width, height, colours, prefixes=None, push_budget=0, waves_per_cu=0,
                     n_searches=None):

@@ -15,6 +15,7 @@
 #include "emp_enum_plan.h"
 #include "emp_enumerate.h"
 #include "emp_glutton.h"
+#include "emp_mlp.h"
 #include "emp_model.h"
 
 #include <unordered_set>
@@ -80,6 +81,8 @@ int emp_hip_create(int device, emp_hip_context **context) {
         ctx->dev.device = device;
         ctx->dev.compute_units = (uint32_t)prop.multiProcessorCount;
         ctx->dev.lds_per_block = prop.sharedMemPerBlock;
+        ctx->dev.lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                    : prop.sharedMemPerBlock;
         ctx->dev.clock_khz = (uint32_t)prop.clockRate;
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
             return fail(EMP_HIP_DEVICE_ERROR, "hipStreamCreate failed");
@@ -180,8 +183,8 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
         if (options) opt = *options;
         if (opt.order != EMP_HIP_ORDER_SPIRAL && opt.order != EMP_HIP_ORDER_BASIC)
             return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
-        if (opt.waves_per_block > 4 || (opt.waves_per_block & (opt.waves_per_block - 1)))
-            return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be 1, 2 or 4");
+        if (opt.waves_per_block > 16)
+            return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be at most 16");
         if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
         std::vector<std::unique_ptr<emp::Puzzle>> owned;
         std::vector<const emp::Puzzle *> list;
@@ -340,3 +343,49 @@ int emp_hip_spiral_prefixes(const emp_hip_puzzle *puzzle, uint32_t length, uint3
 }
 
 }  // extern "C"
+
+static_assert(EMP_HIP_MLP_PARAMS == emp::kMlpParams, "parameter count of the C ABI");
+static_assert(EMP_HIP_MLP_INPUTS == emp::kMlpIn, "feature count of the C ABI");
+
+int emp_hip_mlp_train(emp_hip_context *context, const float *features, const float *targets, uint64_t n_rows,
+                      const emp_hip_mlp_options *options, float *params, float *loss_per_epoch) {
+    return guarded([&]() -> int {
+        if (!context || !features || !targets || !options || !params || !n_rows)
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument or no rows");
+        if (options->batch == 0 || options->batch % 64 || options->batch > 65536)
+            return fail(EMP_HIP_BAD_ARGUMENT, "batch must be a multiple of 64, at most 65536");
+        if (n_rows > (1ull << 31)) return fail(EMP_HIP_BAD_ARGUMENT, "too many rows");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::MlpTrainConfig cfg;
+        cfg.epochs = options->epochs;
+        cfg.batch = options->batch;
+        cfg.lr = options->lr;
+        cfg.seed = options->seed;
+        std::vector<float> losses;
+        emp::mlp_train_on_device(context->stream, context->workspace, features, targets, n_rows, cfg, params, losses);
+        if (loss_per_epoch) std::copy(losses.begin(), losses.end(), loss_per_epoch);
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_mlp_score(emp_hip_context *context, const float *params, const float *features, uint64_t n_rows,
+                      float *scores) {
+    return guarded([&]() -> int {
+        if (!context || !params || (n_rows && (!features || !scores))) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::mlp_score_on_device(context->stream, context->workspace, params, features, n_rows, scores);
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_mlp_gradients(emp_hip_context *context, const float *params, const float *features,
+                          const float *targets, uint64_t n_rows, float *grads, float *loss) {
+    return guarded([&]() -> int {
+        if (!context || !params || !features || !targets || !grads || !loss)
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (n_rows == 0 || n_rows > 65536) return fail(EMP_HIP_BAD_ARGUMENT, "rows must be in 1..65536");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        emp::mlp_gradients_on_device(context->stream, context->workspace, params, features, targets, n_rows, grads, *loss);
+        return EMP_HIP_OK;
+    });
+}

@@ -20,11 +20,12 @@ uint32_t natural_row_words(const Puzzle &pz) {
     return w;
 }
 
-EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
+EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint32_t stack_bytes) {
     EnumTable t;
     EnumTableHeader h{};
     const uint32_t P = pz.size();
     if (P >= kOffBoard) throw std::domain_error("enumeration tables index at most 509 positions");
+    if (pz.max_colour() > 255) throw std::domain_error("enumeration tables hold colours up to 255");
     h.P = P;
     h.W = pz.width();
     h.H = pz.height();
@@ -85,7 +86,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
     h.off_pos = off;
     off += P;
     h.off_nbr = off;
-    off += P;
+    off += P * kNbrWords;
     h.off_base = off;
     off += P * R;
     h.off_ones = off;
@@ -102,6 +103,14 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
         h.off_rot[k] = off;
         off += (4 * n_kind[k] + 1) / 2;
     }
+    uint32_t col_bytes[3], col_total = 0;
+    for (int k = 0; k < 3; ++k) {
+        col_bytes[k] = col_total;
+        col_total += 4 * 4 * n_kind[k];
+    }
+    if (col_total > 0xFFFF) throw std::domain_error("colour rows exceed 64 KiB");
+    h.off_col = off;
+    off += (col_total + 7) / 8;
     h.total_words = off;
     t.blob.assign(off, 0);
     uint64_t *b = t.blob.data();
@@ -159,6 +168,17 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
     }
 
     for (uint32_t w = 0; w < 16; ++w) b[h.off_ones + w] = ~0ull;
+    const uint32_t sb = stack_bytes ? stack_bytes : (R <= 4 ? 1 : 2);
+    {
+        uint8_t *col = reinterpret_cast<uint8_t *>(b + h.off_col);
+        for (int k = 0; k < 3; ++k)
+            for (uint32_t ki = 0; ki < n_kind[k]; ++ki) {
+                const Piece &p = pz.piece(t.piece_of[k][ki]);
+                for (unsigned o = 0; o < 4; ++o)
+                    for (unsigned q = 0; q < 4; ++q)
+                        col[col_bytes[k] + q * 4 * n_kind[k] + 4 * ki + o] = (uint8_t)shown(p, q, o);
+            }
+    }
 
     // Base masks (valid transitions matching the frame sides) and neighbour
     // descriptors.
@@ -167,7 +187,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
         const int k = (int)(rec & 3);
         uint64_t *base = b + h.off_base + (size_t)i * R;
         for (uint32_t w = 0; w < R; ++w) base[w] = b[h.off_valid[k] + w];
-        uint64_t d = (uint64_t)k << 3;
+        uint64_t *rec_out = b + h.off_nbr + (size_t)i * kNbrWords;
         uint32_t n = 0;
         for (unsigned s = 0; s < 4; ++s) {
             const uint32_t code = (uint32_t)(rec >> (2 + 11 * s)) & 0x1FF;
@@ -178,10 +198,15 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words) {
                 continue;
             }
             const uint32_t nk = (uint32_t)(rec >> (11 + 11 * s)) & 3;
-            d |= (uint64_t)(code | nk << 9 | s << 11) << (kNbrShift + kNbrBits * n);
+            const uint32_t facing = (s + 2) & 3;
+            const uint64_t stk_off = (uint64_t)code * 64 * sb;
+            const uint64_t col_off = col_bytes[nk] + facing * 4 * n_kind[nk];
+            const uint64_t cm_block = h.off_cm[k] + (uint64_t)s * h.ncol * R;
+            if (stk_off > 0xFFFF || cm_block >= (1u << 23)) throw std::domain_error("table offsets exceed their fields");
+            rec_out[1 + n] = stk_off | col_off << 16 | cm_block << 32 | (uint64_t)code << 55;
             ++n;
         }
-        b[h.off_nbr + i] = d | n;
+        rec_out[0] = (uint64_t)k << 3 | n;
     }
 
     // Root: first candidate of index 0, where no neighbour is placed yet.

@@ -30,13 +30,18 @@ namespace emp {
 constexpr uint32_t kOffBoard = 0x1FE;
 constexpr uint32_t kUnconstrained = 0x1FF;
 
-// Neighbour descriptor (one u64 per strategy index), the form the kernels use:
-//   bits 0-2            number n of neighbours placed earlier (0..4)
-//   bits 3-4            kind of the position
-//   bits 5+13i .. +12   i-th such neighbour (i < n): strategy index (9 bits),
-//                       its kind << 9, the side of this position facing it << 11
+// Neighbour record (kNbrWords u64 per strategy index), the form the kernels
+// use.  Word 0: number n of neighbours placed earlier (0..4) in bits 0-2, the
+// position's kind in bits 3-4.  Words 1..4, the i-th such neighbour (i < n):
+//   bits 0-15   byte offset of its level in a wave's LDS transition stack
+//               (index * 64 lanes * bytes per transition)
+//   bits 16-31  byte offset in the colour bytes of the row (neighbour kind,
+//               side facing this position)
+//   bits 32-54  u64 offset of the colour-mask block of (this kind, the side
+//               facing the neighbour): off_cm[K] + side * ncol * row
+//   bits 55-63  its strategy index
 // Frame sides are folded into the position's base mask instead.
-constexpr uint32_t kNbrShift = 5, kNbrBits = 13;
+constexpr uint32_t kNbrWords = 5;
 
 struct EnumTableHeader {
     uint32_t P, W, H, ncol;      // ncol = max colour + 1 (colour 0 = frame)
@@ -52,10 +57,10 @@ struct EnumTableHeader {
     uint32_t avail_words;        // words[0] + words[1] + words[2]
     uint32_t row;                // words of every mask row (>= each words[K]; padding is zero)
     uint32_t off_base;           // [P][row]: valid transitions of the position's kind that match its frame sides
-    uint32_t off_nbr;            // [P] neighbour descriptors
+    uint32_t off_nbr;            // [P][kNbrWords] neighbour records
     uint32_t off_ones;           // 16 all-ones words, then 16 all-zeros words
     uint32_t off_cls[3];         // u16 per transition: its appearance class, 0xFFFF for none
-    uint32_t reserved;
+    uint32_t off_col;            // colour bytes: per kind, per side, one byte per transition
 };
 static_assert(sizeof(EnumTableHeader) % 8 == 0, "header must be u64 aligned");
 constexpr uint32_t kHeaderWords = sizeof(EnumTableHeader) / 8;
@@ -72,7 +77,8 @@ uint32_t natural_row_words(const Puzzle &puzzle);
 
 // Builds the table with mask rows of max(row_words, natural_row_words) words;
 // throws std::invalid_argument when the first position has no candidate.
-EnumTable build_enum_table(const Puzzle &puzzle, int order, uint32_t row_words = 0);
+// stack_bytes: bytes per transition in the kernels' LDS stacks (1 or 2).
+EnumTable build_enum_table(const Puzzle &puzzle, int order, uint32_t row_words = 0, uint32_t stack_bytes = 0);
 
 // Converts a stack of transitions (one per strategy index) to a row-major board.
 void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *board);

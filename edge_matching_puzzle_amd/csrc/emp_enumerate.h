@@ -51,6 +51,7 @@ struct DeviceInfo {
     int device = 0;
     uint32_t compute_units = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
     uint32_t clock_khz = 0;
 };
 

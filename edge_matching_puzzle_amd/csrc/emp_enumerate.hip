@@ -105,7 +105,7 @@ struct TableView {
     uint32_t P, ncol;
     uint32_t n0, n1, n2, d0, d1, d2;
     uint32_t cm0, cm1, cm2, va0, va1, va2, du0, du1, du2, ro0, ro1, ro2;
-    uint32_t off_base, off_nbr, off_ones, off_zero, cl0, cl1, cl2;
+    uint32_t off_base, off_nbr, off_ones, off_zero, cl0, cl1, cl2, off_col;
 
     __device__ __forceinline__ void load(const uint64_t *blob) {
         b = blob;
@@ -124,6 +124,15 @@ struct TableView {
         off_ones = h[29];
         off_zero = h[29] + 16;
         cl0 = h[30]; cl1 = h[31]; cl2 = h[32];
+        off_col = h[33];
+    }
+    // One table for the whole grid: keep its fields in scalar registers.
+    __device__ __forceinline__ void make_uniform() {
+        uint32_t *f[] = {&P, &ncol, &n0, &n1, &n2, &d0, &d1, &d2, &cm0, &cm1, &cm2, &va0, &va1, &va2,
+                         &du0, &du1, &du2, &ro0, &ro1, &ro2, &off_base, &off_nbr, &off_ones, &off_zero,
+                         &cl0, &cl1, &cl2, &off_col};
+#pragma unroll
+        for (uint32_t *x : f) *x = __builtin_amdgcn_readfirstlane(*x);
     }
     // natural words of kind K (per-lane available-transition layout)
     __device__ __forceinline__ uint32_t nat(int k) const { return sel3(k, n0, n1, n2); }
@@ -132,7 +141,12 @@ struct TableView {
     __device__ __forceinline__ uint32_t off_cm(int k) const { return sel3(k, cm0, cm1, cm2); }
     __device__ __forceinline__ uint32_t off_valid(int k) const { return sel3(k, va0, va1, va2); }
     __device__ __forceinline__ uint32_t off_dup(int k) const { return sel3(k, du0, du1, du2); }
-    __device__ __forceinline__ uint64_t nbr(uint32_t k) const { return b[off_nbr + k]; }
+    // word 0 of position k's neighbour record: count | kind << 3
+    __device__ __forceinline__ uint64_t nbr(uint32_t k) const { return b[off_nbr + k * kNbrWords]; }
+    __device__ __forceinline__ const uint64_t *nbr_rec(uint32_t k) const { return b + off_nbr + k * kNbrWords; }
+    __device__ __forceinline__ uint32_t colour(uint32_t byte_off) const {
+        return reinterpret_cast<const uint8_t *>(b + off_col)[byte_off];
+    }
     __device__ __forceinline__ uint32_t cls(int kind, uint32_t t) const {
         return reinterpret_cast<const uint16_t *>(b + sel3(kind, cl0, cl1, cl2))[t];
     }
@@ -145,15 +159,6 @@ static_assert(sizeof(EnumTableHeader) == 34 * 4, "TableView::load reads EnumTabl
 
 __device__ __forceinline__ int desc_kind(uint64_t d) { return (int)(d >> 3) & 3; }
 
-// Candidates of strategy index k (descriptor d): its base mask (valid
-// transitions matching the frame) AND the available transitions of its kind
-// AND one colour mask per neighbour placed earlier; in each appearance class
-// only the lowest piece present keeps its transitions; then only transitions
-// above `last`.  The class reduction is applied to the full candidate set, as
-// the reference's identical-piece masks accumulate from the first candidate
-// on.  get_choice(j): transition placed at index j; get_avail(K, w): available
-// transition word w of kind K (only words below the kind's natural count
-// matter: the base mask is zero above).
 template <int W>
 __device__ __forceinline__ int first_bit(const uint64_t (&c)[W]) {
     int t = -1;
@@ -171,6 +176,15 @@ __device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
     return v != 0;
 }
 
+// Candidates of strategy index k (record word 0 = d): its base mask (valid
+// transitions matching the frame) AND the available transitions of its kind
+// AND one colour mask per neighbour placed earlier; in each appearance class
+// only the lowest piece present keeps its transitions; then only transitions
+// above `last`.  The class reduction is applied to the full candidate set, as
+// the reference's identical-piece masks accumulate from the first candidate
+// on.  get_choice(stack_byte_offset, j): transition placed at index j;
+// get_avail(K, w): available transition word w of kind K (only words below
+// the kind's natural count matter: the base mask is zero above).
 template <int W, typename GetChoice, typename GetAvail>
 __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint64_t d, int last,
                                            GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W]) {
@@ -181,15 +195,15 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
     for (int w = 0; w < W; ++w) c[w] = base[w] & get_avail(K, w);
     // Neighbour masks.  The loop runs to the wave's largest count; a lane
     // past its own count ANDs the all-ones row, so the body has no branch.
-    const uint64_t *cmK = tv.b + tv.off_cm(K);
+    const uint64_t *rec = tv.nbr_rec(k) + 1;
 #pragma unroll
     for (uint32_t i = 0; i < 4; ++i) {
         const bool on = i < n;
         if (__ballot(on) == 0) break;
-        const uint32_t e = (uint32_t)(d >> (kNbrShift + kNbrBits * i)) & 0x1FFF;
-        const uint32_t j = on ? (e & 0x1FF) : 0u, nk = (e >> 9) & 3, s = e >> 11;
-        const uint32_t col = (tv.rot((int)nk, get_choice(j)) >> (8 * ((s + 2) & 3))) & 0xFF;
-        const uint64_t *row = on ? cmK + (s * tv.ncol + col) * W : tv.b + tv.off_ones;
+        const uint64_t e = on ? rec[i] : 0ull;
+        const uint32_t lo = (uint32_t)e, hi = (uint32_t)(e >> 32);
+        const uint32_t col = tv.colour((lo >> 16) + get_choice(lo & 0xFFFF, hi >> 23));
+        const uint64_t *row = on ? tv.b + (hi & 0x7FFFFF) + col * W : tv.b + tv.off_ones;
 #pragma unroll
         for (int w = 0; w < W; ++w) c[w] &= row[w];
     }
@@ -264,7 +278,7 @@ __global__ void expand_kernel(DfsArgs a) {
                 if (q == K && w == (int)(bb >> 6)) avail[q][w] &= ~(0xFull << (bb & 63));
     }
     uint64_t c[W];
-    candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t j) { return (uint32_t)stack[j]; },
+    candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t, uint32_t j) { return (uint32_t)stack[j]; },
                   [&](int kk, int w) {
                       uint64_t v = 0;
 #pragma unroll
@@ -332,8 +346,12 @@ __global__ void expand_kernel(DfsArgs a) {
 //     set is not empty -- otherwise the lane stays at level k with c intact,
 //     so a dead-end placement costs one candidate set, not two.
 // ---------------------------------------------------------------------------
+// Blocks of up to 16 waves for narrow rows; wide rows need more registers,
+// so their blocks stay at 4 waves.
+constexpr uint32_t dfs_max_threads(int w) { return w <= 4 ? 1024u : 256u; }
+
 template <int W, bool LDS_TABLE>
-__global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
+__global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     using STK = typename std::conditional<(W <= 4), uint8_t, uint16_t>::type;
     extern __shared__ uint64_t smem[];
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -351,7 +369,10 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
     STK *stk = reinterpret_cast<STK *>(avail + a.avail_words * kWave);
 
     TableView tv;
-    if (LDS_TABLE) tv.load(smem);
+    if (LDS_TABLE) {
+        tv.load(smem);
+        tv.make_uniform();
+    }
 
     bool active = false;
     bool queue_empty = false;
@@ -485,7 +506,11 @@ __global__ void __launch_bounds__(256) dfs_kernel(DfsArgs a) {
             fresh = false;
             const uint64_t d = tv.nbr(tk);
             uint64_t cc[W];
-            candidates<W>(tv, tk, d, tl, [&](uint32_t j) { return (uint32_t)S(j); },
+            candidates<W>(tv, tk, d, tl,
+                          [&](uint32_t off, uint32_t) {
+                              return (uint32_t)*reinterpret_cast<const STK *>(
+                                  reinterpret_cast<const uint8_t *>(stk + lane) + off);
+                          },
                           [&](int kk, int w) {
                               const uint64_t v = A(kk, w);
                               return (kk == xK && (uint32_t)w == xw) ? (v & ~xm) : v;
@@ -556,6 +581,32 @@ template <int W>
 void launch_expand(const DfsArgs &a, hipStream_t s) {
     const uint32_t threads = 256, blocks = (a.n_in + threads - 1) / threads;
     if (blocks) hipLaunchKernelGGL(expand_kernel<W>, dim3(blocks), dim3(threads), 0, s, a);
+}
+
+template <int W>
+const void *dfs_fn(bool lds) {
+    return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true>)
+               : reinterpret_cast<const void *>(&dfs_kernel<W, false>);
+}
+
+// VGPRs of the DFS kernel for this row width and table placement.
+uint32_t dfs_vgprs(int w, bool lds) {
+    const void *fn = nullptr;
+    switch (w) {
+        case 1: fn = dfs_fn<1>(lds); break;
+        case 2: fn = dfs_fn<2>(lds); break;
+        case 3: fn = dfs_fn<3>(lds); break;
+        case 4: fn = dfs_fn<4>(lds); break;
+        case 6: fn = dfs_fn<6>(lds); break;
+        case 8: fn = dfs_fn<8>(lds); break;
+        default: fn = dfs_fn<16>(lds); break;
+    }
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, fn) != hipSuccess || attr.numRegs <= 0) {
+        (void)hipGetLastError();
+        return 128;
+    }
+    return (uint32_t)attr.numRegs;
 }
 
 template <int W>
@@ -657,19 +708,36 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     for (uint32_t i = 0; i < n; ++i)
         std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
 
-    // Launch geometry: as many resident waves as LDS and registers allow.
-    // The kernels use about 80-100 VGPRs (5 waves per SIMD, 20 per CU), the
-    // 16-word variant about 150 (3 per SIMD, 12 per CU).
+    // Launch geometry: as many resident waves as registers (512 VGPRs per
+    // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
+    // wave's stacks) allow; the block size that fits the most waves wins.
     const bool lds_table = n == 1;
     const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
-    const uint32_t wpb = cfg.waves_per_block ? cfg.waves_per_block : 4;
-    const uint32_t threads = wpb * kWave;
     const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
-    const size_t lds_bytes = (lds_table ? (size_t)max_table * 8 : 0) + wpb * per_wave_bytes;
+    const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
+    const size_t lds_cu = dev.lds_per_cu ? dev.lds_per_cu : dev.lds_per_block;
+    const uint32_t vgprs = dfs_vgprs(maxw, lds_table);
+    const uint32_t reg_waves = 4 * std::min<uint32_t>(8, 512 / ((vgprs + 7) & ~7u));
+    auto resident = [&](uint32_t w) -> uint32_t {
+        const size_t bytes = table_bytes + w * per_wave_bytes;
+        if (bytes > dev.lds_per_block) return 0;
+        return std::min<uint32_t>((uint32_t)(lds_cu / bytes) * w, reg_waves / w * w);
+    };
+    const uint32_t max_wpb = dfs_max_threads(maxw) / kWave;
+    uint32_t wpb = cfg.waves_per_block;
+    if (!wpb) {
+        wpb = 4;
+        // multiples of 4 waves spread evenly over the CU's four SIMDs
+        for (uint32_t w : {4u, 8u, 12u, 16u})
+            if (w <= max_wpb && resident(w) > resident(wpb)) wpb = w;
+    }
+    if (wpb > max_wpb)
+        throw std::invalid_argument("at most " + std::to_string(max_wpb) + " waves per block for this puzzle size");
+    const uint32_t threads = wpb * kWave;
+    const size_t lds_bytes = table_bytes + wpb * per_wave_bytes;
     if (lds_bytes > dev.lds_per_block)
         throw Unsupported("per-block LDS of " + std::to_string(lds_bytes) + " bytes exceeds the device limit");
-    const uint32_t resident_blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)(dev.lds_per_block / lds_bytes), (maxw >= 16 ? 12u : 20u) / wpb));
-    const uint32_t wpc = cfg.waves_per_cu ? cfg.waves_per_cu : resident_blocks * wpb;
+    const uint32_t wpc = cfg.waves_per_cu ? cfg.waves_per_cu : std::max(wpb, resident(wpb));
     const uint32_t blocks = std::max(1u, dev.compute_units * wpc / wpb);
     const uint64_t lanes = (uint64_t)blocks * threads;
     stats.lds_tables = lds_table;

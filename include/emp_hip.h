@@ -99,7 +99,9 @@ int emp_hip_situation_string(uint32_t width, uint32_t height, const uint16_t *bo
 typedef struct {
     uint32_t order;            /* EMP_HIP_ORDER_*                               */
     uint32_t waves_per_cu;     /* 0 = tuned default                             */
-    uint32_t waves_per_block;  /* 0 = tuned default (1, 2 or 4)                 */
+    uint32_t waves_per_block;  /* 0 = sized to fit the most resident waves;
+                                  at most 16 (4 for boards with more than 64
+                                  pieces of one kind)                          */
     uint32_t steps_per_round;  /* steps between work refills, 0 = default       */
     uint64_t step_budget;      /* steps per work item per pass before it is
                                   split, 0 = default                           */
@@ -230,6 +232,40 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle,
 int emp_hip_spiral_prefixes(const emp_hip_puzzle *puzzle, uint32_t length, uint32_t count,
                             uint64_t seed, const uint32_t *fixed_moves, uint32_t n_fixed,
                             uint32_t *out, uint32_t *n_out);
+
+/* ------------------------------------------------------------------------
+ * Candidate scorer (no counterpart in the reference: it only orders the
+ * candidates of the search and never removes one).  A 126-256-256-1
+ * perceptron, ReLU hidden layers, BF16 dense-layer weights with FP32
+ * accumulation on the matrix cores, FP32 master parameters:
+ *   W1[128][256] (rows 126 and 127 stay 0), b1[256], W2[256][256], b2[256],
+ *   w3[256], b3  -- EMP_HIP_MLP_PARAMS floats in that order.
+ * Features are rows of EMP_HIP_MLP_INPUTS floats; targets one float per row.
+ * ---------------------------------------------------------------------- */
+#define EMP_HIP_MLP_INPUTS 126
+#define EMP_HIP_MLP_PARAMS 99073
+
+typedef struct {
+    uint32_t epochs;           /* passes over the rows                          */
+    uint32_t batch;            /* rows per Adam step, multiple of 64, <= 65536  */
+    float    lr;               /* Adam step size                                */
+    uint32_t reserved;
+    uint64_t seed;             /* batch order                                   */
+} emp_hip_mlp_options;
+
+/* Trains params (in/out) on the mean squared error of score vs target;
+ * loss_per_epoch (may be NULL) receives each epoch's mean batch loss. */
+int emp_hip_mlp_train(emp_hip_context *context, const float *features, const float *targets, uint64_t n_rows,
+                      const emp_hip_mlp_options *options, float *params, float *loss_per_epoch);
+
+/* scores[i] = network output for row i. */
+int emp_hip_mlp_score(emp_hip_context *context, const float *params, const float *features, uint64_t n_rows,
+                      float *scores);
+
+/* Loss and gradient of every parameter over n_rows <= 65536 rows (one batch),
+ * for checking the backward kernels. */
+int emp_hip_mlp_gradients(emp_hip_context *context, const float *params, const float *features,
+                          const float *targets, uint64_t n_rows, float *grads, float *loss);
 
 #ifdef __cplusplus
 }
