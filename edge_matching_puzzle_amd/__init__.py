@@ -41,7 +41,8 @@ class EnumOptions(ctypes.Structure):
 class EnumResult(ctypes.Structure):
     _fields_ = [("n_nodes", ctypes.c_uint64), ("n_solutions", ctypes.c_uint64),
                 ("n_stored", ctypes.c_uint64), ("complete", ctypes.c_uint32),
-                ("status", ctypes.c_uint32), ("solve_ms", ctypes.c_double)]
+                ("status", ctypes.c_uint32), ("solve_ms", ctypes.c_double),
+                ("max_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class EnumStats(ctypes.Structure):
@@ -91,6 +92,8 @@ SIGNATURES = {
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
+    "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
+    "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
     "emp_hip_mlp_train": (_I32, [_P, _P, _P, _U64, _P, _P, _P]),
     "emp_hip_mlp_score": (_I32, [_P, _P, _P, _U64, _P]),
     "emp_hip_mlp_gradients": (_I32, [_P, _P, _P, _P, _U64, _P, _P]),
@@ -295,13 +298,45 @@ class Context:
                                     ctypes.byref(n), ctypes.byref(nodes), ctypes.byref(sols)))
         return boards[:min(n.value, capacity)].copy(), n.value, nodes.value, sols.value
 
+    def _options(self, order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
+                 target_items, node_limit, queue_items):
+        opt = EnumOptions()
+        lib().emp_hip_enum_default_options(ctypes.byref(opt))
+        opt.order = _ORDERS[order]
+        opt.waves_per_cu, opt.waves_per_block = waves_per_cu, waves_per_block
+        opt.steps_per_round, opt.step_budget = steps_per_round, step_budget
+        opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
+        opt.queue_items = queue_items
+        return opt
+
+    def probe(self, width, height, colours, rows, node_limit, order="spiral", **launch):
+        """One search per row of moves (strategy indices 0..L-1) with
+        node_limit placements each; returns (results, stats) with max_depth and
+        n_nodes below each row."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        rows = np.ascontiguousarray(np.asarray(rows, dtype=np.uint32))
+        n, L = rows.shape
+        opt = self._options(order, 0, launch.get("waves_per_cu", 0), launch.get("waves_per_block", 0),
+                            launch.get("steps_per_round", 0), launch.get("step_budget", 0),
+                            launch.get("target_items", 0), node_limit, launch.get("queue_items", 0))
+        res = (EnumResult * n)()
+        st = EnumStats()
+        _check(lib().emp_hip_probe(self._ctx, ctypes.byref(pz), _ptr(rows), L, n, ctypes.byref(opt),
+                                   ctypes.byref(res), ctypes.byref(st)))
+        out = [{"max_depth": r.max_depth, "n_nodes": r.n_nodes, "n_solutions": r.n_solutions,
+                "complete": bool(r.complete)} for r in res]
+        return out, {f: getattr(st, f) for f, _ in EnumStats._fields_}
+
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
-                  node_limit=0, queue_items=0):
+                  node_limit=0, queue_items=0, ranks=None):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
-        Returns (results, stats): one dict per puzzle with n_nodes, n_solutions,
-        complete, solve_ms and boards ([n_stored, P] uint16 row-major)."""
+        ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
+        order of the candidates at each strategy index.  Returns (results,
+        stats): one dict per puzzle with n_nodes, n_solutions, complete,
+        solve_ms, max_depth and boards ([n_stored, P] uint16 row-major)."""
         n = len(puzzles)
         keep = []
         arr = (Puzzle * n)()
@@ -314,22 +349,25 @@ class Context:
             b = np.zeros((max_solutions, w * h), dtype=np.uint16)
             boards.append(b)
             board_ptrs[i] = b.ctypes.data if max_solutions else None
-        opt = EnumOptions()
-        lib().emp_hip_enum_default_options(ctypes.byref(opt))
-        opt.order = _ORDERS[order]
-        opt.waves_per_cu, opt.waves_per_block = waves_per_cu, waves_per_block
-        opt.steps_per_round, opt.step_budget = steps_per_round, step_budget
-        opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
-        opt.queue_items = queue_items
+        opt = self._options(order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
+                            target_items, node_limit, queue_items)
         res = (EnumResult * n)()
         st = EnumStats()
-        _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
-                                       ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+        if ranks is None:
+            _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
+                                           ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+        else:
+            rk = [np.ascontiguousarray(r, dtype=np.uint16) for r in ranks]
+            if len(rk) != n:
+                raise ValueError("one rank table per puzzle")
+            rank_ptrs = (ctypes.c_void_p * n)(*[r.ctypes.data for r in rk])
+            _check(lib().emp_hip_enumerate_ranked(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt), rank_ptrs,
+                                                  ctypes.byref(res), board_ptrs, ctypes.byref(st)))
         out = []
         for i in range(n):
             r = res[i]
             out.append({"n_nodes": r.n_nodes, "n_solutions": r.n_solutions,
-                        "complete": bool(r.complete), "solve_ms": r.solve_ms,
+                        "complete": bool(r.complete), "solve_ms": r.solve_ms, "max_depth": r.max_depth,
                         "boards": boards[i][:r.n_stored].copy()})
         stats = {f: getattr(st, f) for f, _ in EnumStats._fields_}
         return out, stats

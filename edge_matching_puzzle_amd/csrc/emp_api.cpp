@@ -172,73 +172,129 @@ void emp_hip_enum_default_options(emp_hip_enum_options *options) {
     options->max_solutions = 1024;
 }
 
+namespace {
+
+// Shared body of emp_hip_enumerate, emp_hip_enumerate_ranked and
+// emp_hip_probe.  With probe rows, the single puzzle is searched from each
+// row's situation and results are per row.
+int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                    const emp_hip_enum_options *options, const uint16_t *const *ranks, const uint32_t *probe_moves,
+                    uint32_t probe_len, uint32_t probe_rows, emp_hip_enum_result *results, uint16_t *const *boards,
+                    emp_hip_enum_stats *stats) {
+    if (!context || !puzzles || !results || !n_puzzles) return fail(EMP_HIP_BAD_ARGUMENT, "null argument or empty batch");
+    emp_hip_enum_options opt;
+    emp_hip_enum_default_options(&opt);
+    if (options) opt = *options;
+    if (opt.order != EMP_HIP_ORDER_SPIRAL && opt.order != EMP_HIP_ORDER_BASIC)
+        return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+    if (opt.waves_per_block > 16) return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be at most 16");
+    if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+    std::vector<std::unique_ptr<emp::Puzzle>> owned;
+    std::vector<const emp::Puzzle *> list;
+    for (uint32_t i = 0; i < n_puzzles; ++i) {
+        try {
+            owned.emplace_back(new emp::Puzzle(puzzles[i].width, puzzles[i].height, puzzles[i].colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, "puzzle " + std::to_string(i) + ": " + e.what());
+        }
+        list.push_back(owned.back().get());
+    }
+    std::vector<std::vector<uint16_t>> rank_tables;
+    if (ranks) {
+        for (uint32_t i = 0; i < n_puzzles; ++i) {
+            if (!ranks[i]) return fail(EMP_HIP_BAD_ARGUMENT, "null rank table");
+            uint32_t tmax = 0;
+            for (int k = 0; k < 3; ++k) tmax = std::max<uint32_t>(tmax, 4 * (uint32_t)list[i]->pieces_of(k).size());
+            rank_tables.emplace_back(ranks[i], ranks[i] + (size_t)list[i]->size() * tmax);
+        }
+    }
+    std::vector<emp::EnumRoot> roots;
+    if (probe_moves) {
+        const emp::EnumTable t = emp::build_enum_table(*list[0], (int)opt.order);
+        roots.resize(probe_rows);
+        for (uint32_t r = 0; r < probe_rows; ++r) {
+            try {
+                emp::moves_to_stack(t, *list[0], probe_moves + (size_t)r * probe_len, probe_len, roots[r].stack);
+            } catch (const std::invalid_argument &e) {
+                return fail(EMP_HIP_BAD_ARGUMENT, "row " + std::to_string(r) + ": " + e.what());
+            }
+        }
+    }
+    emp::EnumConfig cfg;
+    cfg.order = (int)opt.order;
+    cfg.waves_per_cu = opt.waves_per_cu;
+    cfg.waves_per_block = opt.waves_per_block;
+    cfg.steps_per_round = opt.steps_per_round;
+    cfg.step_budget = opt.step_budget;
+    cfg.target_items = opt.target_items;
+    cfg.node_limit = opt.node_limit;
+    cfg.queue_items = opt.queue_items;
+    // Device-side store of up to max_solutions boards per search.
+    cfg.solution_capacity = opt.max_solutions;
+    std::vector<emp::EnumOutcome> out;
+    emp::EnumStats st;
+    emp::enumerate_on_device(context->dev, context->stream, context->workspace, list, cfg, out, st,
+                             probe_moves ? &roots : nullptr, ranks ? &rank_tables : nullptr);
+    for (uint32_t i = 0; i < (uint32_t)out.size(); ++i) {
+        emp_hip_enum_result &r = results[i];
+        const uint32_t P = list[probe_moves ? 0 : i]->size();
+        const uint64_t have = out[i].boards.size() / P;
+        r.n_nodes = out[i].nodes;
+        r.n_solutions = out[i].solutions;
+        r.complete = out[i].complete ? 1 : 0;
+        r.status = EMP_HIP_OK;
+        r.solve_ms = out[i].solve_ms;
+        r.max_depth = out[i].max_depth;
+        r.reserved = 0;
+        const uint64_t keep = std::min<uint64_t>(have, opt.max_solutions);
+        r.n_stored = (boards && boards[i]) ? keep : 0;
+        if (boards && boards[i] && keep) std::memcpy(boards[i], out[i].boards.data(), keep * P * sizeof(uint16_t));
+    }
+    if (stats) {
+        stats->total_ms = st.total_ms;
+        stats->dfs_kernel_ms = st.dfs_kernel_ms;
+        stats->expand_kernel_ms = st.expand_kernel_ms;
+        stats->dfs_launches = st.dfs_launches;
+        stats->expand_launches = st.expand_launches;
+        stats->items_processed = st.items_processed;
+        stats->splits = st.splits;
+        stats->candidate_sets = st.candidate_sets;
+        stats->max_words = st.max_words;
+        stats->lds_tables = st.lds_tables;
+        stats->blocks = st.blocks;
+        stats->threads = st.threads;
+    }
+    return EMP_HIP_OK;
+}
+
+}  // namespace
+
 int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
                       const emp_hip_enum_options *options, emp_hip_enum_result *results,
                       uint16_t *const *boards, emp_hip_enum_stats *stats) {
     return guarded([&] {
-        if (!context || !puzzles || !results || !n_puzzles)
-            return fail(EMP_HIP_BAD_ARGUMENT, "null argument or empty batch");
-        emp_hip_enum_options opt;
-        emp_hip_enum_default_options(&opt);
-        if (options) opt = *options;
-        if (opt.order != EMP_HIP_ORDER_SPIRAL && opt.order != EMP_HIP_ORDER_BASIC)
-            return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
-        if (opt.waves_per_block > 16)
-            return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be at most 16");
-        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
-        std::vector<std::unique_ptr<emp::Puzzle>> owned;
-        std::vector<const emp::Puzzle *> list;
-        for (uint32_t i = 0; i < n_puzzles; ++i) {
-            try {
-                owned.emplace_back(new emp::Puzzle(puzzles[i].width, puzzles[i].height, puzzles[i].colours));
-            } catch (const std::invalid_argument &e) {
-                return fail(EMP_HIP_BAD_PUZZLE, "puzzle " + std::to_string(i) + ": " + e.what());
-            }
-            list.push_back(owned.back().get());
-        }
-        emp::EnumConfig cfg;
-        cfg.order = (int)opt.order;
-        cfg.waves_per_cu = opt.waves_per_cu;
-        cfg.waves_per_block = opt.waves_per_block;
-        cfg.steps_per_round = opt.steps_per_round;
-        cfg.step_budget = opt.step_budget;
-        cfg.target_items = opt.target_items;
-        cfg.node_limit = opt.node_limit;
-        cfg.queue_items = opt.queue_items;
-        // Device-side store of up to max_solutions boards per puzzle.
-        cfg.solution_capacity = opt.max_solutions;
-        std::vector<emp::EnumOutcome> out;
-        emp::EnumStats st;
-        emp::enumerate_on_device(context->dev, context->stream, context->workspace, list, cfg, out, st);
-        for (uint32_t i = 0; i < n_puzzles; ++i) {
-            emp_hip_enum_result &r = results[i];
-            const uint32_t P = list[i]->size();
-            const uint64_t have = out[i].boards.size() / P;
-            r.n_nodes = out[i].nodes;
-            r.n_solutions = out[i].solutions;
-            r.complete = out[i].complete ? 1 : 0;
-            r.status = EMP_HIP_OK;
-            r.solve_ms = out[i].solve_ms;
-            const uint64_t keep = std::min<uint64_t>(have, opt.max_solutions);
-            r.n_stored = (boards && boards[i]) ? keep : 0;
-            if (boards && boards[i] && keep)
-                std::memcpy(boards[i], out[i].boards.data(), keep * P * sizeof(uint16_t));
-        }
-        if (stats) {
-            stats->total_ms = st.total_ms;
-            stats->dfs_kernel_ms = st.dfs_kernel_ms;
-            stats->expand_kernel_ms = st.expand_kernel_ms;
-            stats->dfs_launches = st.dfs_launches;
-            stats->expand_launches = st.expand_launches;
-            stats->items_processed = st.items_processed;
-            stats->splits = st.splits;
-            stats->candidate_sets = st.candidate_sets;
-            stats->max_words = st.max_words;
-            stats->lds_tables = st.lds_tables;
-            stats->blocks = st.blocks;
-            stats->threads = st.threads;
-        }
-        return EMP_HIP_OK;
+        return run_enumeration(context, puzzles, n_puzzles, options, nullptr, nullptr, 0, 0, results, boards, stats);
+    });
+}
+
+int emp_hip_enumerate_ranked(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                             const emp_hip_enum_options *options, const uint16_t *const *ranks,
+                             emp_hip_enum_result *results, uint16_t *const *boards, emp_hip_enum_stats *stats) {
+    return guarded([&] {
+        if (!ranks) return fail(EMP_HIP_BAD_ARGUMENT, "null rank tables");
+        return run_enumeration(context, puzzles, n_puzzles, options, ranks, nullptr, 0, 0, results, boards, stats);
+    });
+}
+
+int emp_hip_probe(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *moves, uint32_t row_len,
+                  uint32_t n_rows, const emp_hip_enum_options *options, emp_hip_enum_result *results,
+                  emp_hip_enum_stats *stats) {
+    return guarded([&] {
+        if (!moves || !n_rows || !row_len) return fail(EMP_HIP_BAD_ARGUMENT, "no rows");
+        if (n_rows > (1u << 20)) return fail(EMP_HIP_BAD_ARGUMENT, "at most 2^20 rows");
+        if (puzzle && row_len >= puzzle->width * puzzle->height)
+            return fail(EMP_HIP_BAD_ARGUMENT, "rows place between 1 and P - 1 pieces");
+        return run_enumeration(context, puzzle, 1, options, nullptr, moves, row_len, n_rows, results, nullptr, stats);
     });
 }
 

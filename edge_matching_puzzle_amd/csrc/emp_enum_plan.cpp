@@ -241,6 +241,56 @@ void stack_to_board(const EnumTable &t, const uint16_t *stack, uint16_t *board) 
     }
 }
 
+void moves_to_stack(const EnumTable &t, const Puzzle &pz, const uint32_t *moves, uint32_t len,
+                    std::vector<uint16_t> &stack) {
+    const EnumTableHeader h = t.header();
+    const uint64_t *b = t.blob.data();
+    const uint32_t R = h.row;
+    if (len > h.P) throw std::invalid_argument("more moves than positions");
+    std::vector<int> kind_of(pz.size(), -1), index_of(pz.size(), -1);
+    for (int k = 0; k < 3; ++k)
+        for (uint32_t ki = 0; ki < t.piece_of[k].size(); ++ki) {
+            kind_of[t.piece_of[k][ki]] = k;
+            index_of[t.piece_of[k][ki]] = (int)ki;
+        }
+    std::vector<uint64_t> avail[3];
+    for (int k = 0; k < 3; ++k) avail[k].assign(b + h.off_valid[k], b + h.off_valid[k] + R);
+    const uint8_t *colour = reinterpret_cast<const uint8_t *>(b + h.off_col);
+    stack.assign(len, 0);
+    for (uint32_t i = 0; i < len; ++i) {
+        const uint32_t pos = moves[i] & 0xFF, piece = (moves[i] >> 8) & 0xFF, o = (moves[i] >> 16) & 3;
+        const uint32_t x = t.order[i].first, y = t.order[i].second;
+        if (pos != y * h.W + x) throw std::invalid_argument("move " + std::to_string(i) + " is not at strategy index " + std::to_string(i));
+        const uint64_t *rec = b + h.off_nbr + (size_t)i * kNbrWords;
+        const int K = (int)(rec[0] >> 3) & 3;
+        if (piece >= pz.size() || kind_of[piece] != K) throw std::invalid_argument("move " + std::to_string(i) + ": piece of another kind");
+        const uint32_t tr = 4 * (uint32_t)index_of[piece] + o;
+        std::vector<uint64_t> c(R);
+        for (uint32_t w = 0; w < R; ++w) c[w] = b[h.off_base + (size_t)i * R + w] & avail[K][w];
+        for (uint32_t q = 0, n = (uint32_t)(rec[0] & 7); q < n; ++q) {
+            const uint64_t e = rec[1 + q];
+            const uint32_t j = (uint32_t)(e >> 55);
+            const uint32_t col = colour[((uint32_t)e >> 16) + stack[j]];
+            const uint64_t *row = b + ((e >> 32) & 0x7FFFFF) + (size_t)col * R;
+            for (uint32_t w = 0; w < R; ++w) c[w] &= row[w];
+        }
+        for (uint32_t d = 0; d < h.ndup[K]; ++d) {
+            const uint64_t *dm = b + h.off_dup[K] + (size_t)d * R;
+            bool seen = false;
+            for (uint32_t w = 0; w < R; ++w) {
+                const uint64_t m = c[w] & dm[w];
+                const uint64_t low = m ? (0xFull << (__builtin_ctzll(m) & 60)) : 0ull;
+                c[w] &= ~(m & ~(seen ? 0ull : low));
+                seen |= m != 0;
+            }
+        }
+        if (!((c[tr >> 6] >> (tr & 63)) & 1))
+            throw std::invalid_argument("move " + std::to_string(i) + " is not a candidate at its index");
+        stack[i] = (uint16_t)tr;
+        avail[K][tr >> 6] &= ~(0xFull << (tr & 60));
+    }
+}
+
 void spiral_prefixes(const Puzzle &pz, uint32_t length, uint32_t count, uint64_t seed,
                      const std::vector<uint32_t> &fixed, std::vector<uint32_t> &rows, uint32_t &produced) {
     if (pz.size() > 256) throw std::domain_error("prefix moves pack positions and pieces in 8 bits");

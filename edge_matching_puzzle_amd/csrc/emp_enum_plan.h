@@ -83,6 +83,12 @@ EnumTable build_enum_table(const Puzzle &puzzle, int order, uint32_t row_words =
 // Converts a stack of transitions (one per strategy index) to a row-major board.
 void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *board);
 
+// Transitions of the moves (pos | piece << 8 | o << 16) placed at strategy
+// indices 0..len-1, each checked to be a candidate of the enumeration at its
+// index after the earlier ones; throws std::invalid_argument otherwise.
+void moves_to_stack(const EnumTable &table, const Puzzle &puzzle, const uint32_t *moves, uint32_t len,
+                    std::vector<uint16_t> &stack);
+
 // Random spiral prefixes: up to `count` distinct rows, each the fixed moves
 // followed by a placement of strategy indices 0..length-1 in spiral order,
 // drawn by seeded random dives over matching pieces (pieces used by the fixed

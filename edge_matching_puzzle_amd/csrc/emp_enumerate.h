@@ -72,6 +72,7 @@ struct EnumConfig {
 struct EnumOutcome {
     uint64_t nodes = 0;
     uint64_t solutions = 0;
+    uint32_t max_depth = 0;          // most pieces placed in one situation
     bool complete = true;
     double solve_ms = 0;
     std::vector<uint16_t> boards;   // stored solutions, P cells each
@@ -84,9 +85,23 @@ struct EnumStats {
     uint64_t candidate_sets = 0;   // DFS iterations (one candidate set each)
 };
 
+// A search that starts from a given situation of puzzle `table`: the
+// transitions placed at strategy indices 0..stack.size()-1.
+struct EnumRoot {
+    uint32_t table = 0;
+    std::vector<uint16_t> stack;
+};
+
+// Enumerates every puzzle from its root (index 0 holds its first candidate),
+// or, with `roots`, each given situation (outcomes per root; their own
+// placements are not counted).  With `ranks` (one table per puzzle, P rows of
+// 4 x (pieces of the largest kind) u16), candidates are visited in rank order
+// instead of index order; the set of situations visited is the same.
 void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
-                         std::vector<EnumOutcome> &out, EnumStats &stats);
+                         std::vector<EnumOutcome> &out, EnumStats &stats,
+                         const std::vector<EnumRoot> *roots = nullptr,
+                         const std::vector<std::vector<uint16_t>> *ranks = nullptr);
 
 // Breadth expansion (glutton_wide's play_from applied level by level,
 // include/CUDA_glutton_situations.h:713-757): every situation the DFS visits

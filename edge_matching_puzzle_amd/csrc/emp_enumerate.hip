@@ -76,7 +76,56 @@ struct DfsArgs {
     uint16_t *sol_stack;           // [n_puzzles * sol_per_puzzle][stride]
     uint32_t sol_per_puzzle;
     unsigned long long *candidate_sets;   // candidate sets computed (DFS iterations)
+    unsigned int *max_depth;       // per puzzle: most pieces placed in one situation
+    // visit order (ranked kernels): rank of transition t at index k of puzzle p
+    // is ranks[rank_off[p] + k * rank_stride[p] + t]; lower ranks first
+    const uint16_t *ranks;
+    const uint64_t *rank_off;
+    const uint32_t *rank_stride;
 };
+
+// Plain order: transitions above `last` in index order.  Ranked order: the
+// transitions whose rank at this index is above the rank of `last`.
+template <int W>
+__device__ __forceinline__ void keep_above(uint64_t (&c)[W], int last) {
+    const int from = last + 1;
+    const int fw = from >> 6, fb = from & 63;
+#pragma unroll
+    for (int w = 0; w < W; ++w) c[w] &= w < fw ? 0ull : (w == fw ? ~0ull << fb : ~0ull);
+}
+
+template <int W>
+__device__ __forceinline__ void keep_above_ranked(uint64_t (&c)[W], int last, const uint16_t *row) {
+    if (last < 0) return;
+    const uint32_t rl = row[last];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t v = c[w];
+        for (uint64_t m = v; m; m &= m - 1) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(m);
+            if (row[w * 64 + b] <= rl) v &= ~(1ull << b);
+        }
+        c[w] = v;
+    }
+}
+
+// Lowest-ranked transition of c, or -1.
+template <int W>
+__device__ __forceinline__ int first_ranked(const uint64_t (&c)[W], const uint16_t *row) {
+    int best = -1;
+    uint32_t br = 0x10000u;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        for (uint64_t m = c[w]; m; m &= m - 1) {
+            const uint32_t t = w * 64 + (uint32_t)__builtin_ctzll(m);
+            const uint32_t r = row[t];
+            if (r < br) {
+                br = r;
+                best = (int)t;
+            }
+        }
+    return best;
+}
 
 // Item header: puzzle (bits 0-19) | floor (20-30) | level (31-41) |
 // last + 1 (42-53).  The item's stack row holds the transitions of indices
@@ -187,7 +236,8 @@ __device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
 // the kind's natural count matter: the base mask is zero above).
 template <int W, typename GetChoice, typename GetAvail>
 __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint64_t d, int last,
-                                           GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W]) {
+                                           GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W],
+                                           const uint16_t *rank_row = nullptr) {
     const int K = desc_kind(d);
     const uint32_t n = (uint32_t)d & 7;
     const uint64_t *base = tv.b + tv.off_base + (size_t)k * W;
@@ -229,16 +279,14 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
         }
     }
     // Only transitions above `last` (last = -1 keeps all).
-    const int from = last + 1;
-    const int fw = from >> 6, fb = from & 63;
-#pragma unroll
-    for (int w = 0; w < W; ++w) c[w] &= w < fw ? 0ull : (w == fw ? ~0ull << fb : ~0ull);
+    if (rank_row) keep_above_ranked<W>(c, last, rank_row);
+    else keep_above<W>(c, last);
 }
 
 // ---------------------------------------------------------------------------
 // Prefix expansion: one thread per item.
 // ---------------------------------------------------------------------------
-template <int W>
+template <int W, bool RANKED>
 __global__ void expand_kernel(DfsArgs a) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= a.n_in) return;
@@ -278,6 +326,7 @@ __global__ void expand_kernel(DfsArgs a) {
                 if (q == K && w == (int)(bb >> 6)) avail[q][w] &= ~(0xFull << (bb & 63));
     }
     uint64_t c[W];
+    const uint16_t *rank_row = RANKED ? a.ranks + a.rank_off[puzzle] + (size_t)kmin * a.rank_stride[puzzle] : nullptr;
     candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t, uint32_t j) { return (uint32_t)stack[j]; },
                   [&](int kk, int w) {
                       uint64_t v = 0;
@@ -288,12 +337,13 @@ __global__ void expand_kernel(DfsArgs a) {
                               if (q == kk && x == w) v = avail[q][x];
                       return v;
                   },
-                  c);
+                  c, rank_row);
     uint32_t n = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) n += (uint32_t)__popcll(c[w]);
     if (!n) return;
     atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
+    atomicMax(&a.max_depth[puzzle], kmin + 1);
     if (kmin == tv.P - 1) {
         uint64_t s = atomicAdd(&a.solutions[puzzle], (unsigned long long)n);
         for (int w = 0; w < W; ++w) {
@@ -350,7 +400,7 @@ __global__ void expand_kernel(DfsArgs a) {
 // so their blocks stay at 4 waves.
 constexpr uint32_t dfs_max_threads(int w) { return w <= 4 ? 1024u : 256u; }
 
-template <int W, bool LDS_TABLE>
+template <int W, bool LDS_TABLE, bool RANKED>
 __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     using STK = typename std::conditional<(W <= 4), uint8_t, uint16_t>::type;
     extern __shared__ uint64_t smem[];
@@ -378,8 +428,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     bool queue_empty = false;
     bool may_split = false;   // at most one failed reservation per item
     bool fresh = false;       // c not computed yet for the item's level
-    uint32_t puzzle = 0, k = 0, kmin = 0, P = 0;
+    uint32_t puzzle = 0, k = 0, kmin = 0, P = 0, depth = 0;
     int last = -1, curK = 0;
+    const uint16_t *rank_base = nullptr;   // ranked: this puzzle's rank rows
+    uint32_t rank_stride = 0;
     uint64_t steps = 0, nodes = 0;
     uint64_t c[W];
 #pragma unroll
@@ -396,6 +448,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     auto finish = [&]() {
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
         atomicAdd(a.candidate_sets, (unsigned long long)steps);
+        atomicMax(&a.max_depth[puzzle], depth);
         atomicMax(&a.last_tick[puzzle], (unsigned long long)wall_clock64());
         active = false;
     };
@@ -434,8 +487,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         k = hdr_level(hdr);
                         steps = 0;
                         nodes = 0;
+                        depth = k;
                         may_split = true;
                         fresh = true;
+                        if (RANKED) {
+                            rank_base = a.ranks + a.rank_off[puzzle];
+                            rank_stride = a.rank_stride[puzzle];
+                        }
                         const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
                         for (int K = 0; K < 3; ++K)
                             for (uint32_t w = 0, nw = tv.nat(K); w < nw; ++w)
@@ -470,7 +528,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             // Updates are written unconditionally where the value is inert
             // for the lanes that do not need them, so the body has no
             // divergent branch except the lane that finishes its item.
-            const int t = fresh ? -1 : first_bit<W>(c);
+            const int t = fresh ? -1
+                          : (RANKED ? first_ranked<W>(c, rank_base + (size_t)k * rank_stride) : first_bit<W>(c));
             const bool pop = !fresh && t < 0;
             if (pop && k == kmin) {
                 finish();
@@ -497,6 +556,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                 }
             }
             nodes += plc;
+            depth = plc && k + 1 > depth ? k + 1 : depth;
             ++steps;
             const uint32_t tk = pop ? km1 : (down ? k + 1 : k);
             const int tl = pop ? (int)held : (down ? -1 : (sol ? t : last));
@@ -515,7 +575,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                               const uint64_t v = A(kk, w);
                               return (kk == xK && (uint32_t)w == xw) ? (v & ~xm) : v;
                           },
-                          cc);
+                          cc, RANKED ? rank_base + (size_t)tk * rank_stride : nullptr);
             // A placement whose next level has no candidate is a dead end:
             // the lane stays at level k with the rest of c.
             const bool stay = down && !any_bit<W>(cc);
@@ -580,26 +640,31 @@ __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
 template <int W>
 void launch_expand(const DfsArgs &a, hipStream_t s) {
     const uint32_t threads = 256, blocks = (a.n_in + threads - 1) / threads;
-    if (blocks) hipLaunchKernelGGL(expand_kernel<W>, dim3(blocks), dim3(threads), 0, s, a);
+    if (!blocks) return;
+    if (a.ranks) hipLaunchKernelGGL((expand_kernel<W, true>), dim3(blocks), dim3(threads), 0, s, a);
+    else hipLaunchKernelGGL((expand_kernel<W, false>), dim3(blocks), dim3(threads), 0, s, a);
 }
 
 template <int W>
-const void *dfs_fn(bool lds) {
-    return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true>)
-               : reinterpret_cast<const void *>(&dfs_kernel<W, false>);
+const void *dfs_fn(bool lds, bool ranked) {
+    if (ranked)
+        return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, true>)
+                   : reinterpret_cast<const void *>(&dfs_kernel<W, false, true>);
+    return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, false>)
+               : reinterpret_cast<const void *>(&dfs_kernel<W, false, false>);
 }
 
 // VGPRs of the DFS kernel for this row width and table placement.
-uint32_t dfs_vgprs(int w, bool lds) {
+uint32_t dfs_vgprs(int w, bool lds, bool ranked) {
     const void *fn = nullptr;
     switch (w) {
-        case 1: fn = dfs_fn<1>(lds); break;
-        case 2: fn = dfs_fn<2>(lds); break;
-        case 3: fn = dfs_fn<3>(lds); break;
-        case 4: fn = dfs_fn<4>(lds); break;
-        case 6: fn = dfs_fn<6>(lds); break;
-        case 8: fn = dfs_fn<8>(lds); break;
-        default: fn = dfs_fn<16>(lds); break;
+        case 1: fn = dfs_fn<1>(lds, ranked); break;
+        case 2: fn = dfs_fn<2>(lds, ranked); break;
+        case 3: fn = dfs_fn<3>(lds, ranked); break;
+        case 4: fn = dfs_fn<4>(lds, ranked); break;
+        case 6: fn = dfs_fn<6>(lds, ranked); break;
+        case 8: fn = dfs_fn<8>(lds, ranked); break;
+        default: fn = dfs_fn<16>(lds, ranked); break;
     }
     hipFuncAttributes attr;
     if (hipFuncGetAttributes(&attr, fn) != hipSuccess || attr.numRegs <= 0) {
@@ -611,8 +676,13 @@ uint32_t dfs_vgprs(int w, bool lds) {
 
 template <int W>
 void launch_dfs(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes, hipStream_t s) {
-    if (lds) hipLaunchKernelGGL((dfs_kernel<W, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-    else hipLaunchKernelGGL((dfs_kernel<W, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    if (a.ranks) {
+        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+        else hipLaunchKernelGGL((dfs_kernel<W, false, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    } else {
+        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+        else hipLaunchKernelGGL((dfs_kernel<W, false, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    }
 }
 
 void dispatch_expand(int w, const DfsArgs &a, hipStream_t s) {
@@ -676,22 +746,28 @@ void *Workspace::raw(const std::string &name, size_t bytes) {
 
 void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
-                         std::vector<EnumOutcome> &out, EnumStats &stats) {
-    const uint32_t n = (uint32_t)puzzles.size();
+                         std::vector<EnumOutcome> &out, EnumStats &stats, const std::vector<EnumRoot> *roots,
+                         const std::vector<std::vector<uint16_t>> *ranks) {
+    const uint32_t n_tab = (uint32_t)puzzles.size();
+    // One search per root; by default each puzzle's root item (index 0 holds
+    // its first candidate).
+    const uint32_t n = roots ? (uint32_t)roots->size() : n_tab;
+    if (n == 0 || n > (1u << 20)) throw std::invalid_argument("between 1 and 2^20 searches per launch");
+    if (ranks && ranks->size() != n_tab) throw std::invalid_argument("one rank table per puzzle");
     out.assign(n, EnumOutcome{});
     stats = EnumStats{};
     auto wall0 = std::chrono::steady_clock::now();
 
     // Tables, every mask row padded to the launch's row width.
     uint32_t natural = 1;
-    for (uint32_t i = 0; i < n; ++i) natural = std::max(natural, natural_row_words(*puzzles[i]));
+    for (uint32_t i = 0; i < n_tab; ++i) natural = std::max(natural, natural_row_words(*puzzles[i]));
     if (natural > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
     const uint32_t row = round_words(natural);
     std::vector<EnumTable> tables;
-    tables.reserve(n);
-    std::vector<uint32_t> table_off(n);
+    tables.reserve(n_tab);
+    std::vector<uint32_t> table_off(n_tab);
     uint32_t total_words = 0, max_avail = 1, max_P = 0, max_table = 0;
-    for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t i = 0; i < n_tab; ++i) {
         tables.push_back(build_enum_table(*puzzles[i], cfg.order, row));
         EnumTableHeader h = tables.back().header();
         table_off[i] = total_words;
@@ -705,18 +781,29 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     stats.max_words = natural;
 
     std::vector<uint64_t> blob(total_words);
-    for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t i = 0; i < n_tab; ++i)
         std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
+    std::vector<uint32_t> root_table(n), root_off(n);
+    for (uint32_t r = 0; r < n; ++r) {
+        root_table[r] = roots ? (*roots)[r].table : r;
+        if (root_table[r] >= n_tab) throw std::invalid_argument("root names a puzzle out of range");
+        root_off[r] = table_off[root_table[r]];
+        if (roots) {
+            const size_t len = (*roots)[r].stack.size();
+            if (len < 1 || len >= tables[root_table[r]].header().P)
+                throw std::invalid_argument("a root places between 1 and P - 1 pieces");
+        }
+    }
 
     // Launch geometry: as many resident waves as registers (512 VGPRs per
     // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
     // wave's stacks) allow; the block size that fits the most waves wins.
-    const bool lds_table = n == 1;
+    const bool lds_table = n_tab == 1;
     const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
     const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
     const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
     const size_t lds_cu = dev.lds_per_cu ? dev.lds_per_cu : dev.lds_per_block;
-    const uint32_t vgprs = dfs_vgprs(maxw, lds_table);
+    const uint32_t vgprs = dfs_vgprs(maxw, lds_table, ranks != nullptr);
     const uint32_t reg_waves = 4 * std::min<uint32_t>(8, 512 / ((vgprs + 7) & ~7u));
     auto resident = [&](uint32_t w) -> uint32_t {
         const size_t bytes = table_bytes + w * per_wave_bytes;
@@ -748,7 +835,39 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     uint64_t *d_tables = ws.get<uint64_t>("tables", total_words);
     uint32_t *d_table_off = ws.get<uint32_t>("table_off", n);
     EMP_CHECK(hipMemcpyAsync(d_tables, blob.data(), blob.size() * 8, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_table_off, table_off.data(), n * 4, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_table_off, root_off.data(), n * 4, hipMemcpyHostToDevice, stream));
+    uint16_t *d_ranks = nullptr;
+    uint64_t *d_rank_off = nullptr;
+    uint32_t *d_rank_stride = nullptr;
+    if (ranks) {
+        // rank rows: P indices of 4 * (largest kind) transitions each
+        std::vector<uint64_t> off(n_tab), r_off(n);
+        std::vector<uint32_t> stride_t(n_tab), r_stride(n);
+        uint64_t total = 0;
+        for (uint32_t i = 0; i < n_tab; ++i) {
+            const Puzzle &pz = *puzzles[i];
+            uint32_t tmax = 0;
+            for (int k = 0; k < 3; ++k) tmax = std::max<uint32_t>(tmax, 4 * (uint32_t)pz.pieces_of(k).size());
+            if ((*ranks)[i].size() != (size_t)pz.size() * tmax)
+                throw std::invalid_argument("a rank table holds P x (4 x pieces of the largest kind) entries");
+            off[i] = total;
+            stride_t[i] = tmax;
+            total += (*ranks)[i].size();
+        }
+        std::vector<uint16_t> all(total);
+        for (uint32_t i = 0; i < n_tab; ++i) std::copy((*ranks)[i].begin(), (*ranks)[i].end(), all.begin() + off[i]);
+        for (uint32_t r = 0; r < n; ++r) {
+            r_off[r] = off[root_table[r]];
+            r_stride[r] = stride_t[root_table[r]];
+        }
+        d_ranks = ws.get<uint16_t>("ranks", total);
+        d_rank_off = ws.get<uint64_t>("rank_off", n);
+        d_rank_stride = ws.get<uint32_t>("rank_stride", n);
+        EMP_CHECK(hipMemcpyAsync(d_ranks, all.data(), total * 2, hipMemcpyHostToDevice, stream));
+        EMP_CHECK(hipMemcpyAsync(d_rank_off, r_off.data(), n * 8, hipMemcpyHostToDevice, stream));
+        EMP_CHECK(hipMemcpyAsync(d_rank_stride, r_stride.data(), n * 4, hipMemcpyHostToDevice, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+    }
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
     // A pass emits at most two items per input item; a full queue only makes
@@ -767,17 +886,25 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
     unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
     unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
+    unsigned int *d_depth = ws.get<unsigned int>("max_depth", n);
     const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
     uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
 
-    // Root items: index 0 holds its first candidate (one node each).
+    // Root items: index 0 holds its first candidate (one node each), or the
+    // given situations (their placements are not counted).
     std::vector<uint64_t> h_hdr(n);
     std::vector<uint16_t> h_stack((size_t)n * stride, 0);
-    std::vector<unsigned long long> h_nodes(n, 1), h_zero(n, 0), h_first(n, ~0ull), h_limit(n, cfg.node_limit);
+    std::vector<unsigned long long> h_nodes(n, roots ? 0 : 1), h_zero(n, 0), h_first(n, ~0ull),
+        h_limit(n, cfg.node_limit);
+    std::vector<unsigned int> h_depth(n, 1);
     for (uint32_t i = 0; i < n; ++i) {
-        h_hdr[i] = (uint64_t)i | (1ull << 20) | (1ull << 31);   // floor = level = 1, last = -1
-        h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
+        const uint32_t len = roots ? (uint32_t)(*roots)[i].stack.size() : 1;
+        h_hdr[i] = (uint64_t)i | ((uint64_t)len << 20) | ((uint64_t)len << 31);   // floor = level = len, last = -1
+        if (roots) std::copy((*roots)[i].stack.begin(), (*roots)[i].stack.end(), h_stack.begin() + (size_t)i * stride);
+        else h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
+        h_depth[i] = len;
     }
+    EMP_CHECK(hipMemcpyAsync(d_depth, h_depth.data(), n * 4, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_hdr[0], h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_stack[0], h_stack.data(), h_stack.size() * 2, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_nodes, h_nodes.data(), n * 8, hipMemcpyHostToDevice, stream));
@@ -810,6 +937,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.sol_stack = d_sol_stack;
     a.sol_per_puzzle = sol_per_puzzle;
     a.candidate_sets = d_csets;
+    a.max_depth = d_depth;
+    a.ranks = d_ranks;
+    a.rank_off = d_rank_off;
+    a.rank_stride = d_rank_stride;
     {
         const unsigned long long zero = 0;
         EMP_CHECK(hipMemcpyAsync(d_csets, &zero, 8, hipMemcpyHostToDevice, stream));
@@ -900,6 +1031,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     EMP_CHECK(hipMemcpyAsync(r_trunc.data(), d_trunc, n * 4, hipMemcpyDeviceToHost, stream));
     unsigned long long r_csets = 0;
     EMP_CHECK(hipMemcpyAsync(&r_csets, d_csets, 8, hipMemcpyDeviceToHost, stream));
+    std::vector<unsigned int> r_depth(n);
+    EMP_CHECK(hipMemcpyAsync(r_depth.data(), d_depth, n * 4, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipStreamSynchronize(stream));
     stats.candidate_sets = r_csets;
     std::vector<uint16_t> r_stack((size_t)n * sol_per_puzzle * stride);
@@ -913,15 +1046,17 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         o.nodes = r_nodes[i];
         o.solutions = r_sols[i];
         o.complete = !r_trunc[i];
+        o.max_depth = r_depth[i];
         o.solve_ms = (r_first[i] != ~0ull && r_last[i] >= r_first[i]) ? (double)(r_last[i] - r_first[i]) * tick_ms : 0.0;
     }
     for (uint32_t i = 0; i < n; ++i) {
         EnumOutcome &o = out[i];
-        const uint32_t P = tables[i].header().P;
+        const EnumTable &tb = tables[root_table[i]];
+        const uint32_t P = tb.header().P;
         const uint64_t kept = std::min<uint64_t>(o.solutions, sol_per_puzzle);
         o.boards.resize(kept * P);
         for (uint64_t s = 0; s < kept; ++s)
-            stack_to_board(tables[i], &r_stack[((size_t)i * sol_per_puzzle + s) * stride], &o.boards[s * P]);
+            stack_to_board(tb, &r_stack[((size_t)i * sol_per_puzzle + s) * stride], &o.boards[s * P]);
     }
     stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }
@@ -969,6 +1104,7 @@ void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, 
     a.flags = d_counters + 2;
     a.sol_stack = d_sol_stack;
     a.sol_per_puzzle = 0;
+    a.max_depth = ws.get<unsigned int>("x_depth", 1);
     int cur = 0;
     uint64_t items = 1;
     for (uint32_t level = 1; level < depth && items; ++level) {

@@ -1,0 +1,245 @@
+"""Learned candidate ordering.
+
+The search visits the candidates of a position in an order; the plain order
+is the reference's (kind index, then orientation: include/emp_strategy.h
+:332-415).  A small network scores each (situation, candidate) and the
+ranked enumeration (emp_hip_enumerate_ranked) visits the candidates of each
+strategy index by decreasing score.  Ordering never removes a candidate, so
+the situations visited, the node count and the solution set of a complete
+enumeration are unchanged; only how soon the deep situations are reached
+changes.
+
+Features of a candidate (piece p at orientation o, at strategy index k):
+  * its four edge colours as placed (north, east, south, west), one-hot over
+    colours 1..22 (the frame colour 0 sets no bit): 88 inputs;
+  * the colour histogram of the pieces still to place (edges of colour c over
+    all their edges): 22 inputs;
+  * the strategy index, sin / cos of pi 2^i k / P for i = 0..7: 16 inputs.
+Labels: the most pieces placed in one situation of the plain search below
+the candidate within a node budget (emp_hip_probe), less the pieces placed
+with it, over 64.
+"""
+import numpy as np
+
+import edge_matching_puzzle_amd as emp
+
+N_COLOURS = 22
+N_FEATURES = 4 * N_COLOURS + N_COLOURS + 16
+assert N_FEATURES == emp.MLP_INPUTS
+
+# Shapes of the learned-ordering workload.
+C3_SHAPE = dict(width=16, height=16, border_colours=5, interior_colours=17)
+EXACT_SHAPE = dict(width=8, height=8, border_colours=8, interior_colours=10)
+TRAIN_SEEDS = [4_000_000 + i for i in range(64)]
+HELDOUT_SEEDS = [4_100_000 + i for i in range(16)]
+EXACT_SEEDS = [5_000_000 + i for i in range(256)]
+
+
+def puzzle(seed, width, height, border_colours, interior_colours):
+    colours, _ = emp.generate(seed, width, height, border_colours, interior_colours)
+    return width, height, colours
+
+
+def piece_kinds(colours):
+    """Kind of each piece (number of frame sides) and the pieces of each kind
+    in id order (kind index = rank in that list)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    kind = (colours == 0).sum(1)
+    return kind, [np.flatnonzero(kind == k) for k in range(3)]
+
+
+def cell_kind(x, y, width, height):
+    return int(x == 0) + int(x == width - 1) + int(y == 0) + int(y == height - 1)
+
+
+def rotated(colours, piece, o):
+    """Colours shown north, east, south, west by `piece` at orientation o."""
+    c = np.asarray(colours).reshape(-1, 4)[piece]
+    return [int(c[(s + o) & 3]) for s in range(4)]
+
+
+def orientations_kept(colours, piece):
+    """Orientations the reference tries: one for a piece equal under a
+    quarter turn, 0 and 2 for one equal under a half turn, else all four
+    (include/emp_piece_db.h:1266-1272)."""
+    c = np.asarray(colours).reshape(-1, 4)[piece]
+    if c[0] == c[1] == c[2] == c[3]:
+        return (0,)
+    if c[0] == c[2] and c[1] == c[3]:
+        return (0, 2)
+    return (0, 1, 2, 3)
+
+
+def histogram(colours, remaining):
+    """Edge colour histogram of the pieces in `remaining` (bool mask)."""
+    c = np.asarray(colours).reshape(-1, 4)[np.asarray(remaining, dtype=bool)].reshape(-1)
+    h = np.zeros(N_COLOURS, dtype=np.float32)
+    c = c[(c >= 1) & (c <= N_COLOURS)]
+    np.add.at(h, c - 1, 1.0)
+    total = max(4 * int(np.count_nonzero(remaining)), 1)
+    return h / total
+
+
+def position_embedding(k, P):
+    f = np.pi * (2.0 ** np.arange(8)) * (k / P)
+    return np.concatenate([np.sin(f), np.cos(f)]).astype(np.float32)
+
+
+def features(colours, P, k, candidates, hist):
+    """Rows for candidates [(piece, o), ...] at strategy index k."""
+    out = np.zeros((len(candidates), N_FEATURES), dtype=np.float32)
+    pe = position_embedding(k, P)
+    for r, (p, o) in enumerate(candidates):
+        for s, col in enumerate(rotated(colours, p, o)):
+            if 1 <= col <= N_COLOURS:
+                out[r, s * N_COLOURS + col - 1] = 1.0
+        out[r, 4 * N_COLOURS:5 * N_COLOURS] = hist
+        out[r, 5 * N_COLOURS:] = pe
+    return out
+
+
+def candidates_at(width, height, colours, order_xy, moves):
+    """Candidates of the enumeration at strategy index len(moves) after the
+    packed moves, as (piece, o): pieces of the cell's kind not placed, kept
+    orientations matching the frame and the placed neighbours; of the
+    candidates that look alike (same four colours) only the lowest piece's
+    are kept (include/emp_position_strategy.h:189-202)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    kind, by_kind = piece_kinds(colours)
+    k = len(moves)
+    x, y = order_xy[k]
+    K = cell_kind(x, y, width, height)
+    board = {}
+    used = set()
+    for mv in moves:
+        pos, p, o = mv & 0xFF, (mv >> 8) & 0xFF, (mv >> 16) & 3
+        board[pos] = (p, o)
+        used.add(p)
+    need = []
+    for s, (dx, dy) in enumerate([(0, -1), (1, 0), (0, 1), (-1, 0)]):
+        nx, ny = x + dx, y + dy
+        if nx < 0 or ny < 0 or nx >= width or ny >= height:
+            need.append(0)
+        elif ny * width + nx in board:
+            q, qo = board[ny * width + nx]
+            need.append(rotated(colours, q, qo)[(s + 2) & 3])
+        else:
+            need.append(None)
+    out, seen = [], {}
+    for p in by_kind[K]:
+        if p in used:
+            continue
+        for o in orientations_kept(colours, p):
+            look = rotated(colours, p, o)
+            ok = all((need[s] is None and look[s] != 0) or (need[s] is not None and look[s] == need[s])
+                     for s in range(4))
+            if not ok:
+                continue
+            key = tuple(look)
+            if key in seen and seen[key] != p:
+                continue
+            seen.setdefault(key, p)
+            out.append((int(p), o))
+    return out
+
+
+def traces(ctx, width, height, colours, n_situations, seed, node_limit, depths=(4, 200)):
+    """(features, labels, depth reached) from depth probes of the plain search.
+
+    Situations are random spiral prefixes of random length in `depths`; each
+    of their candidates is probed with node_limit placements."""
+    P = width * height
+    order_xy = emp.order(width, height)
+    rng = np.random.default_rng(seed)
+    rows, feats, base = [], [], []
+    lengths = rng.integers(depths[0], min(depths[1], P - 2), n_situations)
+    for i, L in enumerate(lengths):
+        pre = emp.spiral_prefixes(width, height, colours, int(L), 1, int(rng.integers(1 << 62)))
+        if len(pre) == 0:
+            continue
+        moves = [int(m) for m in pre[0]]
+        cands = candidates_at(width, height, colours, order_xy, moves)
+        if not cands:
+            continue
+        remaining = np.ones(P, dtype=bool)
+        for mv in moves:
+            remaining[(mv >> 8) & 0xFF] = False
+        x, y = order_xy[L]
+        cell = y * width + x
+        for p, o in cands:
+            rows.append(moves + [emp.pack_move(cell, p, o)])
+            base.append(L + 1)
+        feats.append(features(colours, P, L, cands, histogram(colours, remaining)))
+    if not rows:
+        return np.zeros((0, N_FEATURES), np.float32), np.zeros(0, np.float32), np.zeros(0, np.int64)
+    X = np.concatenate(feats)
+    y = np.zeros(len(rows), dtype=np.float32)
+    depth = np.zeros(len(rows), dtype=np.int64)
+    # probes grouped by row length (one launch per length)
+    by_len = {}
+    for r, row in enumerate(rows):
+        by_len.setdefault(len(row), []).append(r)
+    for L, idx in by_len.items():
+        res, _ = ctx.probe(width, height, colours, np.array([rows[r] for r in idx], dtype=np.uint32), node_limit)
+        for r, out in zip(idx, res):
+            depth[r] = out["max_depth"]
+            y[r] = (out["max_depth"] - base[r]) / 64.0
+    return X, y, depth
+
+
+def init_params(seed):
+    """He-style initial parameters in the C ABI's layout."""
+    rng = np.random.default_rng(seed)
+    IN_PAD, HID = 128, 256
+    p = np.zeros(emp.MLP_PARAMS, dtype=np.float32)
+    w1 = rng.normal(0, np.sqrt(2.0 / N_FEATURES), (IN_PAD, HID)).astype(np.float32)
+    w1[N_FEATURES:] = 0
+    o = 0
+    p[o:o + IN_PAD * HID] = w1.reshape(-1); o += IN_PAD * HID
+    p[o:o + HID] = 0.01; o += HID
+    p[o:o + HID * HID] = rng.normal(0, np.sqrt(2.0 / HID), HID * HID); o += HID * HID
+    p[o:o + HID] = 0.01; o += HID
+    p[o:o + HID] = rng.normal(0, np.sqrt(1.0 / HID), HID); o += HID
+    return p
+
+
+def rank_table(ctx, params, width, height, colours, remaining=None):
+    """uint16 [P, T] visit ranks (T = 4 x pieces of the largest kind): at each
+    strategy index the transitions of the cell's kind by decreasing score
+    (ties by transition), then the unused entries."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    P = width * height
+    _, by_kind = piece_kinds(colours)
+    T = 4 * max(len(b) for b in by_kind)
+    order_xy = emp.order(width, height)
+    hist = histogram(colours, np.ones(P, bool) if remaining is None else remaining)
+    feats, spans = [], []
+    for k, (x, y) in enumerate(order_xy):
+        K = cell_kind(x, y, width, height)
+        cands = [(int(p), o) for p in by_kind[K] for o in range(4)]
+        spans.append((K, len(cands)))
+        feats.append(features(colours, P, k, cands, hist))
+    scores = ctx.mlp_score(params, np.concatenate(feats))
+    ranks = np.zeros((P, T), dtype=np.uint16)
+    at = 0
+    for k, (K, n) in enumerate(spans):
+        s = scores[at:at + n]
+        at += n
+        order = np.lexsort((np.arange(n), -s))      # by score, then transition
+        ranks[k, order] = np.arange(n, dtype=np.uint16)
+        ranks[k, n:] = np.arange(n, T, dtype=np.uint16)
+    return ranks
+
+
+def random_rank_table(width, height, colours, seed):
+    """A random visit order per strategy index (for exactness checks)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    _, by_kind = piece_kinds(colours)
+    T = 4 * max(len(b) for b in by_kind)
+    rng = np.random.default_rng(seed)
+    ranks = np.zeros((width * height, T), dtype=np.uint16)
+    for k, (x, y) in enumerate(emp.order(width, height)):
+        n = 4 * len(by_kind[cell_kind(x, y, width, height)])
+        ranks[k, :n] = rng.permutation(n)
+        ranks[k, n:] = np.arange(n, T)
+    return ranks

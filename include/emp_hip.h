@@ -121,6 +121,8 @@ typedef struct {
     uint32_t complete;         /* 1: whole tree enumerated                     */
     uint32_t status;           /* EMP_HIP_OK or an error for this puzzle       */
     double   solve_ms;         /* first work item start to last item end       */
+    uint32_t max_depth;        /* most pieces placed in one situation visited  */
+    uint32_t reserved;
 } emp_hip_enum_result;
 
 typedef struct {
@@ -232,6 +234,29 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle,
 int emp_hip_spiral_prefixes(const emp_hip_puzzle *puzzle, uint32_t length, uint32_t count,
                             uint64_t seed, const uint32_t *fixed_moves, uint32_t n_fixed,
                             uint32_t *out, uint32_t *n_out);
+
+/* emp_hip_enumerate with a visit order per puzzle: ranks[i] holds P rows
+ * (strategy index k) of T_i = 4 x (pieces of puzzle i's largest kind) u16;
+ * at index k the candidates are tried by increasing ranks[i][k * T_i + t]
+ * (t = 4 * kind_index + orientation), each row a permutation of 0..T_i-1.
+ * Index 0 still takes its first candidate in index order.  The situations
+ * visited, the node count and the solution set are those of the plain
+ * order; only the order of the visits changes.  No counterpart in the
+ * reference: it serves the learned candidate ordering. */
+int emp_hip_enumerate_ranked(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                             const emp_hip_enum_options *options, const uint16_t *const *ranks,
+                             emp_hip_enum_result *results, uint16_t *const *boards, emp_hip_enum_stats *stats);
+
+/* Depth probes: one search per row from the situation the row describes --
+ * moves (pos | piece << 8 | o << 16) for strategy indices 0..row_len-1, each
+ * a candidate of the enumeration after the earlier ones (else
+ * EMP_HIP_BAD_ARGUMENT) -- with options->node_limit placements each.
+ * results[r].max_depth is the most pieces placed in one situation below the
+ * row, n_nodes the placements below it.  No counterpart in the reference: it
+ * labels the traces the candidate scorer is trained on. */
+int emp_hip_probe(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *moves, uint32_t row_len,
+                  uint32_t n_rows, const emp_hip_enum_options *options, emp_hip_enum_result *results,
+                  emp_hip_enum_stats *stats);
 
 /* ------------------------------------------------------------------------
  * Candidate scorer (no counterpart in the reference: it only orders the

@@ -981,6 +981,24 @@ int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *co
     return ORACLE_OK;
 }
 
+int oracle_subtree_depth(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                         uint32_t prefix_len, const uint16_t *prefix, uint64_t *n_nodes, uint32_t *max_depth) {
+    Model m;
+    if (order != ORACLE_ORDER_SPIRAL && order != ORACLE_ORDER_BASIC) return ORACLE_BAD_ARGUMENT;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order) || prefix_len < 1 || prefix_len >= m.P || !prefix) return ORACLE_BAD_ARGUMENT;
+    Enumerator e(m, order);
+    if (!e.preset(prefix, prefix_len)) return ORACLE_BAD_ARGUMENT;
+    Enumerator::Result r;
+    e.run(r, 0, 0, false, prefix_len);
+    uint32_t depth = prefix_len;
+    for (uint32_t i = 0; i < m.P; ++i)
+        if (r.depth_nodes[i]) depth = std::max(depth, i + 1);
+    if (n_nodes) *n_nodes = r.nodes;
+    if (max_depth) *max_depth = depth;
+    return ORACLE_OK;
+}
+
 // results.bin v1 (include/emp_situation_binary_dumper.h:72-126, records
 // written by include/emp_strategy.h:258-266 / :367-375 and :410).
 int oracle_results_bin(uint32_t width, uint32_t height, const uint32_t *colours,

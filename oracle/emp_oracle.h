@@ -78,6 +78,13 @@ int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *co
                              uint16_t *boards, uint64_t *n_solutions, uint64_t *n_nodes);
 
 /*
+ * Same DFS below one prefix, reporting its placements and the most pieces
+ * placed in one situation (prefix_len when nothing fits below it).
+ */
+int oracle_subtree_depth(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                         uint32_t prefix_len, const uint16_t *prefix, uint64_t *n_nodes, uint32_t *max_depth);
+
+/*
  * results.bin v1 image of an enumeration (include/emp_situation_binary_dumper.h
  * :72-126): header, strategy positions, one record per solution, total nodes.
  * Writes at most capacity bytes into out and returns the full size in *size.

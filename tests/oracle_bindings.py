@@ -34,6 +34,8 @@ def lib():
         _lib.oracle_enumerate_prefixes.restype = i32
         _lib.oracle_enumerate_subtree.argtypes = [u32, u32, p, i32, u32, p, u64, p, p, p]
         _lib.oracle_enumerate_subtree.restype = i32
+        _lib.oracle_subtree_depth.argtypes = [u32, u32, p, i32, u32, p, p, p]
+        _lib.oracle_subtree_depth.restype = i32
         _lib.oracle_backtracker.argtypes = [u32, u32, p, i32, p, p, p, u64, p]
         _lib.oracle_backtracker.restype = i32
         _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
@@ -106,6 +108,17 @@ def subtree(w, h, colours, prefix, order=SPIRAL, max_solutions=1 << 12):
         raise ValueError("oracle_enumerate_subtree status %d" % st)
     n = int(sols[0])
     return {"n_nodes": int(nodes[0]), "n_solutions": n, "boards": boards[:min(n, max_solutions)].copy()}
+
+
+def subtree_depth(w, h, colours, prefix, order=SPIRAL):
+    """(placements below the prefix, most pieces placed in one situation)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    prefix = np.ascontiguousarray(prefix, dtype=np.uint16)
+    nodes, depth = np.zeros(1, np.uint64), np.zeros(1, np.uint32)
+    st = lib().oracle_subtree_depth(w, h, _ptr(colours), order, prefix.size, _ptr(prefix), _ptr(nodes), _ptr(depth))
+    if st != OK:
+        raise ValueError("oracle_subtree_depth status %d" % st)
+    return int(nodes[0]), int(depth[0])
 
 
 def results_bin(w, h, colours, order=SPIRAL):
