@@ -35,7 +35,8 @@ class EnumOptions(ctypes.Structure):
                 ("waves_per_block", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
                 ("step_budget", ctypes.c_uint64), ("target_items", ctypes.c_uint64),
                 ("node_limit", ctypes.c_uint64), ("max_solutions", ctypes.c_uint64),
-                ("use_graph", ctypes.c_uint32), ("queue_items", ctypes.c_uint32)]
+                ("use_graph", ctypes.c_uint32), ("queue_items", ctypes.c_uint32),
+                ("time_limit_ms", ctypes.c_double)]
 
 
 class EnumResult(ctypes.Structure):
@@ -330,7 +331,7 @@ class Context:
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
-                  node_limit=0, queue_items=0, ranks=None):
+                  node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
@@ -351,6 +352,7 @@ class Context:
             board_ptrs[i] = b.ctypes.data if max_solutions else None
         opt = self._options(order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
                             target_items, node_limit, queue_items)
+        opt.time_limit_ms = time_limit_ms
         res = (EnumResult * n)()
         st = EnumStats()
         if ranks is None:

@@ -67,6 +67,7 @@ struct EnumConfig {
     uint64_t solution_capacity = 1 << 16;
     uint32_t max_expansions = 0;
     uint32_t queue_items = 0;       // capacity of each work queue, 0 = sized from the grid
+    double time_limit_ms = 0;       // stop every search after this device time (0 = none)
 };
 
 struct EnumOutcome {

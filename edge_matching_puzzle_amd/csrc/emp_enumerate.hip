@@ -82,6 +82,8 @@ struct DfsArgs {
     const uint16_t *ranks;
     const uint64_t *rank_off;
     const uint32_t *rank_stride;
+    uint64_t deadline;             // wall_clock64 tick after which searches stop (0 = none)
+    unsigned int *timed_out;       // set when the deadline cut the search
 };
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
@@ -480,6 +482,11 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         atomicOr(&a.truncated[puzzle], 1u);
                         take = false;
                     }
+                    if (take && a.deadline && wall_clock64() > a.deadline) {
+                        atomicOr(&a.truncated[puzzle], 1u);
+                        atomicOr(a.timed_out, 1u);
+                        take = false;
+                    }
                     if (take) {
                         if (!LDS_TABLE) tv.load(a.tables + a.table_off[puzzle]);
                         P = tv.P;
@@ -517,6 +524,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
         }
         if (!active) continue;
 
+        if (a.deadline && wall_clock64() > a.deadline) {
+            // out of time: the rest of this item is not searched
+            atomicOr(&a.truncated[puzzle], 1u);
+            atomicOr(a.timed_out, 1u);
+            finish();
+            continue;
+        }
         const bool drain = __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         const uint64_t budget = drain ? a.drain_budget : a.step_budget;
         for (uint32_t r = 0; r < a.steps_per_round; ++r) {
@@ -630,6 +644,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
         if (err_ != hipSuccess)                                                           \
             throw DeviceError(std::string(#call) + ": " + hipGetErrorString(err_));       \
     } while (0)
+
+__global__ void read_clock(unsigned long long *out) { *out = wall_clock64(); }
 
 // Fill kernel used instead of hipMemsetAsync so passes can be graph-captured.
 __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
@@ -798,9 +814,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // Launch geometry: as many resident waves as registers (512 VGPRs per
     // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
     // wave's stacks) allow; the block size that fits the most waves wins.
-    const bool lds_table = n_tab == 1;
     const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
     const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
+    // One table goes to LDS when it leaves room for four waves per block.
+    const bool lds_table = n_tab == 1 && (size_t)max_table * 8 + 4 * per_wave_bytes <= dev.lds_per_block;
     const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
     const size_t lds_cu = dev.lds_per_cu ? dev.lds_per_cu : dev.lds_per_block;
     const uint32_t vgprs = dfs_vgprs(maxw, lds_table, ranks != nullptr);
@@ -814,8 +831,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     uint32_t wpb = cfg.waves_per_block;
     if (!wpb) {
         wpb = 4;
-        // multiples of 4 waves spread evenly over the CU's four SIMDs
-        for (uint32_t w : {4u, 8u, 12u, 16u})
+        // multiples of 4 waves spread evenly over the CU's four SIMDs; smaller
+        // blocks only when per-wave LDS allows no more
+        for (uint32_t w : {4u, 8u, 12u, 16u, 3u, 2u, 1u})
             if (w <= max_wpb && resident(w) > resident(wpb)) wpb = w;
     }
     if (wpb > max_wpb)
@@ -941,6 +959,21 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.ranks = d_ranks;
     a.rank_off = d_rank_off;
     a.rank_stride = d_rank_stride;
+    unsigned int *d_timed_out = ws.get<unsigned int>("timed_out", 1);
+    a.timed_out = d_timed_out;
+    {
+        const unsigned int zero = 0;
+        EMP_CHECK(hipMemcpyAsync(d_timed_out, &zero, 4, hipMemcpyHostToDevice, stream));
+    }
+    if (cfg.time_limit_ms) {
+        // the device clock counts 100 MHz ticks
+        unsigned long long *d_now = ws.get<unsigned long long>("clock_now", 1);
+        hipLaunchKernelGGL(read_clock, dim3(1), dim3(1), 0, stream, d_now);
+        unsigned long long now = 0;
+        EMP_CHECK(hipMemcpyAsync(&now, d_now, 8, hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+        a.deadline = now + (uint64_t)(cfg.time_limit_ms * 1e5);
+    }
     {
         const unsigned long long zero = 0;
         EMP_CHECK(hipMemcpyAsync(d_csets, &zero, 8, hipMemcpyHostToDevice, stream));

@@ -143,6 +143,46 @@ def candidates_at(width, height, colours, order_xy, moves):
     return out
 
 
+def canonical_moves(colours, moves):
+    """The same situation with identical pieces relabelled so that every move
+    uses the lowest piece still available among those showing the same
+    colours (the enumeration only tries that one,
+    include/emp_position_strategy.h:189-202).  Identical pieces are rotations
+    of each other, so swapping two of them everywhere leaves the board's
+    colours unchanged."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    kind, by_kind = piece_kinds(colours)
+    moves = [int(m) for m in moves]
+
+    def key(p):
+        c = [int(v) for v in colours[p]]
+        return min(tuple(c[r:] + c[:r]) for r in range(4))
+
+    used = set()
+    for i, mv in enumerate(moves):
+        pos, p, o = mv & 0xFF, (mv >> 8) & 0xFF, (mv >> 16) & 3
+        look = rotated(colours, p, o)
+        for q in by_kind[kind[p]]:
+            q = int(q)
+            if q in used or key(q) != key(p):
+                continue
+            if q != p:
+                # q shows the same look at some orientation; swap p and q in
+                # every later move
+                oq = next(r for r in orientations_kept(colours, q) if rotated(colours, q, r) == look)
+                for j in range(i + 1, len(moves)):
+                    pj, oj = (moves[j] >> 8) & 0xFF, (moves[j] >> 16) & 3
+                    if pj == q:
+                        lj = rotated(colours, q, oj)
+                        op = next(r for r in orientations_kept(colours, p) if rotated(colours, p, r) == lj)
+                        moves[j] = emp.pack_move(moves[j] & 0xFF, p, op)
+                moves[i] = emp.pack_move(pos, q, oq)
+                p = q
+            break
+        used.add(p)
+    return moves
+
+
 def traces(ctx, width, height, colours, n_situations, seed, node_limit, depths=(4, 200)):
     """(features, labels, depth reached) from depth probes of the plain search.
 
@@ -157,7 +197,7 @@ def traces(ctx, width, height, colours, n_situations, seed, node_limit, depths=(
         pre = emp.spiral_prefixes(width, height, colours, int(L), 1, int(rng.integers(1 << 62)))
         if len(pre) == 0:
             continue
-        moves = [int(m) for m in pre[0]]
+        moves = canonical_moves(colours, pre[0])
         cands = candidates_at(width, height, colours, order_xy, moves)
         if not cands:
             continue
@@ -243,3 +283,67 @@ def random_rank_table(width, height, colours, seed):
         ranks[k, :n] = rng.permutation(n)
         ranks[k, n:] = np.arange(n, T)
     return ranks
+
+
+def experiment(ctx, situations_per_seed=64, probe_nodes=20000, epochs=10, batch=256, lr=1e-3, time_ms=2000.0,
+               train_seeds=None, heldout_seeds=None, log=None):
+    """Train the scorer on depth-probe traces of the training seeds and
+    compare the max depth reached on the held-out seeds, for the same device
+    time, by the plain order, the learned order and a random order."""
+    import time
+    train_seeds = TRAIN_SEEDS if train_seeds is None else train_seeds
+    heldout_seeds = HELDOUT_SEEDS if heldout_seeds is None else heldout_seeds
+    say = log or (lambda m: None)
+    t0 = time.perf_counter()
+    Xs, ys = [], []
+    for seed in train_seeds:
+        w, h, c = puzzle(seed, **C3_SHAPE)
+        X, y, _ = traces(ctx, w, h, c, situations_per_seed, seed, probe_nodes, depths=(4, 180))
+        Xs.append(X)
+        ys.append(y)
+    X, y = np.concatenate(Xs), np.concatenate(ys)
+    trace_s = time.perf_counter() - t0
+    say("traces: %d rows from %d seeds in %.1f s" % (len(y), len(train_seeds), trace_s))
+    t0 = time.perf_counter()
+    params, losses = ctx.mlp_train(X, y, init_params(1), epochs=epochs, batch=batch, lr=lr, seed=1)
+    train_s = time.perf_counter() - t0
+    say("training: %d epochs in %.1f s, loss %.4f -> %.4f" % (epochs, train_s, losses[0], losses[-1]))
+    # held-out trace error against predicting the training mean
+    hx, hy = [], []
+    for seed in heldout_seeds[:4]:
+        w, h, c = puzzle(seed, **C3_SHAPE)
+        Xh, yh, _ = traces(ctx, w, h, c, 16, seed + 7, probe_nodes, depths=(4, 180))
+        hx.append(Xh)
+        hy.append(yh)
+    Xh, yh = np.concatenate(hx), np.concatenate(hy)
+    pred = ctx.mlp_score(params, Xh)
+    heldout_mse = float(np.mean((pred - yh) ** 2))
+    mean_mse = float(np.mean((y.mean() - yh) ** 2))
+    rows = []
+    for seed in heldout_seeds:
+        w, h, c = puzzle(seed, **C3_SHAPE)
+        t0 = time.perf_counter()
+        ranks = rank_table(ctx, params, w, h, c)
+        rank_s = time.perf_counter() - t0
+        out = {"seed": seed, "rank_table_s": rank_s}
+        for name, rk in (("plain", None), ("learned", [ranks]), ("random", [random_rank_table(w, h, c, seed)])):
+            (r,), st = ctx.enumerate([(w, h, c)], max_solutions=0, ranks=rk, time_limit_ms=time_ms)
+            out[name] = {"max_depth": r["max_depth"], "nodes": r["n_nodes"], "kernel_ms": st["dfs_kernel_ms"]}
+        rows.append(out)
+        say("seed %d: plain %d, learned %d, random %d" % (seed, out["plain"]["max_depth"],
+                                                          out["learned"]["max_depth"], out["random"]["max_depth"]))
+
+    def summary(name):
+        d = np.array([r[name]["max_depth"] for r in rows], dtype=float)
+        n = np.array([r[name]["nodes"] for r in rows], dtype=float)
+        ms = np.array([r[name]["kernel_ms"] for r in rows], dtype=float)
+        return {"mean_max_depth": float(d.mean()), "max_max_depth": int(d.max()),
+                "nodes_per_s": float(n.sum() / (ms.sum() * 1e-3))}
+
+    return {"trace_rows": int(len(y)), "trace_s": trace_s, "train_s": train_s, "loss_first": float(losses[0]),
+            "loss_last": float(losses[-1]), "heldout_mse": heldout_mse, "heldout_mse_of_mean": mean_mse,
+            "time_ms_per_run": time_ms, "plain": summary("plain"), "learned": summary("learned"),
+            "random": summary("random"),
+            "learned_deeper": int(sum(r["learned"]["max_depth"] > r["plain"]["max_depth"] for r in rows)),
+            "learned_shallower": int(sum(r["learned"]["max_depth"] < r["plain"]["max_depth"] for r in rows)),
+            "per_seed": rows, "params": params}

@@ -112,6 +112,8 @@ typedef struct {
     uint32_t use_graph;        /* capture each pass in a hipGraph               */
     uint32_t queue_items;      /* work queue capacity in items, 0 = sized from
                                   the grid (a full queue only stops splitting) */
+    double   time_limit_ms;    /* device time after which every search stops
+                                  (results then incomplete), 0 = none         */
 } emp_hip_enum_options;
 
 typedef struct {
