@@ -175,13 +175,52 @@ def _cpu_c3(w, h, c, row):
             "sample": "one prefix, one push (CPU oracle, %.1f s)" % dt}
 
 
+def hbm_traffic():
+    """HBM bytes per dfs_kernel launch of the C1 step, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the reading
+    of a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "dfs_kernel_c1_hbm.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def bench_c4(args, ctx):
+    """C4: learned candidate ordering on the 16x16 shape (5 border / 17
+    interior colours): train on depth-probe traces of 64 seeds, then compare
+    the max depth reached on 16 held-out seeds in equal device time."""
+    from edge_matching_puzzle_amd import learned
+    t0 = time.perf_counter()
+    res = learned.experiment(ctx, time_ms=args.c4_time_ms, log=lambda m: print(m, file=sys.stderr, flush=True))
+    elapsed = time.perf_counter() - t0
+    res.pop("params")
+    per = res.pop("per_seed")
+    return {
+        "metric": METRIC, "value": res["learned_relabel"]["mean_max_depth"],
+        "unit": "mean max depth (pieces placed) per %.0f ms search" % args.c4_time_ms, "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 scorer, u64 search", "data": "synthetic",
+        "config": {"workload": "C4 learned ordering: 16x16, 5 border / 17 interior colours, 64 training seeds, "
+                               "16 held-out seeds, %.0f ms device time per search" % args.c4_time_ms,
+                   "summary": res, "per_seed_max_depth": [{k: (v["max_depth"] if isinstance(v, dict) else v)
+                                                           for k, v in r.items() if k != "rank_table_s"} for r in per]},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1")
+    ap.add_argument("--c4-time-ms", type=float, default=2000.0, help="C4 device time per search")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, copy 1 GiB with a plain kernel (HBM counter calibration)")
     ap.add_argument("--instances", type=int, default=4096, help="C2 instances")
     ap.add_argument("--node-limit", type=int, default=200_000_000, help="C2 nodes per instance")
     ap.add_argument("--prefixes", type=int, default=4096, help="C3 prefixes")
@@ -204,7 +243,7 @@ def main():
         print("cpu baseline done", file=sys.stderr, flush=True)
     ctx = emp.Context(0)
     if args.workload != "c1":
-        line = bench_c2(args, ctx) if args.workload == "c2" else bench_c3(args, ctx)
+        line = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args, ctx)
         ctx.close()
         print(json.dumps(line), flush=True)
         return
@@ -272,13 +311,15 @@ def main():
                    "dfs_launches_per_step": st["dfs_launches"], "grid": [st["blocks"], st["threads"]],
                    "lds_tables": bool(st["lds_tables"]), "tuned": tuned or None},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+                     "frac": achieved / LDS_PEAK_GBS, "traffic": hbm_traffic(),
                      "bytes_per_node": bytes_node, "words_per_mask": M,
                      "kernel_ms": kernel_ms, "kernel_launches": launches,
                      "valu_int_ops_per_node": ops_node, "valu_achieved_gops": valu,
                      "valu_peak_gops": VALU_PEAK_GOPS, "valu_frac": valu / VALU_PEAK_GOPS},
         "cpu_baseline": cpu,
     }
+    if args.calibrate_copy:
+        print("copy kernel: 1 GiB in %.3f ms" % ctx.stream_copy(1 << 30), file=sys.stderr, flush=True)
     ctx.close()
     print(json.dumps(line), flush=True)
 

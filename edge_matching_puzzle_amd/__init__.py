@@ -95,6 +95,7 @@ SIGNATURES = {
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
+    "emp_hip_stream_copy": (_I32, [_P, _U64, _P]),
     "emp_hip_mlp_train": (_I32, [_P, _P, _P, _U64, _P, _P, _P]),
     "emp_hip_mlp_score": (_I32, [_P, _P, _P, _U64, _P]),
     "emp_hip_mlp_gradients": (_I32, [_P, _P, _P, _P, _U64, _P, _P]),
@@ -222,6 +223,13 @@ class Context:
         cu, lds, clk = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib().emp_hip_device_info(self._ctx, ctypes.byref(cu), ctypes.byref(lds), ctypes.byref(clk)))
         return {"compute_units": cu.value, "lds_per_block": lds.value, "clock_khz": clk.value}
+
+    def stream_copy(self, nbytes):
+        """Device copy of nbytes with a plain kernel (counter calibration);
+        returns the kernel time in ms."""
+        ms = ctypes.c_double()
+        _check(lib().emp_hip_stream_copy(self._ctx, nbytes, ctypes.byref(ms)))
+        return ms.value
 
     def mlp_train(self, features, targets, params, epochs=4, batch=1024, lr=1e-3, seed=1):
         """Adam on the scorer's MSE; returns (params, loss per epoch)."""

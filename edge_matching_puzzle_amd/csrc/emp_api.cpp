@@ -447,3 +447,37 @@ int emp_hip_mlp_gradients(emp_hip_context *context, const float *params, const f
         return EMP_HIP_OK;
     });
 }
+
+namespace {
+
+__global__ void stream_copy_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace
+
+int emp_hip_stream_copy(emp_hip_context *context, uint64_t bytes, double *ms) {
+    return guarded([&]() -> int {
+        if (!context || !bytes || bytes % 16) return fail(EMP_HIP_BAD_ARGUMENT, "bytes must be a positive multiple of 16");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        uint4 *src = context->workspace.get<uint4>("copy_src", bytes / 16);
+        uint4 *dst = context->workspace.get<uint4>("copy_dst", bytes / 16);
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail(EMP_HIP_DEVICE_ERROR, "hipEventCreate failed");
+        const uint32_t blocks = context->dev.compute_units * 8;
+        hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, context->stream, src, dst, bytes / 16);
+        (void)hipEventRecord(e0, context->stream);
+        hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, context->stream, src, dst, bytes / 16);
+        (void)hipEventRecord(e1, context->stream);
+        const hipError_t err = hipStreamSynchronize(context->stream);
+        float t = 0;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (err != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, hipGetErrorString(err));
+        if (ms) *ms = t;
+        return EMP_HIP_OK;
+    });
+}

@@ -271,6 +271,32 @@ def rank_table(ctx, params, width, height, colours, remaining=None):
     return ranks
 
 
+def relabelled(ctx, params, width, height, colours):
+    """The puzzle with its pieces renumbered by decreasing score, so that the
+    plain order (kind index = id order within a kind) visits the better
+    scored pieces first at full speed.  A piece's score is its best
+    orientation's score averaged over the strategy indices of its kind.
+    Returns (colours in the new numbering, old id of each new id)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    P = width * height
+    _, by_kind = piece_kinds(colours)
+    order_xy = emp.order(width, height)
+    hist = histogram(colours, np.ones(P, bool))
+    prio = np.zeros(P, dtype=np.float64)
+    for K in range(3):
+        ks = [k for k, (x, y) in enumerate(order_xy) if cell_kind(x, y, width, height) == K]
+        if not ks or len(by_kind[K]) == 0:
+            continue
+        cands = [(int(p), o) for p in by_kind[K] for o in range(4)]
+        feats = np.concatenate([features(colours, P, k, cands, hist) for k in ks])
+        sc = ctx.mlp_score(params, feats).reshape(len(ks), len(by_kind[K]), 4)
+        for i, p in enumerate(by_kind[K]):
+            kept = list(orientations_kept(colours, p))
+            prio[p] = sc[:, i, kept].max(axis=1).mean()
+    perm = np.lexsort((np.arange(P), -prio))
+    return colours[perm].copy(), perm
+
+
 def random_rank_table(width, height, colours, seed):
     """A random visit order per strategy index (for exactness checks)."""
     colours = np.asarray(colours).reshape(-1, 4)
@@ -326,12 +352,16 @@ def experiment(ctx, situations_per_seed=64, probe_nodes=20000, epochs=10, batch=
         ranks = rank_table(ctx, params, w, h, c)
         rank_s = time.perf_counter() - t0
         out = {"seed": seed, "rank_table_s": rank_s}
-        for name, rk in (("plain", None), ("learned", [ranks]), ("random", [random_rank_table(w, h, c, seed)])):
-            (r,), st = ctx.enumerate([(w, h, c)], max_solutions=0, ranks=rk, time_limit_ms=time_ms)
+        relab, _ = relabelled(ctx, params, w, h, c)
+        shuffled = np.asarray(c).reshape(-1, 4)[np.random.default_rng(seed).permutation(w * h)]
+        for name, cc, rk in (("plain", c, None), ("learned", c, [ranks]), ("random", c, [random_rank_table(w, h, c, seed)]),
+                             ("learned_relabel", relab, None), ("random_relabel", shuffled, None)):
+            (r,), st = ctx.enumerate([(w, h, cc)], max_solutions=0, ranks=rk, time_limit_ms=time_ms)
             out[name] = {"max_depth": r["max_depth"], "nodes": r["n_nodes"], "kernel_ms": st["dfs_kernel_ms"]}
         rows.append(out)
-        say("seed %d: plain %d, learned %d, random %d" % (seed, out["plain"]["max_depth"],
-                                                          out["learned"]["max_depth"], out["random"]["max_depth"]))
+        say("seed %d: plain %d, learned %d, random %d, learned_relabel %d, random_relabel %d" % (
+            seed, out["plain"]["max_depth"], out["learned"]["max_depth"], out["random"]["max_depth"],
+            out["learned_relabel"]["max_depth"], out["random_relabel"]["max_depth"]))
 
     def summary(name):
         d = np.array([r[name]["max_depth"] for r in rows], dtype=float)
@@ -343,7 +373,10 @@ def experiment(ctx, situations_per_seed=64, probe_nodes=20000, epochs=10, batch=
     return {"trace_rows": int(len(y)), "trace_s": trace_s, "train_s": train_s, "loss_first": float(losses[0]),
             "loss_last": float(losses[-1]), "heldout_mse": heldout_mse, "heldout_mse_of_mean": mean_mse,
             "time_ms_per_run": time_ms, "plain": summary("plain"), "learned": summary("learned"),
-            "random": summary("random"),
+            "random": summary("random"), "learned_relabel": summary("learned_relabel"),
+            "random_relabel": summary("random_relabel"),
+            "relabel_deeper": int(sum(r["learned_relabel"]["max_depth"] > r["plain"]["max_depth"] for r in rows)),
+            "relabel_shallower": int(sum(r["learned_relabel"]["max_depth"] < r["plain"]["max_depth"] for r in rows)),
             "learned_deeper": int(sum(r["learned"]["max_depth"] > r["plain"]["max_depth"] for r in rows)),
             "learned_shallower": int(sum(r["learned"]["max_depth"] < r["plain"]["max_depth"] for r in rows)),
             "per_seed": rows, "params": params}

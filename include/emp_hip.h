@@ -260,6 +260,13 @@ int emp_hip_probe(emp_hip_context *context, const emp_hip_puzzle *puzzle, const 
                   uint32_t n_rows, const emp_hip_enum_options *options, emp_hip_enum_result *results,
                   emp_hip_enum_stats *stats);
 
+/* Copies `bytes` (a multiple of 16) between two device buffers of the
+ * context's workspace with a plain 16-byte load / store kernel, twice, and
+ * returns the second copy's kernel time.  A measurement utility with no
+ * counterpart in the reference: HBM byte counters of a profiler read on this
+ * kernel give the scale of the same counters on the search kernels. */
+int emp_hip_stream_copy(emp_hip_context *context, uint64_t bytes, double *ms);
+
 /* ------------------------------------------------------------------------
  * Candidate scorer (no counterpart in the reference: it only orders the
  * candidates of the search and never removes one).  A 126-256-256-1
