@@ -339,7 +339,7 @@ class Context:
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
-                  node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0):
+                  node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0, use_graph=False):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
@@ -361,6 +361,7 @@ class Context:
         opt = self._options(order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
                             target_items, node_limit, queue_items)
         opt.time_limit_ms = time_limit_ms
+        opt.use_graph = 1 if use_graph else 0
         res = (EnumResult * n)()
         st = EnumStats()
         if ranks is None:

@@ -231,6 +231,7 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
     cfg.queue_items = opt.queue_items;
     if (!(opt.time_limit_ms >= 0)) return fail(EMP_HIP_BAD_ARGUMENT, "time_limit_ms must be >= 0");
     cfg.time_limit_ms = opt.time_limit_ms;
+    cfg.use_graph = opt.use_graph != 0;
     // Device-side store of up to max_solutions boards per search.
     cfg.solution_capacity = opt.max_solutions;
     std::vector<emp::EnumOutcome> out;

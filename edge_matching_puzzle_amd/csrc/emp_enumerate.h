@@ -68,6 +68,7 @@ struct EnumConfig {
     uint32_t max_expansions = 0;
     uint32_t queue_items = 0;       // capacity of each work queue, 0 = sized from the grid
     double time_limit_ms = 0;       // stop every search after this device time (0 = none)
+    bool use_graph = false;         // replay the DFS passes from two captured hipGraphs
 };
 
 struct EnumOutcome {

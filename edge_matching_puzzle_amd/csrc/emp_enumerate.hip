@@ -84,6 +84,7 @@ struct DfsArgs {
     const uint32_t *rank_stride;
     uint64_t deadline;             // wall_clock64 tick after which searches stop (0 = none)
     unsigned int *timed_out;       // set when the deadline cut the search
+    const unsigned int *n_in_dev;  // when set, the input item count is read here (graph replays)
 };
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
@@ -425,6 +426,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
         tv.load(smem);
         tv.make_uniform();
     }
+    const uint32_t n_in = a.n_in_dev ? *a.n_in_dev : a.n_in;
 
     bool active = false;
     bool queue_empty = false;
@@ -464,7 +466,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(a.in_head, n_idle);
             base = __shfl(base, leader);
-            if (base + n_idle >= a.n_in) {
+            if (base + n_idle >= n_in) {
                 queue_empty = true;
                 // Every item is taken: from now on busy lanes split early so
                 // the pass ends soon and the rest is shared out next pass.
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             if (!active) {
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
                 const uint32_t idx = base + rank;
-                if (idx < a.n_in) {
+                if (idx < n_in) {
                     const uint64_t hdr = a.in_hdr[idx];
                     kmin = hdr_floor(hdr);
                     puzzle = hdr_puzzle(hdr);
@@ -995,6 +997,44 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
 
     const char *env = std::getenv("EMP_HIP_VERBOSE");
     const bool verbose = env && *env && *env != '0';
+
+    // Graph replay of the DFS passes: one graph per queue direction holding
+    // the counter clear (a fill kernel), the DFS kernel and the copy of the
+    // counters to pinned host memory; the pass's input count is read from
+    // device memory, which the host sets before each replay.
+    struct Graphs {
+        hipGraph_t g[2] = {nullptr, nullptr};
+        hipGraphExec_t x[2] = {nullptr, nullptr};
+        unsigned int *pinned = nullptr;
+        ~Graphs() {
+            for (int i = 0; i < 2; ++i) {
+                if (x[i]) (void)hipGraphExecDestroy(x[i]);
+                if (g[i]) (void)hipGraphDestroy(g[i]);
+            }
+            if (pinned) (void)hipHostFree(pinned);
+        }
+    } graphs;
+    unsigned int *d_n_in = nullptr;
+    if (cfg.use_graph) {
+        d_n_in = ws.get<unsigned int>("n_in", 1);
+        EMP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&graphs.pinned), 8 * sizeof(unsigned int)));
+        for (int dir = 0; dir < 2; ++dir) {
+            DfsArgs ga = a;
+            ga.in_hdr = d_hdr[dir];
+            ga.in_stack = d_stack[dir];
+            ga.out_hdr = d_hdr[1 - dir];
+            ga.out_stack = d_stack[1 - dir];
+            ga.out_cap = cap;
+            ga.n_in = 0;
+            ga.n_in_dev = d_n_in;
+            EMP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
+            dispatch_dfs(maxw, ga, lds_table, blocks, threads, lds_bytes, stream);
+            (void)hipMemcpyAsync(graphs.pinned, d_counters, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
+            EMP_CHECK(hipStreamEndCapture(stream, &graphs.g[dir]));
+            EMP_CHECK(hipGraphInstantiate(&graphs.x[dir], graphs.g[dir], nullptr, nullptr, 0));
+        }
+    }
     int cur = 0;
     bool expansion_grows = true;
     uint64_t n_items = n;
@@ -1007,23 +1047,34 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         a.out_hdr = d_hdr[1 - cur];
         a.out_stack = d_stack[1 - cur];
         a.out_cap = cap;
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
         // Expand while the queue is short of the target and expansions still
         // multiply it (resumed searches pass through an expansion unchanged).
         const bool expand = n_items < target && expansions < max_expansions && expansion_grows;
+        if (!cfg.use_graph || expand)
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
         if (expand) {
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_counters_save, d_counters, 8 * 4, hipMemcpyDeviceToDevice, stream));
         }
-        EMP_CHECK(hipEventRecord(ev0, stream));
-        if (expand) dispatch_expand(maxw, a, stream);
-        else dispatch_dfs(maxw, a, lds_table, blocks, threads, lds_bytes, stream);
-        EMP_CHECK(hipGetLastError());
-        EMP_CHECK(hipEventRecord(ev1, stream));
         unsigned int counters[8];
-        EMP_CHECK(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
-        EMP_CHECK(hipStreamSynchronize(stream));
+        if (cfg.use_graph && !expand) {
+            const unsigned int n_in32 = (uint32_t)n_items;
+            EMP_CHECK(hipMemcpyAsync(d_n_in, &n_in32, 4, hipMemcpyHostToDevice, stream));
+            EMP_CHECK(hipEventRecord(ev0, stream));
+            EMP_CHECK(hipGraphLaunch(graphs.x[cur], stream));
+            EMP_CHECK(hipEventRecord(ev1, stream));
+            EMP_CHECK(hipStreamSynchronize(stream));
+            std::memcpy(counters, graphs.pinned, sizeof(counters));
+        } else {
+            EMP_CHECK(hipEventRecord(ev0, stream));
+            if (expand) dispatch_expand(maxw, a, stream);
+            else dispatch_dfs(maxw, a, lds_table, blocks, threads, lds_bytes, stream);
+            EMP_CHECK(hipGetLastError());
+            EMP_CHECK(hipEventRecord(ev1, stream));
+            EMP_CHECK(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
+            EMP_CHECK(hipStreamSynchronize(stream));
+        }
         float ms = 0;
         EMP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
         if (expand) {

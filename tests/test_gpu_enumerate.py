@@ -191,3 +191,27 @@ def test_c2_slice_with_node_limit(gpu_context):
         if ref["n_solutions"] <= 256:
             assert _sorted_boards(r["boards"]) == _sorted_boards(ref["boards"])
     assert complete >= 10
+
+
+def test_graph_replays_match_eager(gpu_context):
+    """DFS passes replayed from captured hipGraphs, with eager expansions and
+    host synchronisations between the replays, give the eager results; each
+    configuration runs twice in a row."""
+    rng = np.random.default_rng(9)
+    batch = [(w, w, emp.generate(2000 + s, w, w, 3, int(rng.integers(6, 10)))[0])
+             for s, w in enumerate([4, 5, 5, 6, 6, 5])]
+    w, h = 6, 6
+    c6, _ = emp.generate(1, w, h, 3, 6)
+    cases = [("batch", batch, {}), ("forced splits", [(w, h, c6)], dict(step_budget=64, target_items=1)),
+             ("single", [(w, h, c6)], {})]
+    for name, puzzles, kw in cases:
+        print("case: graph replay,", name, flush=True)
+        eager, st_e = gpu_context.enumerate(puzzles, max_solutions=4096, **kw)
+        for rep in range(2):
+            graph, st_g = gpu_context.enumerate(puzzles, max_solutions=4096, use_graph=True, **kw)
+            assert st_g["dfs_launches"] >= 1
+            for (pw, ph, pc), e, g in zip(puzzles, eager, graph):
+                assert (g["n_nodes"], g["n_solutions"], g["complete"]) == (e["n_nodes"], e["n_solutions"], True)
+                assert _sorted_boards(g["boards"]) == _sorted_boards(e["boards"])
+        for (pw, ph, pc), e in zip(puzzles, eager):
+            _check_same(e, o.enumerate_(pw, ph, pc, max_solutions=4096))
