@@ -65,7 +65,7 @@ struct GlArgs {
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
 struct GlLayout {
-    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, played, avail, scratch, total;
+    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, played, avail, scratch, lo, hi, total;
     __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw) {
         GlLayout l;
         uint32_t o = 0;
@@ -81,6 +81,8 @@ struct GlLayout {
         l.played = take(S * 4);
         l.avail = take(4 * 8);
         l.scratch = take(64);
+        l.lo = take(256 * 4);
+        l.hi = take(256 * 4);
         l.total = o;
         return l;
     }
@@ -88,6 +90,19 @@ struct GlLayout {
 
 __device__ __forceinline__ uint32_t nib(uint64_t v, uint32_t q) {
     return (uint32_t)__popcll((v >> (4 * (q & 15))) & 0xFull);
+}
+
+// Popcount of each nibble of v, in place (0..4 per nibble).
+__device__ __forceinline__ uint64_t nib_counts(uint64_t v) {
+    v = v - ((v >> 1) & 0x5555555555555555ull);
+    return (v & 0x3333333333333333ull) + ((v >> 2) & 0x3333333333333333ull);
+}
+// Nibble counts spread into bytes: even nibbles, odd nibbles.
+__device__ __forceinline__ uint64_t spread_even(uint64_t v) {
+    return v & 0x0F0F0F0F0F0F0F0Full;
+}
+__device__ __forceinline__ uint64_t spread_odd(uint64_t v) {
+    return (v >> 4) & 0x0F0F0F0F0F0F0F0Full;
 }
 
 struct Wave {
@@ -98,6 +113,7 @@ struct Wave {
     uint32_t *played;
     uint64_t *avail;
     uint32_t *scratch;
+    uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q), value | slot << 16
     uint64_t *stack;
     uint32_t *max_played;
     uint32_t lane;
@@ -117,6 +133,8 @@ struct Wave {
         played = reinterpret_cast<uint32_t *>(lds + L.played);
         avail = reinterpret_cast<uint64_t *>(lds + L.avail);
         scratch = reinterpret_cast<uint32_t *>(lds + L.scratch);
+        lo = reinterpret_cast<uint32_t *>(lds + L.lo);
+        hi = reinterpret_cast<uint32_t *>(lds + L.hi);
         stack = a.stacks + (size_t)wave_id * ((size_t)a.S * (a.S + 1) / 2) * a.nw;
         max_played = a.max_played + (size_t)wave_id * a.S;
         lane = threadIdx.x;
@@ -155,8 +173,10 @@ struct Wave {
     }
 
     // Per-slot popcount, single-piece slot marker and per-piece totals;
-    // returns false when a slot is empty.  Sets PC.
-    __device__ bool stats(uint32_t &PC) {
+    // returns false when a slot is empty.  Sets PC.  With `extremes`, also
+    // the per-piece min / max over free slots of pc_j - cnt_j(q) and the slot
+    // holding it (scoring reads these instead of scanning every slot).
+    __device__ bool stats(uint32_t &PC, bool extremes = false) {
         const uint32_t nn = n();
         uint32_t local = 0;
         bool empty = false;
@@ -201,6 +221,20 @@ struct Wave {
         PC = local;
         const bool any_empty = __ballot(empty) != 0;
         __syncthreads();
+        if (extremes) {
+            for (uint32_t q = lane; q < a.P; q += kWave) {
+                uint32_t mnv = 0xFFFFu, mns = 0, mxv = 0, mxs = 0;
+                const uint32_t w = q >> 4;
+                for (uint32_t j = 0; j < nn; ++j) {
+                    const uint32_t v = pc[j] - nib(M[j * a.nw + w], q);
+                    if (v < mnv) { mnv = v; mns = j; }
+                    if (v > mxv || j == 0) { mxv = v; mxs = j; }
+                }
+                lo[q] = mnv | mns << 16;
+                hi[q] = mxv | mxs << 16;
+            }
+            __syncthreads();
+        }
         return !any_empty;
     }
 
@@ -246,11 +280,24 @@ struct Wave {
         for (uint32_t l = 0; l < 4; ++l) own_single += nb[l] != 0xFFFFFFFFu && sp[nb[l]] == p;
         if (single[p] > own_single) return false;
         if (score) {
-            for (uint32_t j = 0; j < nn; ++j) {
-                if (j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3]) continue;
-                const uint32_t v = pc[j] - nib(M[j * a.nw + pw], p);
-                mn = v < mn ? v : mn;
-                mx = v > mx ? v : mx;
+            // Free slots other than i and the constrained neighbours: the
+            // per-piece extremes, unless they sit in one of those slots.
+            auto excluded = [&](uint32_t j) {
+                return j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3];
+            };
+            const uint32_t l = lo[p], h = hi[p];
+            if (!excluded(l >> 16) && !excluded(h >> 16)) {
+                if (nn > 1) {
+                    mn = (l & 0xFFFF) < mn ? (l & 0xFFFF) : mn;
+                    mx = (h & 0xFFFF) > mx ? (h & 0xFFFF) : mx;
+                }
+            } else {
+                for (uint32_t j = 0; j < nn; ++j) {
+                    if (excluded(j)) continue;
+                    const uint32_t v = pc[j] - nib(M[j * a.nw + pw], p);
+                    mn = v < mn ? v : mn;
+                    mx = v > mx ? v : mx;
+                }
             }
         }
         if (!score) {
@@ -270,22 +317,32 @@ struct Wave {
             }
             return true;
         }
-        // Option counts of the other available pieces.
+        // Option counts of the other available pieces: the options a piece
+        // loses are its orientations in slot i plus those the colour
+        // constraints remove at the neighbours, counted for the 16 pieces of
+        // a word at once (nibble popcounts spread into bytes).
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
         for (uint32_t w = 0; w < a.nw; ++w) {
-            const uint64_t mi = M[i * a.nw + w];
-            uint64_t mj[4], rj[4];
+            // per-piece sums reach 4 + 4 * 4 = 20, so even and odd nibbles are
+            // spread into bytes before adding; per byte gain >= lose
+            const uint64_t ci = nib_counts(M[i * a.nw + w]);
+            uint64_t ge = spread_even(ci), go = spread_odd(ci), le = 0, lo2 = 0;
             for (uint32_t l = 0; l < 4; ++l) {
-                if (nb[l] == 0xFFFFFFFFu) { mj[l] = 0; rj[l] = 0; continue; }
-                mj[l] = M[nb[l] * a.nw + w];
-                rj[l] = mj[l] & cmp[l][w];
+                if (nb[l] == 0xFFFFFFFFu) continue;
+                const uint64_t mj = M[nb[l] * a.nw + w];
+                const uint64_t cj = nib_counts(mj), rj = nib_counts(mj & cmp[l][w]);
+                ge += spread_even(cj);
+                go += spread_odd(cj);
+                le += spread_even(rj);
+                lo2 += spread_odd(rj);
             }
+            const uint64_t even = ge - le, odd = go - lo2;
+            const uint64_t av = avail[w >> 2] >> (16 * (w & 3));
             for (uint32_t b = 0; b < 16; ++b) {
                 const uint32_t q = w * 16 + b;
                 if (q >= a.P) break;
-                if (q == p || !piece_avail(q)) continue;
-                uint32_t d = nib(mi, q);
-                for (uint32_t l = 0; l < 4; ++l) d += nib(mj[l], q) - nib(rj[l], q);
+                if (q == p || !((av >> b) & 1)) continue;
+                const uint32_t d = (uint32_t)(((b & 1) ? odd : even) >> (8 * (b >> 1))) & 0xFF;
                 const uint32_t c = (uint32_t)T[q] - d;
                 if (!c) return false;
                 pmn = c < pmn ? c : pmn;
@@ -336,7 +393,7 @@ struct Wave {
     // get_best_candidate: returns (slot << 16) | bit, or 0xFFFFFFFF.
     __device__ uint32_t best() {
         uint32_t PC;
-        stats(PC);
+        stats(PC, true);
         const bool last_level = level == a.S - 1;
         uint64_t best_key = 0;
         uint32_t best_idx = 0xFFFFFFFFu;
