@@ -110,7 +110,7 @@ def bench_c2(args, ctx):
     nodes = 0
     kernel_ms = 0.0
     for _ in range(args.steps):
-        res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit)
+        res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True)
         nodes += sum(r["n_nodes"] for r in res)
         kernel_ms += st["dfs_kernel_ms"]
         print("step done: %d nodes, %.1f ms kernel" % (sum(r["n_nodes"] for r in res), st["dfs_kernel_ms"]),
@@ -125,7 +125,8 @@ def bench_c2(args, ctx):
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "C2 batched enumeration: %d puzzles, side 5..10, border colours 3..5, "
-                               "interior colours 4..12, one launch, node limit %d per instance" % (len(puzzles), limit),
+                               "interior colours 4..12, one batch, DFS passes replayed from hipGraphs, "
+                               "node limit %d per instance" % (len(puzzles), limit),
                    "completed": len(done), "truncated": len(res) - len(done),
                    "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99),
                    "nodes_per_step": sum(r["n_nodes"] for r in res)},

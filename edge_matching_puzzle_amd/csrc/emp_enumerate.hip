@@ -441,13 +441,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 #pragma unroll
     for (int w = 0; w < W; ++w) c[w] = 0;
 
-    // Available word w of kind K; words past the kind's own are read clamped
-    // (the base mask zeroes them).
-    auto A = [&](int K, int w) -> uint64_t & {
-        const uint32_t nw = tv.nat(K);
-        const uint32_t ww = (uint32_t)w < nw ? (uint32_t)w : nw - 1;
-        return avail[(tv.abase(K) + ww) * kWave + lane];
-    };
+    // Available word w of kind K.  Words past the kind's own read the next
+    // kind's words or the region's padding of W - 1 words; the base mask
+    // zeroes them.
+    auto A = [&](int K, int w) -> uint64_t & { return avail[(tv.abase(K) + (uint32_t)w) * kWave + lane]; };
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
@@ -817,7 +814,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
     // wave's stacks) allow; the block size that fits the most waves wins.
     const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
-    const size_t per_wave_bytes = (size_t)max_avail * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
+    // available words of the kinds plus W - 1 words of padding for reads
+    // past the last kind's own
+    const uint32_t avail_region = max_avail + (uint32_t)maxw - 1;
+    const size_t per_wave_bytes = (size_t)avail_region * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
     // One table goes to LDS when it leaves room for four waves per block.
     const bool lds_table = n_tab == 1 && (size_t)max_table * 8 + 4 * per_wave_bytes <= dev.lds_per_block;
     const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
@@ -940,7 +940,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.tables = d_tables;
     a.table_off = d_table_off;
     a.lds_table_words = lds_table ? max_table : 0;
-    a.avail_words = max_avail;
+    a.avail_words = avail_region;
     a.stride = stride;
     a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 32;
     a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 16);
