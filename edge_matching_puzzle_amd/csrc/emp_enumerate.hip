@@ -12,17 +12,18 @@
 //                  include/CUDA_glutton_situations.h:713-757).
 //   dfs_kernel     persistent: each lane of a wave64 owns one item and runs
 //                  the DFS below it.  Idle lanes are refilled from the item
-//                  queue with one atomic per wave (ballot / popcount / mbcnt).
+//                  queue with one atomic per wave (ballot / popcount / rank).
 //                  A lane that exceeds the step budget splits what remains of
-//                  its subtree into at most kmin..k residual items written to
-//                  the next pass's queue, so heavy subtrees are shared out.
+//                  its subtree into two items written to the next pass's
+//                  queue, so heavy subtrees are shared out.
 //
 // Per-lane state lives in LDS (lane-interleaved so a wave's accesses fall in
-// distinct banks): the stack of chosen transitions (u16 per index) and the
-// available-piece bitfield of each kind.  The candidate bitfield of a position
-// is recomputed on every visit from the colour masks of its placed neighbours
-// (held in LDS when one puzzle is searched), which keeps the per-lane stack at
-// 2 bytes per level instead of one bitfield per level.
+// distinct banks): the stack of placed transitions (one byte per index for
+// boards up to 10x10, two above) and the available transitions of each kind.
+// The candidates of the current level are kept in registers; a level's
+// candidates are recomputed from the colour masks of its placed neighbours
+// only when the search returns to it, which keeps the stack at one or two
+// bytes per level instead of one bitfield per level.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
