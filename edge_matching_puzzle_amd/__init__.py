@@ -318,7 +318,7 @@ class Context:
         opt.queue_items = queue_items
         return opt
 
-    def probe(self, width, height, colours, rows, node_limit, order="spiral", **launch):
+    def probe(self, width, height, colours, rows, node_limit, order="spiral", time_limit_ms=0.0, **launch):
         """One search per row of moves (strategy indices 0..L-1) with
         node_limit placements each; returns (results, stats) with max_depth and
         n_nodes below each row."""
@@ -329,6 +329,7 @@ class Context:
         opt = self._options(order, 0, launch.get("waves_per_cu", 0), launch.get("waves_per_block", 0),
                             launch.get("steps_per_round", 0), launch.get("step_budget", 0),
                             launch.get("target_items", 0), node_limit, launch.get("queue_items", 0))
+        opt.time_limit_ms = time_limit_ms
         res = (EnumResult * n)()
         st = EnumStats()
         _check(lib().emp_hip_probe(self._ctx, ctypes.byref(pz), _ptr(rows), L, n, ctypes.byref(opt),

@@ -380,3 +380,78 @@ def experiment(ctx, situations_per_seed=64, probe_nodes=20000, epochs=10, batch=
             "learned_deeper": int(sum(r["learned"]["max_depth"] > r["plain"]["max_depth"] for r in rows)),
             "learned_shallower": int(sum(r["learned"]["max_depth"] < r["plain"]["max_depth"] for r in rows)),
             "per_seed": rows, "params": params}
+
+
+def situation_rows(ctx, width, height, colours, depth, capacity):
+    """Every situation of the enumeration with `depth` pieces placed (in the
+    DFS's order), as packed moves [n, depth]; None when there are more than
+    `capacity`."""
+    try:
+        boards, n, _, _ = ctx.expand(width, height, colours, depth, capacity=capacity)
+    except emp.EmpError:
+        return None   # a level did not fit the device queues
+    if n > capacity:
+        return None
+    xy = emp.order(width, height)
+    cells = np.array([y * width + x for x, y in xy[:depth]], dtype=np.uint32)
+    b = boards[:, cells].astype(np.uint32)
+    return cells[None, :] | ((b >> 2) << 8) | ((b & 3) << 16)
+
+
+def row_features(colours, width, height, rows):
+    """Features of each row's last placement in its situation (vectorised
+    form of `features`)."""
+    colours = np.asarray(colours).reshape(-1, 4).astype(np.int64)
+    P = width * height
+    n, d = rows.shape
+    pieces = ((rows >> 8) & 0xFF).astype(np.int64)
+    orient = ((rows >> 16) & 3).astype(np.int64)
+    onehot = np.zeros((P, N_COLOURS), dtype=np.float32)
+    for p in range(P):
+        for col in colours[p]:
+            if 1 <= col <= N_COLOURS:
+                onehot[p, col - 1] += 1
+    remaining = onehot.sum(0)[None, :] - onehot[pieces[:, :-1]].sum(1)   # before the last placement
+    X = np.zeros((n, N_FEATURES), dtype=np.float32)
+    X[:, 4 * N_COLOURS:5 * N_COLOURS] = remaining / (4.0 * (P - (d - 1)))
+    X[:, 5 * N_COLOURS:] = position_embedding(d - 1, P)[None, :]
+    p, o = pieces[:, -1], orient[:, -1]
+    for s in range(4):
+        col = colours[p, (s + o) & 3]
+        ok = (col >= 1) & (col <= N_COLOURS)
+        X[np.flatnonzero(ok), s * N_COLOURS + col[ok] - 1] = 1.0
+    return X
+
+
+def item_order_experiment(ctx, params, seeds, depth=6, capacity=1 << 18, time_ms=2000.0, log=None):
+    """Max depth reached in `time_ms` of device time when the search starts
+    from the situations of one depth taken in DFS order (plain), by
+    decreasing score of their last placement (learned), or shuffled
+    (random)."""
+    say = log or (lambda m: None)
+    rows_out = []
+    for seed in seeds:
+        w, h, c = puzzle(seed, **C3_SHAPE)
+        d = depth
+        rows = situation_rows(ctx, w, h, c, d, capacity)
+        while rows is None and d > 2:
+            d -= 1
+            rows = situation_rows(ctx, w, h, c, d, capacity)
+        scores = ctx.mlp_score(params, row_features(c, w, h, rows))
+        orders = {"plain": np.arange(len(rows)), "learned": np.lexsort((np.arange(len(rows)), -scores)),
+                  "random": np.random.default_rng(seed).permutation(len(rows))}
+        out = {"seed": seed, "depth": d, "situations": int(len(rows))}
+        for name, perm in orders.items():
+            res, st = ctx.probe(w, h, c, rows[perm], node_limit=0, time_limit_ms=time_ms)
+            out[name] = {"max_depth": max(r["max_depth"] for r in res), "nodes": int(sum(r["n_nodes"] for r in res)),
+                         "kernel_ms": st["dfs_kernel_ms"]}
+        rows_out.append(out)
+        say("seed %d (%d situations at depth %d): plain %d, learned %d, random %d" % (
+            seed, len(rows), d, out["plain"]["max_depth"], out["learned"]["max_depth"], out["random"]["max_depth"]))
+    summ = {}
+    for name in ("plain", "learned", "random"):
+        dd = np.array([r[name]["max_depth"] for r in rows_out], dtype=float)
+        summ[name] = {"mean_max_depth": float(dd.mean()), "max_max_depth": int(dd.max())}
+    summ["learned_deeper"] = int(sum(r["learned"]["max_depth"] > r["plain"]["max_depth"] for r in rows_out))
+    summ["learned_shallower"] = int(sum(r["learned"]["max_depth"] < r["plain"]["max_depth"] for r in rows_out))
+    return summ, rows_out

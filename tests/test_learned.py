@@ -147,3 +147,29 @@ def test_learned_order_exact_on_256_8x8(gpu_context):
         assert r["complete"]
         assert (r["n_nodes"], r["n_solutions"]) == (g["n_nodes"], g["n_solutions"]), g["seed"]
         assert (_board_hash(r["boards"]) if r["n_solutions"] else "") == g["boards_sha256"], g["seed"]
+
+
+def test_row_features_match_per_candidate_features():
+    w, h, c = learned.puzzle(learned.TRAIN_SEEDS[0], **learned.C3_SHAPE)
+    pre = emp.spiral_prefixes(w, h, c, 7, 6, 3)
+    rows = np.array([learned.canonical_moves(c, r) for r in pre], dtype=np.uint32)
+    X = learned.row_features(c, w, h, rows)
+    for i, r in enumerate(rows):
+        remaining = np.ones(w * h, bool)
+        for mv in r[:-1]:
+            remaining[(int(mv) >> 8) & 0xFF] = False
+        p, o_ = (int(r[-1]) >> 8) & 0xFF, (int(r[-1]) >> 16) & 3
+        ref = learned.features(c, w * h, len(r) - 1, [(p, o_)], learned.histogram(c, remaining))[0]
+        assert np.allclose(X[i], ref)
+
+
+def test_canonical_moves_are_enumeration_candidates():
+    w, h, c = learned.puzzle(learned.TRAIN_SEEDS[1], **learned.C3_SHAPE)
+    xy = emp.order(w, h)
+    for s in range(6):
+        pre = emp.spiral_prefixes(w, h, c, 30, 1, s)
+        if len(pre) == 0:
+            continue
+        mv = learned.canonical_moves(c, pre[0])
+        for i in range(len(mv)):
+            assert ((mv[i] >> 8) & 0xFF, (mv[i] >> 16) & 3) in learned.candidates_at(w, h, c, xy, mv[:i])
