@@ -206,7 +206,9 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
             rec_out[1 + n] = stk_off | col_off << 16 | cm_block << 32 | (uint64_t)code << 55;
             ++n;
         }
-        rec_out[0] = (uint64_t)k << 3 | n;
+        const uint64_t abase = k == 0 ? 0 : (k == 1 ? h.words[0] : h.words[0] + h.words[1]);
+        const uint64_t uni = h.off_dup[k] + (uint64_t)h.ndup[k] * R;
+        rec_out[0] = uni << 32 | abase << 8 | (uint64_t)k << 3 | n;
     }
 
     // Root: first candidate of index 0, where no neighbour is placed yet.

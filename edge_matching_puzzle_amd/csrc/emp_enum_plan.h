@@ -32,7 +32,9 @@ constexpr uint32_t kUnconstrained = 0x1FF;
 
 // Neighbour record (kNbrWords u64 per strategy index), the form the kernels
 // use.  Word 0: number n of neighbours placed earlier (0..4) in bits 0-2, the
-// position's kind in bits 3-4.  Words 1..4, the i-th such neighbour (i < n):
+// position's kind in bits 3-4, the offset of the kind's words in a lane's
+// available-transition region in bits 8-23, and the u64 offset of the kind's
+// class-union row in bits 32-63.  Words 1..4, the i-th such neighbour (i < n):
 //   bits 0-15   byte offset of its level in a wave's LDS transition stack
 //               (index * 64 lanes * bytes per transition)
 //   bits 16-31  byte offset in the colour bytes of the row (neighbour kind,

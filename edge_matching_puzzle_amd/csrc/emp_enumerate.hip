@@ -211,6 +211,10 @@ struct TableView {
 static_assert(sizeof(EnumTableHeader) == 34 * 4, "TableView::load reads EnumTableHeader by field index");
 
 __device__ __forceinline__ int desc_kind(uint64_t d) { return (int)(d >> 3) & 3; }
+// offset of the position's kind in a lane's available-transition region
+__device__ __forceinline__ uint32_t desc_abase(uint64_t d) { return (uint32_t)(d >> 8) & 0xFFFF; }
+// u64 offset of the kind's class-union row in the table
+__device__ __forceinline__ uint32_t desc_union(uint64_t d) { return (uint32_t)(d >> 32); }
 
 template <int W>
 __device__ __forceinline__ int first_bit(const uint64_t (&c)[W]) {
@@ -236,8 +240,9 @@ __device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
 // above `last`.  The class reduction is applied to the full candidate set, as
 // the reference's identical-piece masks accumulate from the first candidate
 // on.  get_choice(stack_byte_offset, j): transition placed at index j;
-// get_avail(K, w): available transition word w of kind K (only words below
-// the kind's natural count matter: the base mask is zero above).
+// get_avail(d, w): available transition word w of the kind of the position
+// whose record word is d (only words below the kind's natural count matter:
+// the base mask is zero above).
 template <int W, typename GetChoice, typename GetAvail>
 __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint64_t d, int last,
                                            GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W],
@@ -246,7 +251,7 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
     const uint32_t n = (uint32_t)d & 7;
     const uint64_t *base = tv.b + tv.off_base + (size_t)k * W;
 #pragma unroll
-    for (int w = 0; w < W; ++w) c[w] = base[w] & get_avail(K, w);
+    for (int w = 0; w < W; ++w) c[w] = base[w] & get_avail(d, w);
     // Neighbour masks.  The loop runs to the wave's largest count; a lane
     // past its own count ANDs the all-ones row, so the body has no branch.
     const uint64_t *rec = tv.nbr_rec(k) + 1;
@@ -263,14 +268,13 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
     }
     // Appearance classes present among the candidates, lowest first; most
     // candidate sets meet none, so the loop usually does not run.
-    const uint64_t *dup = tv.b + tv.off_dup(K);
-    const uint64_t *uni = dup + tv.ndup(K) * W;
+    const uint64_t *uni = tv.b + desc_union(d);
     uint64_t rem[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) rem[w] = c[w] & uni[w];
     while (__ballot(any_bit<W>(rem)) != 0) {
         const int b = first_bit<W>(rem);
-        const uint64_t *dm = b >= 0 ? dup + tv.cls(K, (uint32_t)b) * W : tv.b + tv.off_zero;
+        const uint64_t *dm = b >= 0 ? tv.b + tv.off_dup(K) + tv.cls(K, (uint32_t)b) * W : tv.b + tv.off_zero;
         bool seen = false;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -332,7 +336,8 @@ __global__ void expand_kernel(DfsArgs a) {
     uint64_t c[W];
     const uint16_t *rank_row = RANKED ? a.ranks + a.rank_off[puzzle] + (size_t)kmin * a.rank_stride[puzzle] : nullptr;
     candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t, uint32_t j) { return (uint32_t)stack[j]; },
-                  [&](int kk, int w) {
+                  [&](uint64_t d, int w) {
+                      const int kk = desc_kind(d);
                       uint64_t v = 0;
 #pragma unroll
                       for (int q = 0; q < 3; ++q)
@@ -434,7 +439,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     bool may_split = false;   // at most one failed reservation per item
     bool fresh = false;       // c not computed yet for the item's level
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0, depth = 0;
-    int last = -1, curK = 0;
+    int last = -1;
+    uint32_t cur_ab = 0;   // available-region offset of level k's kind
     const uint16_t *rank_base = nullptr;   // ranked: this puzzle's rank rows
     uint32_t rank_stride = 0;
     uint64_t steps = 0, nodes = 0;
@@ -445,7 +451,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     // Available word w of kind K.  Words past the kind's own read the next
     // kind's words or the region's padding of W - 1 words; the base mask
     // zeroes them.
-    auto A = [&](int K, int w) -> uint64_t & { return avail[(tv.abase(K) + (uint32_t)w) * kWave + lane]; };
+    auto A = [&](uint32_t ab, uint32_t w) -> uint64_t & { return avail[(ab + w) * kWave + lane]; };
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
@@ -509,7 +515,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                             const uint32_t t = src[j];
                             S(j) = (STK)t;
                             const uint32_t bb = t & ~3u;
-                            A(desc_kind(tv.nbr(j)), (int)(bb >> 6)) &= ~(0xFull << (bb & 63));
+                            A(desc_abase(tv.nbr(j)), bb >> 6) &= ~(0xFull << (bb & 63));
                         }
                         atomicMin(&a.first_tick[puzzle], (unsigned long long)wall_clock64());
                         atomicAdd(&a.flags[2], 1u);
@@ -560,7 +566,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             // pop: level k - 1 gets its piece back (a no-op OR otherwise)
             {
                 const uint32_t bb = held & ~3u;
-                A(desc_kind(dprev), (int)(bb >> 6)) |= pop ? (0xFull << (bb & 63)) : 0ull;
+                A(desc_abase(dprev), bb >> 6) |= pop ? (0xFull << (bb & 63)) : 0ull;
             }
             if (sol) {
                 const uint64_t sidx = atomicAdd(&a.solutions[puzzle], 1ull);
@@ -576,7 +582,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const int tl = pop ? (int)held : (down ? -1 : (sol ? t : last));
             const uint32_t xw = down ? ((uint32_t)t >> 6) : 0u;
             const uint64_t xm = down ? (0xFull << ((uint32_t)t & 60)) : 0ull;
-            const int xK = curK;
+            const uint32_t xab = cur_ab;
             fresh = false;
             const uint64_t d = tv.nbr(tk);
             uint64_t cc[W];
@@ -585,9 +591,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                               return (uint32_t)*reinterpret_cast<const STK *>(
                                   reinterpret_cast<const uint8_t *>(stk + lane) + off);
                           },
-                          [&](int kk, int w) {
-                              const uint64_t v = A(kk, w);
-                              return (kk == xK && (uint32_t)w == xw) ? (v & ~xm) : v;
+                          [&](uint64_t dd, int w) {
+                              const uint32_t ab = desc_abase(dd);
+                              const uint64_t v = A(ab, (uint32_t)w);
+                              return (ab == xab && (uint32_t)w == xw) ? (v & ~xm) : v;
                           },
                           cc, RANKED ? rank_base + (size_t)tk * rank_stride : nullptr);
             // A placement whose next level has no candidate is a dead end:
@@ -598,10 +605,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             for (int w = 0; w < W; ++w)
                 c[w] = stay ? (c[w] & ~((w == (t >> 6)) ? tbit : 0ull)) : cc[w];
             // commit the take when descending (a no-op AND otherwise)
-            A(xK, (int)xw) &= (down && !stay) ? ~xm : ~0ull;
+            A(xab, xw) &= (down && !stay) ? ~xm : ~0ull;
             k = stay ? k : tk;
             last = stay ? t : tl;
-            curK = stay ? curK : desc_kind(d);
+            cur_ab = stay ? cur_ab : desc_abase(d);
             if (steps >= budget && may_split) {
                 // Split in two: (a) the candidates above the current choice at
                 // the floor index -- usually the bulk of what remains -- and
