@@ -159,6 +159,9 @@ def bench_c3(args, ctx):
                                "spiral moves, %d prefixes, %d pushes each" % (len(rows), args.push_budget),
                    "max_depth": max(depths), "prefixes_depth_ge_200": sum(1 for d in depths if d >= 200),
                    "mean_depth": sum(depths) / len(depths), "evaluations_per_s": st["evaluations"] / (st["kernel_ms"] * 1e-3),
+                   "evaluations_per_push": st["evaluations"] / max(st["pushes"], 1),
+                   "scoring_share": st["score_evaluations"] / max(st["evaluations"], 1),
+                   "removal_rounds_per_push": st["rounds"] / max(st["pushes"], 1),
                    "waves": st["waves"], "lds_bytes_per_wave": st["lds_bytes"]},
         "roofline": None,
         "cpu_baseline": None if args.no_cpu else _cpu_c3(w, h, c, rows[0]),

@@ -67,7 +67,8 @@ class GluttonResult(ctypes.Structure):
 class GluttonStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("pushes", ctypes.c_uint64), ("evaluations", ctypes.c_uint64),
-                ("waves", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32)]
+                ("waves", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
+                ("score_evaluations", ctypes.c_uint64), ("rounds", ctypes.c_uint64)]
 
 
 GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)

@@ -377,6 +377,8 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, 
             stats->kernel_ms = st.kernel_ms;
             stats->pushes = st.pushes;
             stats->evaluations = st.evaluations;
+            stats->score_evaluations = st.score_evaluations;
+            stats->rounds = st.rounds;
             stats->waves = st.waves;
             stats->lds_bytes = st.lds_bytes;
         }

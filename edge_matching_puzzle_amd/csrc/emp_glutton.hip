@@ -55,18 +55,19 @@ struct GlArgs {
     uint64_t push_budget;
     uint64_t *stacks;            // per wave: S(S+1)/2 * nw words
     uint32_t *max_played;        // per wave: S words
+    uint32_t *played;            // per wave: S words, the moves of the current stack
     unsigned int *next_item;
     uint32_t *out_max_depth;     // [n_items]
     uint64_t *out_pushes;        // [n_items]
     uint32_t *out_status;        // [n_items]
     uint16_t *out_board;         // [n_items][P]
-    unsigned long long *evals;   // candidates evaluated
+    unsigned long long *evals;   // [0] candidates evaluated, [1] of them scored, [2] removal rounds
 };
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
 struct GlLayout {
-    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, played, avail, scratch, lo, hi, total;
-    __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw) {
+    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, risk, pcol, avail, scratch, cnt, snb, lo, hi, total;
+    __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw, uint32_t max_colour) {
         GlLayout l;
         uint32_t o = 0;
         auto take = [&](uint32_t bytes) { uint32_t r = o; o += (bytes + 7) & ~7u; return r; };
@@ -78,9 +79,12 @@ struct GlLayout {
         l.T = take(256 * 2);
         l.single = take(256 * 4);
         l.crit = take(256 * 2);
-        l.played = take(S * 4);
+        l.risk = take(256 * 2);
+        l.pcol = take(P * 4);
         l.avail = take(4 * 8);
-        l.scratch = take(64);
+        l.scratch = take(128);
+        l.cnt = take(4 * (max_colour + 1) * 2);
+        l.snb = take(4 * 4);
         l.lo = take(256 * 4);
         l.hi = take(256 * 4);
         l.total = o;
@@ -92,36 +96,43 @@ __device__ __forceinline__ uint32_t nib(uint64_t v, uint32_t q) {
     return (uint32_t)__popcll((v >> (4 * (q & 15))) & 0xFull);
 }
 
-// Popcount of each nibble of v, in place (0..4 per nibble).
-__device__ __forceinline__ uint64_t nib_counts(uint64_t v) {
-    v = v - ((v >> 1) & 0x5555555555555555ull);
-    return (v & 0x3333333333333333ull) + ((v >> 2) & 0x3333333333333333ull);
+// Orientations o in the 4-bit set `ori` at which a piece with colours qcol
+// shows colour col on side (l + 2) & 3 -- the side facing a neighbour on side l.
+__device__ __forceinline__ uint32_t facing(uint32_t ori, uint32_t qcol, uint32_t l, uint32_t col) {
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t o = 0; o < 4; ++o)
+        n += ((ori >> o) & 1) && ((qcol >> (8 * ((l + 2 + o) & 3))) & 0xFF) == col;
+    return n;
 }
-// Nibble counts spread into bytes: even nibbles, odd nibbles.
-__device__ __forceinline__ uint64_t spread_even(uint64_t v) {
-    return v & 0x0F0F0F0F0F0F0F0Full;
-}
-__device__ __forceinline__ uint64_t spread_odd(uint64_t v) {
-    return (v >> 4) & 0x0F0F0F0F0F0F0F0Full;
-}
+
+__device__ __forceinline__ uint32_t nibble(uint64_t v, uint32_t q) { return (uint32_t)(v >> (4 * (q & 15))) & 0xF; }
 
 struct Wave {
     const GlArgs &a;
     uint64_t *M;
     uint16_t *pos_of_slot, *slot_of_pos, *pc, *sp, *T, *crit;
     uint32_t *single;   // per piece: free slots holding only that piece
-    uint32_t *played;
+    uint32_t *played;   // HBM
+    uint32_t *pcol;     // per piece N | E << 8 | S << 16 | W << 24
     uint64_t *avail;
     uint32_t *scratch;
     uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q), value | slot << 16
+    uint16_t *cnt;      // slot context: [side l][colour c] options of the side-l neighbour matching c
+    uint16_t *risk;     // slot context: critical pieces whose options all lie in the slot and its neighbours
+    uint32_t *snb;      // slot context: free neighbour slot on each side (0xFFFFFFFF: none)
     uint64_t *stack;
     uint32_t *max_played;
     uint32_t lane;
     uint32_t level, max, base;
+    // slot context for scoring: pieces of the min / max sets touched by the
+    // slot or its neighbours, and the extremes over the untouched ones
+    uint64_t lt[4], ht[4];
+    uint32_t min_u, max_u;
     int off[4];
 
     __device__ Wave(const GlArgs &args, unsigned char *lds, uint32_t wave_id) : a(args) {
-        GlLayout L = GlLayout::make(a.S, a.P, a.nw);
+        GlLayout L = GlLayout::make(a.S, a.P, a.nw, a.max_colour);
         M = reinterpret_cast<uint64_t *>(lds + L.masks);
         pos_of_slot = reinterpret_cast<uint16_t *>(lds + L.pos_of_slot);
         slot_of_pos = reinterpret_cast<uint16_t *>(lds + L.slot_of_pos);
@@ -130,10 +141,14 @@ struct Wave {
         T = reinterpret_cast<uint16_t *>(lds + L.T);
         single = reinterpret_cast<uint32_t *>(lds + L.single);
         crit = reinterpret_cast<uint16_t *>(lds + L.crit);
-        played = reinterpret_cast<uint32_t *>(lds + L.played);
+        pcol = reinterpret_cast<uint32_t *>(lds + L.pcol);
+        played = a.played + (size_t)wave_id * a.S;
         avail = reinterpret_cast<uint64_t *>(lds + L.avail);
         scratch = reinterpret_cast<uint32_t *>(lds + L.scratch);
         lo = reinterpret_cast<uint32_t *>(lds + L.lo);
+        cnt = reinterpret_cast<uint16_t *>(lds + L.cnt);
+        risk = reinterpret_cast<uint16_t *>(lds + L.risk);
+        snb = reinterpret_cast<uint32_t *>(lds + L.snb);
         hi = reinterpret_cast<uint32_t *>(lds + L.hi);
         stack = a.stacks + (size_t)wave_id * ((size_t)a.S * (a.S + 1) / 2) * a.nw;
         max_played = a.max_played + (size_t)wave_id * a.S;
@@ -145,9 +160,13 @@ struct Wave {
     }
 
     __device__ uint32_t n() const { return a.S - level; }
-    __device__ void count(uint64_t evaluated) {
+    // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
+    __device__ void count(uint64_t evaluated, bool scoring) {
         for (int o = 32; o > 0; o >>= 1) evaluated += __shfl_xor(evaluated, o);
-        if (lane == 0 && evaluated) atomicAdd(a.evals, (unsigned long long)evaluated);
+        if (lane == 0 && evaluated) {
+            atomicAdd(&a.evals[0], (unsigned long long)evaluated);
+            if (scoring) atomicAdd(&a.evals[1], (unsigned long long)evaluated);
+        }
     }
     __device__ bool is_free(uint32_t pos) const { return slot_of_pos[pos] < n(); }
     __device__ bool piece_avail(uint32_t q) const { return (avail[q >> 6] >> (q & 63)) & 1; }
@@ -161,7 +180,10 @@ struct Wave {
             pos_of_slot[j] = (uint16_t)j;
             played[j] = 0xFFFFFFFFu;
         }
-        for (uint32_t q = lane; q < a.P; q += kWave) slot_of_pos[q] = (uint16_t)q;
+        for (uint32_t q = lane; q < a.P; q += kWave) {
+            slot_of_pos[q] = (uint16_t)q;
+            pcol[q] = a.piece_col[q];
+        }
         if (lane < 4) {
             uint64_t v = 0;
             for (uint32_t b = 0; b < 64; ++b)
@@ -222,6 +244,44 @@ struct Wave {
         const bool any_empty = __ballot(empty) != 0;
         __syncthreads();
         if (extremes) {
+            // Pieces that can hold the min / max option count after a move:
+            // a move takes at most 4 + 4 * 4 = 20 options from a piece, so the
+            // min lies within 20 of the second smallest total and the max
+            // within 20 of the second largest (the second, so that the
+            // candidate's own piece can be left out).  Kept as 256-bit masks
+            // at scratch[16..23] (low) and scratch[24..31] (high).
+            uint32_t m1 = 0xFFFFu, m2 = 0xFFFFu, x1 = 0, x2 = 0;
+            auto lo2 = [](uint32_t &a1, uint32_t &a2, uint32_t v) {
+                if (v < a1) { a2 = a1; a1 = v; } else if (v < a2) a2 = v;
+            };
+            auto hi2 = [](uint32_t &a1, uint32_t &a2, uint32_t v) {
+                if (v > a1) { a2 = a1; a1 = v; } else if (v > a2) a2 = v;
+            };
+            for (uint32_t q = lane; q < a.P; q += kWave)
+                if (piece_avail(q)) {
+                    lo2(m1, m2, T[q]);
+                    hi2(x1, x2, T[q]);
+                }
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t om1 = __shfl_xor(m1, o), om2 = __shfl_xor(m2, o);
+                const uint32_t ox1 = __shfl_xor(x1, o), ox2 = __shfl_xor(x2, o);
+                lo2(m1, m2, om1);
+                lo2(m1, m2, om2);
+                hi2(x1, x2, ox1);
+                hi2(x1, x2, ox2);
+            }
+            const uint32_t lo_cut = m2 + 20, hi_cut = x2 > 20 ? x2 - 20 : 0;
+            uint64_t *sets = reinterpret_cast<uint64_t *>(scratch + 16);
+            for (uint32_t base = 0; base < 256; base += kWave) {
+                const uint32_t q = base + lane;
+                const bool av = q < a.P && piece_avail(q);
+                const uint64_t in_lo = __ballot(av && T[q] <= lo_cut);
+                const uint64_t in_hi = __ballot(av && T[q] >= hi_cut);
+                if (lane == 0) {
+                    sets[base / 64] = in_lo;
+                    sets[4 + base / 64] = in_hi;
+                }
+            }
             for (uint32_t q = lane; q < a.P; q += kWave) {
                 uint32_t mnv = 0xFFFFu, mns = 0, mxv = 0, mxs = 0;
                 const uint32_t w = q >> 4;
@@ -244,30 +304,29 @@ struct Wave {
                              uint32_t &total, uint32_t &minmax) {
         const uint32_t nn = n();
         const uint32_t pos = pos_of_slot[i];
-        const uint32_t pcol = a.piece_col[p];
+        const uint32_t pc_p = pcol[p];
         const uint32_t pw = p >> 4;
         const uint64_t keep_p = ~(0xFull << (4 * (p & 15)));
         uint32_t nb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-        const uint64_t *cmp[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint32_t cs[4] = {0, 0, 0, 0};   // colour the candidate shows toward each constrained neighbour
         uint32_t rsum = 0, nb_pc_minus = 0;
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
-        // Colour-constrained free neighbours (apply_color_constraints :528-612).
+        // Colour-constrained free neighbours (apply_color_constraints :528-612):
+        // the options of the neighbour matching the candidate's side, from
+        // the slot context, less those of piece p itself.  (A coloured side
+        // never faces the frame, so the neighbour lookup needs no edge test.)
+        (void)pos;
+        (void)keep_p;
         for (uint32_t l = 0; l < 4; ++l) {
-            const uint32_t col = (pcol >> (8 * ((l + o) & 3))) & 0xFF;
+            const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
             if (!col) continue;
-            const uint32_t q = (uint32_t)((int)pos + off[l]);
-            if (q >= a.P || !is_free(q)) continue;   // a coloured side never faces the frame
-            const uint32_t j = slot_of_pos[q];
-            const uint64_t *c = a.cm + ((size_t)(col - 1) * 4 + l) * a.nw;
-            uint32_t rp = 0;
-            for (uint32_t w = 0; w < a.nw; ++w) {
-                uint64_t r = M[j * a.nw + w] & c[w];
-                if (w == pw) r &= keep_p;
-                rp += (uint32_t)__popcll(r);
-            }
+            const uint32_t j = snb[l];
+            if (j == 0xFFFFFFFFu) continue;
+            const uint32_t rp =
+                cnt[l * (a.max_colour + 1) + col] - facing(nibble(M[j * a.nw + pw], p), pc_p, l, col);
             if (!rp) return false;
             nb[l] = j;
-            cmp[l] = c;
+            cs[l] = col;
             rsum += rp;
             nb_pc_minus += pc[j] - nib(M[j * a.nw + pw], p);
             if (score) {
@@ -301,54 +360,60 @@ struct Wave {
             }
         }
         if (!score) {
-            // Only pieces with few options can be left without one.
-            const uint32_t nc = scratch[0];
+            // Only the critical pieces at risk in this slot can be left
+            // without option (slot context).
+            const uint32_t nc = scratch[1];
             for (uint32_t k = 0; k < nc; ++k) {
-                const uint32_t q = crit[k];
+                const uint32_t q = risk[k];
                 if (q == p) continue;
                 const uint32_t w = q >> 4;
                 uint32_t d = nib(M[i * a.nw + w], q);
+                const uint32_t qc = pcol[q];
                 for (uint32_t l = 0; l < 4; ++l) {
                     if (nb[l] == 0xFFFFFFFFu) continue;
-                    const uint64_t mj = M[nb[l] * a.nw + w];
-                    d += nib(mj, q) - nib(mj & cmp[l][w], q);
+                    const uint32_t ori = nibble(M[nb[l] * a.nw + w], q);
+                    d += (uint32_t)__popc(ori) - facing(ori, qc, l, cs[l]);
                 }
                 if ((uint32_t)T[q] == d) return false;
             }
             return true;
         }
-        // Option counts of the other available pieces: the options a piece
-        // loses are its orientations in slot i plus those the colour
-        // constraints remove at the neighbours, counted for the 16 pieces of
-        // a word at once (nibble popcounts spread into bytes).
-        uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
-        for (uint32_t w = 0; w < a.nw; ++w) {
-            // per-piece sums reach 4 + 4 * 4 = 20, so even and odd nibbles are
-            // spread into bytes before adding; per byte gain >= lose
-            const uint64_t ci = nib_counts(M[i * a.nw + w]);
-            uint64_t ge = spread_even(ci), go = spread_odd(ci), le = 0, lo2 = 0;
+        // Option counts of the other available pieces after the move: T(q)
+        // less its orientations in slot i and those the colour constraints
+        // remove at the neighbours.  Only the pieces that can hold the min
+        // (which include every piece that could drop to zero) and the max
+        // are visited (sets built by stats()).
+        // Pieces untouched by the slot and its neighbours keep their totals:
+        // their extremes come from the slot context (a zero total there
+        // means no candidate of the slot is valid).
+        if (min_u == 0) return false;
+        uint32_t pmn = min_u, pmx = max_u;
+        auto lost = [&](uint32_t q) {
+            const uint32_t w = q >> 4;
+            uint32_t d = nib(M[i * a.nw + w], q);
+            const uint32_t qc = pcol[q];
             for (uint32_t l = 0; l < 4; ++l) {
                 if (nb[l] == 0xFFFFFFFFu) continue;
-                const uint64_t mj = M[nb[l] * a.nw + w];
-                const uint64_t cj = nib_counts(mj), rj = nib_counts(mj & cmp[l][w]);
-                ge += spread_even(cj);
-                go += spread_odd(cj);
-                le += spread_even(rj);
-                lo2 += spread_odd(rj);
+                const uint32_t ori = nibble(M[nb[l] * a.nw + w], q);
+                d += (uint32_t)__popc(ori) - facing(ori, qc, l, cs[l]);
             }
-            const uint64_t even = ge - le, odd = go - lo2;
-            const uint64_t av = avail[w >> 2] >> (16 * (w & 3));
-            for (uint32_t b = 0; b < 16; ++b) {
-                const uint32_t q = w * 16 + b;
-                if (q >= a.P) break;
-                if (q == p || !((av >> b) & 1)) continue;
-                const uint32_t d = (uint32_t)(((b & 1) ? odd : even) >> (8 * (b >> 1))) & 0xFF;
-                const uint32_t c = (uint32_t)T[q] - d;
+            return d;
+        };
+        for (uint32_t sw = 0; sw < 4; ++sw)
+            for (uint64_t m = lt[sw]; m; m &= m - 1) {
+                const uint32_t q = sw * 64 + (uint32_t)__builtin_ctzll(m);
+                if (q == p) continue;
+                const uint32_t c = (uint32_t)T[q] - lost(q);
                 if (!c) return false;
                 pmn = c < pmn ? c : pmn;
+            }
+        for (uint32_t sw = 0; sw < 4; ++sw)
+            for (uint64_t m = ht[sw]; m; m &= m - 1) {
+                const uint32_t q = sw * 64 + (uint32_t)__builtin_ctzll(m);
+                if (q == p) continue;
+                const uint32_t c = (uint32_t)T[q] - lost(q);
                 pmx = c > pmx ? c : pmx;
             }
-        }
         if (score) {
             mn = pmn < mn ? pmn : mn;
             mx = pmx > mx ? pmx : mx;
@@ -359,33 +424,104 @@ struct Wave {
         return true;
     }
 
-    // remove_invalid_transitions: false when a slot runs empty.
+    // Slot context shared by every candidate of slot i: its free neighbour
+    // on each side and, for each side and colour, how many options of that
+    // neighbour show the colour on the facing side.
+    __device__ void slot_context(uint32_t i, bool scoring = false) {
+        const uint32_t nc = a.max_colour + 1;
+        if (lane < 4) {
+            const uint32_t q = (uint32_t)((int)pos_of_slot[i] + off[lane]);
+            snb[lane] = (q < a.P && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        for (uint32_t e = lane; e < 4 * nc; e += kWave) {
+            const uint32_t l = e / nc, c = e % nc, j = snb[l];
+            uint32_t v = 0;
+            if (c && j != 0xFFFFFFFFu) {
+                const uint64_t *cm = a.cm + ((size_t)(c - 1) * 4 + l) * a.nw;
+                for (uint32_t w = 0; w < a.nw; ++w) v += (uint32_t)__popcll(M[j * a.nw + w] & cm[w]);
+            }
+            cnt[e] = (uint16_t)v;
+        }
+        // A candidate of slot i can leave a piece without option only if
+        // every option of that piece lies in slot i or its free neighbours.
+        const uint32_t n_crit = scratch[0];
+        uint32_t n_risk = 0;
+        for (uint32_t base = 0; base < n_crit; base += kWave) {
+            const uint32_t k = base + lane;
+            bool at_risk = false;
+            uint32_t q = 0;
+            if (k < n_crit) {
+                q = crit[k];
+                const uint32_t w = q >> 4;
+                uint32_t here = nib(M[i * a.nw + w], q);
+                for (uint32_t l = 0; l < 4; ++l)
+                    if (snb[l] != 0xFFFFFFFFu) here += nib(M[snb[l] * a.nw + w], q);
+                at_risk = here == (uint32_t)T[q];
+            }
+            const uint64_t m = __ballot(at_risk);
+            if (at_risk) risk[n_risk + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = (uint16_t)q;
+            n_risk += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) scratch[1] = n_risk;
+        __syncthreads();
+        if (scoring) {
+            const uint64_t *sets = reinterpret_cast<const uint64_t *>(scratch + 16);
+            uint32_t mn = 0xFFFFFFFFu, mx = 0;
+            for (uint32_t sw = 0; sw < 4; ++sw) {
+                const uint32_t q = sw * 64 + lane;
+                bool touched = false;
+                if (q < a.P) {
+                    const uint32_t w = q >> 4;
+                    uint32_t ori = nibble(M[i * a.nw + w], q);
+                    for (uint32_t l = 0; l < 4; ++l)
+                        if (snb[l] != 0xFFFFFFFFu) ori |= nibble(M[snb[l] * a.nw + w], q);
+                    touched = ori != 0;
+                }
+                const bool in_lo = (sets[sw] >> lane) & 1, in_hi = (sets[4 + sw] >> lane) & 1;
+                lt[sw] = __ballot(in_lo && touched);
+                ht[sw] = __ballot(in_hi && touched);
+                if (in_lo && !touched) mn = (uint32_t)T[q] < mn ? (uint32_t)T[q] : mn;
+                if (in_hi && !touched) mx = (uint32_t)T[q] > mx ? (uint32_t)T[q] : mx;
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t omn = __shfl_xor(mn, o), omx = __shfl_xor(mx, o);
+                mn = omn < mn ? omn : mn;
+                mx = omx > mx ? omx : mx;
+            }
+            min_u = mn;
+            max_u = mx;
+        }
+    }
+
+    // remove_invalid_transitions: false when a slot runs empty.  Slot by
+    // slot; lane b takes bit b of each word of the slot's mask.
     __device__ bool fixpoint() {
         uint64_t evaluated = 0;
         while (true) {
             uint32_t PC;
             if (!stats(PC)) return false;
             bool removed = false;
-            const uint32_t pairs = n() * a.nw;
-            for (uint32_t k = lane; k < pairs; k += kWave) {
-                uint64_t v = M[k];
-                const uint32_t j = k / a.nw, w = k % a.nw;
-                while (v) {
-                    const uint32_t b = (uint32_t)__builtin_ctzll(v);
-                    v &= v - 1;
-                    const uint32_t bit = w * 64 + b;
+            const uint32_t nn = n();
+            for (uint32_t i = 0; i < nn; ++i) {
+                slot_context(i);
+                for (uint32_t w = 0; w < a.nw; ++w) {
+                    const uint64_t v = M[i * a.nw + w];
+                    if (!((v >> lane) & 1)) continue;
+                    const uint32_t bit = w * 64 + lane;
                     uint32_t t, mm;
                     ++evaluated;
-                    if (!evaluate(j, bit >> 2, bit & 3, false, PC, t, mm)) {
-                        atomicAnd(reinterpret_cast<unsigned long long *>(&M[k]), ~(1ull << b));
+                    if (!evaluate(i, bit >> 2, bit & 3, false, PC, t, mm)) {
+                        atomicAnd(reinterpret_cast<unsigned long long *>(&M[i * a.nw + w]), ~(1ull << lane));
                         removed = true;
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
+            if (lane == 0) atomicAdd(&a.evals[2], 1ull);
             if (!__ballot(removed)) break;
         }
-        count(evaluated);
+        count(evaluated, false);
         uint32_t PC;
         return stats(PC);
     }
@@ -399,14 +535,13 @@ struct Wave {
         uint32_t best_idx = 0xFFFFFFFFu;
         bool have = false;
         uint64_t evaluated = 0;
-        const uint32_t pairs = n() * a.nw;
-        for (uint32_t k = lane; k < pairs; k += kWave) {
-            uint64_t v = M[k];
-            const uint32_t j = k / a.nw, w = k % a.nw;
-            while (v) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(v);
-                v &= v - 1;
-                const uint32_t bit = w * 64 + b;
+        const uint32_t nn = n();
+        for (uint32_t j = 0; j < nn; ++j) {
+            slot_context(j, true);
+            for (uint32_t w = 0; w < a.nw; ++w) {
+                const uint64_t v = M[j * a.nw + w];
+                if (!((v >> lane) & 1)) continue;
+                const uint32_t bit = w * 64 + lane;
                 uint32_t total = 0, mm = 0;
                 ++evaluated;
                 if (!evaluate(j, bit >> 2, bit & 3, true, PC, total, mm)) continue;
@@ -422,8 +557,9 @@ struct Wave {
                     have = true;
                 }
             }
+            __syncthreads();
         }
-        count(evaluated);
+        count(evaluated, true);
         for (int o = 32; o > 0; o >>= 1) {
             const uint64_t ok = __shfl_xor(best_key, o);
             const uint32_t oi = __shfl_xor(best_idx, o);
@@ -701,7 +837,7 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     auto wall0 = std::chrono::steady_clock::now();
     GluttonTables t = build_glutton_tables(pz);
     const uint32_t S = t.P;
-    const GlLayout L = GlLayout::make(S, t.P, t.nw);
+    const GlLayout L = GlLayout::make(S, t.P, t.nw, t.max_colour);
     if (L.total > dev.lds_per_block) throw Unsupported("glutton LDS layout exceeds the device limit");
     const uint32_t waves_per_cu = cfg.waves_per_cu ? cfg.waves_per_cu
                                                    : std::max<uint32_t>(1, std::min<uint32_t>(8, (uint32_t)(160 * 1024 / L.total)));
@@ -714,21 +850,22 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     uint32_t *d_pre = ws.get<uint32_t>("gl_prefixes", std::max<size_t>(prefixes.size(), 1));
     uint64_t *d_stacks = ws.get<uint64_t>("gl_stacks", tri * waves);
     uint32_t *d_maxp = ws.get<uint32_t>("gl_max_played", (size_t)S * waves);
+    uint32_t *d_played = ws.get<uint32_t>("gl_played", (size_t)S * waves);
     unsigned int *d_next = ws.get<unsigned int>("gl_next", 2);
     uint32_t *d_depth = ws.get<uint32_t>("gl_depth", n_items);
     uint64_t *d_pushes = ws.get<uint64_t>("gl_pushes", n_items);
     uint32_t *d_status = ws.get<uint32_t>("gl_status", n_items);
     uint16_t *d_board = ws.get<uint16_t>("gl_board", (size_t)n_items * t.P);
-    unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 1);
+    unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 3);
     GL_CHECK(hipMemcpyAsync(d_cm, t.cm.data(), t.cm.size() * 8, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_pcol, t.piece_col.data(), t.piece_col.size() * 4, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_init, t.init_masks.data(), t.init_masks.size() * 8, hipMemcpyHostToDevice, stream));
     if (!prefixes.empty())
         GL_CHECK(hipMemcpyAsync(d_pre, prefixes.data(), prefixes.size() * 4, hipMemcpyHostToDevice, stream));
     const unsigned int zero2[2] = {0, 0};
-    const unsigned long long zero64 = 0;
+    const unsigned long long zero64[3] = {0, 0, 0};
     GL_CHECK(hipMemcpyAsync(d_next, zero2, sizeof(zero2), hipMemcpyHostToDevice, stream));
-    GL_CHECK(hipMemcpyAsync(d_evals, &zero64, 8, hipMemcpyHostToDevice, stream));
+    GL_CHECK(hipMemcpyAsync(d_evals, zero64, sizeof(zero64), hipMemcpyHostToDevice, stream));
 
     GlArgs a{};
     a.P = t.P;
@@ -745,6 +882,7 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     a.push_budget = cfg.push_budget;
     a.stacks = d_stacks;
     a.max_played = d_maxp;
+    a.played = d_played;
     a.next_item = d_next;
     a.out_max_depth = d_depth;
     a.out_pushes = d_pushes;
@@ -768,12 +906,12 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     std::vector<uint32_t> depth(n_items), status(n_items);
     std::vector<uint64_t> pushes(n_items);
     std::vector<uint16_t> boards((size_t)n_items * t.P);
-    unsigned long long evals = 0;
+    unsigned long long evals[3] = {0, 0, 0};
     GL_CHECK(hipMemcpyAsync(depth.data(), d_depth, n_items * 4, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(status.data(), d_status, n_items * 4, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(pushes.data(), d_pushes, n_items * 8, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(boards.data(), d_board, boards.size() * 2, hipMemcpyDeviceToHost, stream));
-    GL_CHECK(hipMemcpyAsync(&evals, d_evals, 8, hipMemcpyDeviceToHost, stream));
+    GL_CHECK(hipMemcpyAsync(evals, d_evals, sizeof(evals), hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipStreamSynchronize(stream));
     out.assign(n_items, GluttonOutcome{});
     stats = GluttonStats{};
@@ -785,7 +923,9 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
         stats.pushes += pushes[i];
     }
     stats.kernel_ms = ms;
-    stats.evaluations = evals;
+    stats.evaluations = evals[0];
+    stats.score_evaluations = evals[1];
+    stats.rounds = evals[2];
     stats.waves = waves;
     stats.lds_bytes = L.total;
     stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();

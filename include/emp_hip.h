@@ -217,6 +217,8 @@ typedef struct {
     uint64_t evaluations;      /* candidate evaluations                      */
     uint32_t waves;
     uint32_t lds_bytes;        /* LDS per wave                               */
+    uint64_t score_evaluations; /* evaluations that scored a candidate       */
+    uint64_t rounds;           /* rounds of invalid-candidate removal        */
 } emp_hip_glutton_stats;
 
 /* boards: [n_searches][P], the solution or the deepest situation reached
