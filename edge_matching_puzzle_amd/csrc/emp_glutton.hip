@@ -108,6 +108,47 @@ __device__ __forceinline__ uint32_t facing(uint32_t ori, uint32_t qcol, uint32_t
 
 __device__ __forceinline__ uint32_t nibble(uint64_t v, uint32_t q) { return (uint32_t)(v >> (4 * (q & 15))) & 0xF; }
 
+// Largest facing(ori, qcol, l, col) over colours col: the most options in
+// `ori` that one colour toward the slot on side l can leave.
+__device__ __forceinline__ uint32_t most_facing(uint32_t ori, uint32_t qcol, uint32_t l) {
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t o = 0; o < 4; ++o) {
+        const uint32_t f = ((ori >> o) & 1) ? facing(ori, qcol, l, (qcol >> (8 * ((l + 2 + o) & 3))) & 0xFF) : 0;
+        m = f > m ? f : m;
+    }
+    return m;
+}
+
+// The 64 smallest of the 256 values v[0..3] of the 64 lanes, in ascending
+// order: lane r returns the r-th smallest.  Bitonic sort of each column
+// across the lanes, then two rounds of merges (the smallest 64 of two sorted
+// lists are min(x[r], y[63 - r]), a bitonic sequence, cleaned in 6 steps).
+__device__ __forceinline__ uint32_t smallest64(uint32_t (&v)[4], uint32_t lane) {
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t pv = (uint32_t)__shfl_xor((int)v[r], (int)j);
+                v[r] = keep_min ? (pv < v[r] ? pv : v[r]) : (pv > v[r] ? pv : v[r]);
+            }
+        }
+    auto merge = [lane](uint32_t x, uint32_t y) {
+        const uint32_t yr = (uint32_t)__shfl((int)y, (int)(63 - lane));
+        uint32_t c = yr < x ? yr : x;
+#pragma unroll
+        for (uint32_t j = 32; j > 0; j >>= 1) {
+            const uint32_t pv = (uint32_t)__shfl_xor((int)c, (int)j);
+            c = (lane & j) == 0 ? (pv < c ? pv : c) : (pv > c ? pv : c);
+        }
+        return c;
+    };
+    return merge(merge(v[0], v[1]), merge(v[2], v[3]));
+}
+
 struct Wave {
     const GlArgs &a;
     uint64_t *M;
@@ -125,11 +166,11 @@ struct Wave {
     uint32_t *max_played;
     uint32_t lane;
     uint32_t level, max, base;
-    // slot context for scoring: pieces of the min / max sets touched by the
-    // slot or its neighbours, and the extremes over the untouched ones
-    uint64_t lt[4], ht[4];
-    uint32_t min_u, max_u;
-    int off[4];
+    // slot context for scoring, one key per lane, ascending across lanes:
+    // the 64 available pieces q with the smallest lower bound lb(q) on the
+    // options q keeps after a move into the slot (key lb << 8 | q), and the
+    // 64 with the largest upper bound ub(q) (key (0xFFFF - ub) << 8 | q).
+    uint32_t key_lo, key_hi;
 
     __device__ Wave(const GlArgs &args, unsigned char *lds, uint32_t wave_id) : a(args) {
         GlLayout L = GlLayout::make(a.S, a.P, a.nw, a.max_colour);
@@ -153,12 +194,12 @@ struct Wave {
         stack = a.stacks + (size_t)wave_id * ((size_t)a.S * (a.S + 1) / 2) * a.nw;
         max_played = a.max_played + (size_t)wave_id * a.S;
         lane = threadIdx.x;
-        off[0] = -(int)a.W;
-        off[1] = 1;
-        off[2] = (int)a.W;
-        off[3] = -1;
     }
 
+    // position offset of the neighbour on side l (north, east, south, west)
+    __device__ int off(uint32_t l) const {
+        return l == 0 ? -(int)a.W : (l == 1 ? 1 : (l == 2 ? (int)a.W : -1));
+    }
     __device__ uint32_t n() const { return a.S - level; }
     // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
     __device__ void count(uint64_t evaluated, bool scoring) {
@@ -244,44 +285,6 @@ struct Wave {
         const bool any_empty = __ballot(empty) != 0;
         __syncthreads();
         if (extremes) {
-            // Pieces that can hold the min / max option count after a move:
-            // a move takes at most 4 + 4 * 4 = 20 options from a piece, so the
-            // min lies within 20 of the second smallest total and the max
-            // within 20 of the second largest (the second, so that the
-            // candidate's own piece can be left out).  Kept as 256-bit masks
-            // at scratch[16..23] (low) and scratch[24..31] (high).
-            uint32_t m1 = 0xFFFFu, m2 = 0xFFFFu, x1 = 0, x2 = 0;
-            auto lo2 = [](uint32_t &a1, uint32_t &a2, uint32_t v) {
-                if (v < a1) { a2 = a1; a1 = v; } else if (v < a2) a2 = v;
-            };
-            auto hi2 = [](uint32_t &a1, uint32_t &a2, uint32_t v) {
-                if (v > a1) { a2 = a1; a1 = v; } else if (v > a2) a2 = v;
-            };
-            for (uint32_t q = lane; q < a.P; q += kWave)
-                if (piece_avail(q)) {
-                    lo2(m1, m2, T[q]);
-                    hi2(x1, x2, T[q]);
-                }
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint32_t om1 = __shfl_xor(m1, o), om2 = __shfl_xor(m2, o);
-                const uint32_t ox1 = __shfl_xor(x1, o), ox2 = __shfl_xor(x2, o);
-                lo2(m1, m2, om1);
-                lo2(m1, m2, om2);
-                hi2(x1, x2, ox1);
-                hi2(x1, x2, ox2);
-            }
-            const uint32_t lo_cut = m2 + 20, hi_cut = x2 > 20 ? x2 - 20 : 0;
-            uint64_t *sets = reinterpret_cast<uint64_t *>(scratch + 16);
-            for (uint32_t base = 0; base < 256; base += kWave) {
-                const uint32_t q = base + lane;
-                const bool av = q < a.P && piece_avail(q);
-                const uint64_t in_lo = __ballot(av && T[q] <= lo_cut);
-                const uint64_t in_hi = __ballot(av && T[q] >= hi_cut);
-                if (lane == 0) {
-                    sets[base / 64] = in_lo;
-                    sets[4 + base / 64] = in_hi;
-                }
-            }
             for (uint32_t q = lane; q < a.P; q += kWave) {
                 uint32_t mnv = 0xFFFFu, mns = 0, mxv = 0, mxs = 0;
                 const uint32_t w = q >> 4;
@@ -303,30 +306,36 @@ struct Wave {
     __device__ bool evaluate(uint32_t i, uint32_t p, uint32_t o, bool score, uint32_t PC,
                              uint32_t &total, uint32_t &minmax) {
         const uint32_t nn = n();
-        const uint32_t pos = pos_of_slot[i];
         const uint32_t pc_p = pcol[p];
         const uint32_t pw = p >> 4;
-        const uint64_t keep_p = ~(0xFull << (4 * (p & 15)));
-        uint32_t nb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-        uint32_t cs[4] = {0, 0, 0, 0};   // colour the candidate shows toward each constrained neighbour
+        // Constrained neighbours packed in registers: slot of side l in bits
+        // 16l..16l+15 (0xFFFF: none), colour the candidate shows toward it in
+        // bits 8l..8l+7.  Rolled loops over the packed words keep the code
+        // small (an unrolled form is slower: more code, more branches).
+        uint64_t nbp = ~0ull;
+        uint32_t csp = 0;
         uint32_t rsum = 0, nb_pc_minus = 0;
         uint32_t mn = 0xFFFFFFFFu, mx = 0;
+        // every free neighbour constrained (the upper bounds of the slot
+        // context assume it; a frame colour only ever faces the frame)
+        bool bounded = true;
         // Colour-constrained free neighbours (apply_color_constraints :528-612):
         // the options of the neighbour matching the candidate's side, from
         // the slot context, less those of piece p itself.  (A coloured side
         // never faces the frame, so the neighbour lookup needs no edge test.)
-        (void)pos;
-        (void)keep_p;
         for (uint32_t l = 0; l < 4; ++l) {
             const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
-            if (!col) continue;
+            if (!col) {
+                bounded &= snb[l] == 0xFFFFFFFFu;
+                continue;
+            }
             const uint32_t j = snb[l];
             if (j == 0xFFFFFFFFu) continue;
             const uint32_t rp =
                 cnt[l * (a.max_colour + 1) + col] - facing(nibble(M[j * a.nw + pw], p), pc_p, l, col);
             if (!rp) return false;
-            nb[l] = j;
-            cs[l] = col;
+            nbp &= ~(0xFFFFull << (16 * l)) | ((uint64_t)j << (16 * l));
+            csp |= col << (8 * l);
             rsum += rp;
             nb_pc_minus += pc[j] - nib(M[j * a.nw + pw], p);
             if (score) {
@@ -334,15 +343,33 @@ struct Wave {
                 mx = rp > mx ? rp : mx;
             }
         }
+        auto nbs = [&](uint32_t l) { return (uint32_t)(nbp >> (16 * l)) & 0xFFFFu; };
         // Other free slots: invalid when piece p is all one of them holds.
         uint32_t own_single = sp[i] == p;
-        for (uint32_t l = 0; l < 4; ++l) own_single += nb[l] != 0xFFFFFFFFu && sp[nb[l]] == p;
+        for (uint32_t l = 0; l < 4; ++l) {
+            const uint32_t j = nbs(l);
+            own_single += j != 0xFFFFu && sp[j] == p;
+        }
         if (single[p] > own_single) return false;
+        // Options piece q loses by the move: its orientations in slot i and
+        // those the colour constraints remove at the neighbours.
+        auto lost = [&](uint32_t q) {
+            const uint32_t w = q >> 4;
+            uint32_t d = nib(M[i * a.nw + w], q);
+            const uint32_t qc = pcol[q];
+            for (uint32_t l = 0; l < 4; ++l) {
+                const uint32_t j = nbs(l);
+                if (j == 0xFFFFu) continue;
+                const uint32_t ori = nibble(M[j * a.nw + w], q);
+                d += (uint32_t)__popc(ori) - facing(ori, qc, l, (csp >> (8 * l)) & 0xFF);
+            }
+            return d;
+        };
         if (score) {
             // Free slots other than i and the constrained neighbours: the
             // per-piece extremes, unless they sit in one of those slots.
             auto excluded = [&](uint32_t j) {
-                return j == i || j == nb[0] || j == nb[1] || j == nb[2] || j == nb[3];
+                return j == i || j == nbs(0) || j == nbs(1) || j == nbs(2) || j == nbs(3);
             };
             const uint32_t l = lo[p], h = hi[p];
             if (!excluded(l >> 16) && !excluded(h >> 16)) {
@@ -366,54 +393,52 @@ struct Wave {
             for (uint32_t k = 0; k < nc; ++k) {
                 const uint32_t q = risk[k];
                 if (q == p) continue;
-                const uint32_t w = q >> 4;
-                uint32_t d = nib(M[i * a.nw + w], q);
-                const uint32_t qc = pcol[q];
-                for (uint32_t l = 0; l < 4; ++l) {
-                    if (nb[l] == 0xFFFFFFFFu) continue;
-                    const uint32_t ori = nibble(M[nb[l] * a.nw + w], q);
-                    d += (uint32_t)__popc(ori) - facing(ori, qc, l, cs[l]);
-                }
-                if ((uint32_t)T[q] == d) return false;
+                if ((uint32_t)T[q] == lost(q)) return false;
             }
             return true;
         }
         // Option counts of the other available pieces after the move: T(q)
-        // less its orientations in slot i and those the colour constraints
-        // remove at the neighbours.  Only the pieces that can hold the min
-        // (which include every piece that could drop to zero) and the max
-        // are visited (sets built by stats()).
-        // Pieces untouched by the slot and its neighbours keep their totals:
-        // their extremes come from the slot context (a zero total there
-        // means no candidate of the slot is valid).
-        if (min_u == 0) return false;
-        uint32_t pmn = min_u, pmx = max_u;
-        auto lost = [&](uint32_t q) {
-            const uint32_t w = q >> 4;
-            uint32_t d = nib(M[i * a.nw + w], q);
-            const uint32_t qc = pcol[q];
-            for (uint32_t l = 0; l < 4; ++l) {
-                if (nb[l] == 0xFFFFFFFFu) continue;
-                const uint32_t ori = nibble(M[nb[l] * a.nw + w], q);
-                d += (uint32_t)__popc(ori) - facing(ori, qc, l, cs[l]);
+        // less what the move takes; their min (0: invalid move) and max.
+        // Pieces are visited in the order of the slot context's bounds, until
+        // the next bound cannot improve the extreme; past the 64 listed
+        // pieces (or without bounds), every available piece.
+        uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
+        bool open = true;
+        for (uint32_t r = 0; r < kWave; ++r) {
+            const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_lo, (int)r);
+            if (k == 0xFFFFFFFFu || (k >> 8) >= pmn) {
+                open = false;
+                break;
             }
-            return d;
-        };
-        for (uint32_t sw = 0; sw < 4; ++sw)
-            for (uint64_t m = lt[sw]; m; m &= m - 1) {
-                const uint32_t q = sw * 64 + (uint32_t)__builtin_ctzll(m);
-                if (q == p) continue;
-                const uint32_t c = (uint32_t)T[q] - lost(q);
-                if (!c) return false;
-                pmn = c < pmn ? c : pmn;
-            }
-        for (uint32_t sw = 0; sw < 4; ++sw)
-            for (uint64_t m = ht[sw]; m; m &= m - 1) {
-                const uint32_t q = sw * 64 + (uint32_t)__builtin_ctzll(m);
+            const uint32_t q = k & 0xFF;
+            if (q == p) continue;
+            const uint32_t c = (uint32_t)T[q] - lost(q);
+            pmn = c < pmn ? c : pmn;
+        }
+        if (open)
+            for (uint32_t q = 0; q < a.P; ++q)
+                if (q != p && piece_avail(q)) {
+                    const uint32_t c = (uint32_t)T[q] - lost(q);
+                    pmn = c < pmn ? c : pmn;
+                }
+        if (pmn == 0) return false;
+        open = !bounded;
+        if (bounded)
+            for (uint32_t r = 0; r < kWave; ++r) {
+                const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_hi, (int)r);
+                if (k == 0xFFFFFFFFu || 0xFFFFu - (k >> 8) <= pmx) break;
+                if (r == kWave - 1) open = true;
+                const uint32_t q = k & 0xFF;
                 if (q == p) continue;
                 const uint32_t c = (uint32_t)T[q] - lost(q);
                 pmx = c > pmx ? c : pmx;
             }
+        if (open)
+            for (uint32_t q = 0; q < a.P; ++q)
+                if (q != p && piece_avail(q)) {
+                    const uint32_t c = (uint32_t)T[q] - lost(q);
+                    pmx = c > pmx ? c : pmx;
+                }
         if (score) {
             mn = pmn < mn ? pmn : mn;
             mx = pmx > mx ? pmx : mx;
@@ -430,7 +455,7 @@ struct Wave {
     __device__ void slot_context(uint32_t i, bool scoring = false) {
         const uint32_t nc = a.max_colour + 1;
         if (lane < 4) {
-            const uint32_t q = (uint32_t)((int)pos_of_slot[i] + off[lane]);
+            const uint32_t q = (uint32_t)((int)pos_of_slot[i] + off(lane));
             snb[lane] = (q < a.P && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
         }
         __syncthreads();
@@ -455,6 +480,7 @@ struct Wave {
                 q = crit[k];
                 const uint32_t w = q >> 4;
                 uint32_t here = nib(M[i * a.nw + w], q);
+#pragma unroll
                 for (uint32_t l = 0; l < 4; ++l)
                     if (snb[l] != 0xFFFFFFFFu) here += nib(M[snb[l] * a.nw + w], q);
                 at_risk = here == (uint32_t)T[q];
@@ -466,31 +492,34 @@ struct Wave {
         if (lane == 0) scratch[1] = n_risk;
         __syncthreads();
         if (scoring) {
-            const uint64_t *sets = reinterpret_cast<const uint64_t *>(scratch + 16);
-            uint32_t mn = 0xFFFFFFFFu, mx = 0;
-            for (uint32_t sw = 0; sw < 4; ++sw) {
-                const uint32_t q = sw * 64 + lane;
-                bool touched = false;
-                if (q < a.P) {
-                    const uint32_t w = q >> 4;
-                    uint32_t ori = nibble(M[i * a.nw + w], q);
-                    for (uint32_t l = 0; l < 4; ++l)
-                        if (snb[l] != 0xFFFFFFFFu) ori |= nibble(M[snb[l] * a.nw + w], q);
-                    touched = ori != 0;
+            // Per available piece q: lb = T(q) less its options in slot i and
+            // in the free neighbours (what q keeps after the worst move into
+            // slot i); ub = lb plus, per free neighbour, the most options
+            // there that one colour toward slot i can leave (what q keeps
+            // after the best move).  A move's effect on q lies in [lb, ub].
+            uint32_t kl[4], kh[4];
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t q = r * 64 + lane;
+                kl[r] = kh[r] = 0xFFFFFFFFu;
+                if (q < a.P && piece_avail(q)) {
+                    const uint32_t w = q >> 4, qc = pcol[q];
+                    uint32_t touched = nib(M[i * a.nw + w], q), keep = 0;
+#pragma unroll
+                    for (uint32_t l = 0; l < 4; ++l) {
+                        const uint32_t j = snb[l];
+                        if (j == 0xFFFFFFFFu) continue;
+                        const uint32_t ori = nibble(M[j * a.nw + w], q);
+                        touched += (uint32_t)__popc(ori);
+                        keep += most_facing(ori, qc, l);
+                    }
+                    const uint32_t lb = (uint32_t)T[q] - touched;
+                    kl[r] = lb << 8 | q;
+                    kh[r] = (0xFFFFu - (lb + keep)) << 8 | q;
                 }
-                const bool in_lo = (sets[sw] >> lane) & 1, in_hi = (sets[4 + sw] >> lane) & 1;
-                lt[sw] = __ballot(in_lo && touched);
-                ht[sw] = __ballot(in_hi && touched);
-                if (in_lo && !touched) mn = (uint32_t)T[q] < mn ? (uint32_t)T[q] : mn;
-                if (in_hi && !touched) mx = (uint32_t)T[q] > mx ? (uint32_t)T[q] : mx;
             }
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint32_t omn = __shfl_xor(mn, o), omx = __shfl_xor(mx, o);
-                mn = omn < mn ? omn : mn;
-                mx = omx > mx ? omx : mx;
-            }
-            min_u = mn;
-            max_u = mx;
+            key_lo = smallest64(kl, lane);
+            key_hi = smallest64(kh, lane);
         }
     }
 
@@ -664,10 +693,11 @@ struct Wave {
         push(slot, pos, piece, o);
         const uint32_t pcol = a.piece_col[piece];
         bool emptied = false;
+#pragma unroll
         for (uint32_t l = 0; l < 4; ++l) {
             const uint32_t col = (pcol >> (8 * ((l + o) & 3))) & 0xFF;
             if (!col) continue;
-            const uint32_t q = (uint32_t)((int)pos + off[l]);
+            const uint32_t q = (uint32_t)((int)pos + off(l));
             if (q >= a.P || !is_free(q)) continue;
             const uint32_t j = slot_of_pos[q];
             const uint64_t *c = a.cm + ((size_t)(col - 1) * 4 + l) * a.nw;
