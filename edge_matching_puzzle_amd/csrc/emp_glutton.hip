@@ -455,8 +455,13 @@ struct Wave {
     __device__ void slot_context(uint32_t i, bool scoring = false) {
         const uint32_t nc = a.max_colour + 1;
         if (lane < 4) {
-            const uint32_t q = (uint32_t)((int)pos_of_slot[i] + off(lane));
-            snb[lane] = (q < a.P && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
+            const uint32_t pos = pos_of_slot[i], x = pos % a.W;
+            const bool inside = lane == 0   ? pos >= a.W
+                                : lane == 1 ? x + 1 < a.W
+                                : lane == 2 ? pos + a.W < a.P
+                                            : x > 0;
+            const uint32_t q = (uint32_t)((int)pos + off(lane));
+            snb[lane] = (inside && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
         }
         __syncthreads();
         for (uint32_t e = lane; e < 4 * nc; e += kWave) {
