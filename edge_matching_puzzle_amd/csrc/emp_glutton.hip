@@ -120,6 +120,21 @@ __device__ __forceinline__ uint32_t most_facing(uint32_t ori, uint32_t qcol, uin
     return m;
 }
 
+// Colours that piece colours qcol shows toward the slot on side l (its side
+// (l + 2 + o) & 3 at orientation o) in byte o, for the orientations in the
+// 4-bit set `ori`; 0 for the others (0 is never a candidate's colour there).
+__device__ __forceinline__ uint32_t facing_colours(uint32_t ori, uint32_t qcol, uint32_t l) {
+    const uint32_t r = 8 * ((l + 2) & 3);
+    const uint32_t rot = r ? (qcol >> r) | (qcol << (32 - r)) : qcol;
+    return rot & (((ori * 0x00204081u) & 0x01010101u) * 0xFFu);
+}
+
+// Bytes of x equal to zero.
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {
+    const uint32_t y = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return (uint32_t)__popc(~(y | x | 0x7F7F7F7Fu));
+}
+
 // The 64 smallest of the 256 values v[0..3] of the 64 lanes, in ascending
 // order: lane r returns the r-th smallest.  Bitonic sort of each column
 // across the lanes, then two rounds of merges (the smallest 64 of two sorted
@@ -171,6 +186,11 @@ struct Wave {
     // options q keeps after a move into the slot (key lb << 8 | q), and the
     // 64 with the largest upper bound ub(q) (key (0xFFFF - ub) << 8 | q).
     uint32_t key_lo, key_hi;
+    // for the piece of each listed key: the colours it shows toward the slot
+    // from each free neighbour (facing_colours), and lb for the upper list
+    uint32_t lo_f0, lo_f1, lo_f2, lo_f3, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3;
+    // slot context: free neighbour slot of side l in bits 16l..16l+15 (0xFFFF: none)
+    uint64_t snbp;
 
     __device__ Wave(const GlArgs &args, unsigned char *lds, uint32_t wave_id) : a(args) {
         GlLayout L = GlLayout::make(a.S, a.P, a.nw, a.max_colour);
@@ -325,12 +345,12 @@ struct Wave {
         // never faces the frame, so the neighbour lookup needs no edge test.)
         for (uint32_t l = 0; l < 4; ++l) {
             const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
+            const uint32_t j = (uint32_t)(snbp >> (16 * l)) & 0xFFFFu;
             if (!col) {
-                bounded &= snb[l] == 0xFFFFFFFFu;
+                bounded &= j == 0xFFFFu;
                 continue;
             }
-            const uint32_t j = snb[l];
-            if (j == 0xFFFFFFFFu) continue;
+            if (j == 0xFFFFu) continue;
             const uint32_t rp =
                 cnt[l * (a.max_colour + 1) + col] - facing(nibble(M[j * a.nw + pw], p), pc_p, l, col);
             if (!rp) return false;
@@ -401,10 +421,19 @@ struct Wave {
         // less what the move takes; their min (0: invalid move) and max.
         // Pieces are visited in the order of the slot context's bounds, until
         // the next bound cannot improve the extreme; past the 64 listed
-        // pieces (or without bounds), every available piece.
+        // pieces (or when a free neighbour is left unconstrained, which the
+        // bounds and the listed colours do not cover), every available piece.
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
+        // T(q) - lost(q) = lb(q) + the options of q at the neighbours that
+        // show the candidate's colour toward slot i
+        const uint32_t b0 = (csp & 0xFF) * 0x01010101u, b1 = ((csp >> 8) & 0xFF) * 0x01010101u;
+        const uint32_t b2 = ((csp >> 16) & 0xFF) * 0x01010101u, b3 = (csp >> 24) * 0x01010101u;
+        auto matches = [&](uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
+            return (b0 ? zero_bytes(f0 ^ b0) : 0u) + (b1 ? zero_bytes(f1 ^ b1) : 0u) +
+                   (b2 ? zero_bytes(f2 ^ b2) : 0u) + (b3 ? zero_bytes(f3 ^ b3) : 0u);
+        };
         bool open = true;
-        for (uint32_t r = 0; r < kWave; ++r) {
+        for (uint32_t r = 0; r < kWave && bounded; ++r) {
             const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_lo, (int)r);
             if (k == 0xFFFFFFFFu || (k >> 8) >= pmn) {
                 open = false;
@@ -412,7 +441,10 @@ struct Wave {
             }
             const uint32_t q = k & 0xFF;
             if (q == p) continue;
-            const uint32_t c = (uint32_t)T[q] - lost(q);
+            const uint32_t c = (k >> 8) + matches(__builtin_amdgcn_readlane((int)lo_f0, (int)r),
+                                                  __builtin_amdgcn_readlane((int)lo_f1, (int)r),
+                                                  __builtin_amdgcn_readlane((int)lo_f2, (int)r),
+                                                  __builtin_amdgcn_readlane((int)lo_f3, (int)r));
             pmn = c < pmn ? c : pmn;
         }
         if (open)
@@ -430,7 +462,11 @@ struct Wave {
                 if (r == kWave - 1) open = true;
                 const uint32_t q = k & 0xFF;
                 if (q == p) continue;
-                const uint32_t c = (uint32_t)T[q] - lost(q);
+                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)r) +
+                                   matches(__builtin_amdgcn_readlane((int)hi_f0, (int)r),
+                                           __builtin_amdgcn_readlane((int)hi_f1, (int)r),
+                                           __builtin_amdgcn_readlane((int)hi_f2, (int)r),
+                                           __builtin_amdgcn_readlane((int)hi_f3, (int)r));
                 pmx = c > pmx ? c : pmx;
             }
         if (open)
@@ -464,6 +500,9 @@ struct Wave {
             snb[lane] = (inside && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
         }
         __syncthreads();
+        snbp = 0;
+#pragma unroll
+        for (uint32_t l = 0; l < 4; ++l) snbp |= (uint64_t)(snb[l] & 0xFFFFu) << (16 * l);
         for (uint32_t e = lane; e < 4 * nc; e += kWave) {
             const uint32_t l = e / nc, c = e % nc, j = snb[l];
             uint32_t v = 0;
@@ -525,6 +564,28 @@ struct Wave {
             }
             key_lo = smallest64(kl, lane);
             key_hi = smallest64(kh, lane);
+            // payload of each listed piece
+            auto payload = [&](uint32_t key, uint32_t &lb, uint32_t &f0, uint32_t &f1, uint32_t &f2, uint32_t &f3) {
+                lb = f0 = f1 = f2 = f3 = 0;
+                if (key == 0xFFFFFFFFu) return;
+                const uint32_t q = key & 0xFF, w = q >> 4, qc = pcol[q];
+                uint32_t touched = nib(M[i * a.nw + w], q);
+                auto side = [&](uint32_t l) {
+                    const uint32_t j = (uint32_t)(snbp >> (16 * l)) & 0xFFFFu;
+                    if (j == 0xFFFFu) return 0u;
+                    const uint32_t ori = nibble(M[j * a.nw + w], q);
+                    touched += (uint32_t)__popc(ori);
+                    return facing_colours(ori, qc, l);
+                };
+                f0 = side(0);
+                f1 = side(1);
+                f2 = side(2);
+                f3 = side(3);
+                lb = (uint32_t)T[q] - touched;
+            };
+            uint32_t lb_unused;
+            payload(key_lo, lb_unused, lo_f0, lo_f1, lo_f2, lo_f3);
+            payload(key_hi, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3);
         }
     }
 
