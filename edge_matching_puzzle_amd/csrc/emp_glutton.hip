@@ -191,6 +191,9 @@ struct Wave {
     uint32_t lo_f0, lo_f1, lo_f2, lo_f3, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3;
     // slot context: free neighbour slot of side l in bits 16l..16l+15 (0xFFFF: none)
     uint64_t snbp;
+    // slot context: the first 64 at-risk pieces, one per lane, with the
+    // colours they show toward the slot (facing_colours per side)
+    uint32_t rk_q, rk_f0, rk_f1, rk_f2, rk_f3;
 
     __device__ Wave(const GlArgs &args, unsigned char *lds, uint32_t wave_id) : a(args) {
         GlLayout L = GlLayout::make(a.S, a.P, a.nw, a.max_colour);
@@ -406,11 +409,31 @@ struct Wave {
                 }
             }
         }
+        // With every free neighbour constrained, T(q) - lost(q) = lb(q) + the
+        // options of q at the neighbours that show the candidate's colour
+        // toward slot i (byte compares against the listed facing colours).
+        const uint32_t b0 = (csp & 0xFF) * 0x01010101u, b1 = ((csp >> 8) & 0xFF) * 0x01010101u;
+        const uint32_t b2 = ((csp >> 16) & 0xFF) * 0x01010101u, b3 = (csp >> 24) * 0x01010101u;
+        auto matches = [&](uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
+            return (b0 ? zero_bytes(f0 ^ b0) : 0u) + (b1 ? zero_bytes(f1 ^ b1) : 0u) +
+                   (b2 ? zero_bytes(f2 ^ b2) : 0u) + (b3 ? zero_bytes(f3 ^ b3) : 0u);
+        };
         if (!score) {
-            // Only the critical pieces at risk in this slot can be left
-            // without option (slot context).
+            // Only the critical pieces at risk in this slot (lb = 0) can be
+            // left without option (slot context).
             const uint32_t nc = scratch[1];
-            for (uint32_t k = 0; k < nc; ++k) {
+            uint32_t k = 0;
+            if (bounded)
+                for (; k < nc && k < kWave; ++k) {
+                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)rk_q, (int)k);
+                    if (q == p) continue;
+                    if (!matches(__builtin_amdgcn_readlane((int)rk_f0, (int)k),
+                                 __builtin_amdgcn_readlane((int)rk_f1, (int)k),
+                                 __builtin_amdgcn_readlane((int)rk_f2, (int)k),
+                                 __builtin_amdgcn_readlane((int)rk_f3, (int)k)))
+                        return false;
+                }
+            for (; k < nc; ++k) {
                 const uint32_t q = risk[k];
                 if (q == p) continue;
                 if ((uint32_t)T[q] == lost(q)) return false;
@@ -424,14 +447,6 @@ struct Wave {
         // pieces (or when a free neighbour is left unconstrained, which the
         // bounds and the listed colours do not cover), every available piece.
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
-        // T(q) - lost(q) = lb(q) + the options of q at the neighbours that
-        // show the candidate's colour toward slot i
-        const uint32_t b0 = (csp & 0xFF) * 0x01010101u, b1 = ((csp >> 8) & 0xFF) * 0x01010101u;
-        const uint32_t b2 = ((csp >> 16) & 0xFF) * 0x01010101u, b3 = (csp >> 24) * 0x01010101u;
-        auto matches = [&](uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
-            return (b0 ? zero_bytes(f0 ^ b0) : 0u) + (b1 ? zero_bytes(f1 ^ b1) : 0u) +
-                   (b2 ? zero_bytes(f2 ^ b2) : 0u) + (b3 ? zero_bytes(f3 ^ b3) : 0u);
-        };
         bool open = true;
         for (uint32_t r = 0; r < kWave && bounded; ++r) {
             const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_lo, (int)r);
@@ -454,20 +469,27 @@ struct Wave {
                     pmn = c < pmn ? c : pmn;
                 }
         if (pmn == 0) return false;
+        // The max in steps of 4 listed pieces (independent, so their reads
+        // and compares overlap); a piece past the stopping bound cannot
+        // raise the max, so taking whole steps changes nothing.
         open = !bounded;
         if (bounded)
-            for (uint32_t r = 0; r < kWave; ++r) {
-                const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_hi, (int)r);
-                if (k == 0xFFFFFFFFu || 0xFFFFu - (k >> 8) <= pmx) break;
-                if (r == kWave - 1) open = true;
-                const uint32_t q = k & 0xFF;
-                if (q == p) continue;
-                const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)r) +
-                                   matches(__builtin_amdgcn_readlane((int)hi_f0, (int)r),
-                                           __builtin_amdgcn_readlane((int)hi_f1, (int)r),
-                                           __builtin_amdgcn_readlane((int)hi_f2, (int)r),
-                                           __builtin_amdgcn_readlane((int)hi_f3, (int)r));
-                pmx = c > pmx ? c : pmx;
+            for (uint32_t r = 0; r < kWave; r += 4) {
+                uint32_t last = 0;
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_hi, (int)(r + e));
+                    last = k;
+                    if (k == 0xFFFFFFFFu || (k & 0xFF) == p) continue;
+                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)(r + e)) +
+                                       matches(__builtin_amdgcn_readlane((int)hi_f0, (int)(r + e)),
+                                               __builtin_amdgcn_readlane((int)hi_f1, (int)(r + e)),
+                                               __builtin_amdgcn_readlane((int)hi_f2, (int)(r + e)),
+                                               __builtin_amdgcn_readlane((int)hi_f3, (int)(r + e)));
+                    pmx = c > pmx ? c : pmx;
+                }
+                if (last == 0xFFFFFFFFu || 0xFFFFu - (last >> 8) <= pmx) break;
+                if (r + 4 == kWave) open = true;
             }
         if (open)
             for (uint32_t q = 0; q < a.P; ++q)
@@ -483,6 +505,29 @@ struct Wave {
             minmax = (mx << 16) + mn;
         }
         return true;
+    }
+
+    // For the piece q = key & 0xFF (none: key 0xFFFFFFFF): lb = T(q) less its
+    // options in slot i and the free neighbours, and per side the colours q
+    // shows toward slot i from that neighbour (facing_colours; 0: none).
+    __device__ void payload(uint32_t i, uint32_t key, uint32_t &lb, uint32_t &f0, uint32_t &f1, uint32_t &f2,
+                            uint32_t &f3) {
+        lb = f0 = f1 = f2 = f3 = 0;
+        if (key == 0xFFFFFFFFu) return;
+        const uint32_t q = key & 0xFF, w = q >> 4, qc = pcol[q];
+        uint32_t touched = nib(M[i * a.nw + w], q);
+        auto side = [&](uint32_t l) {
+            const uint32_t j = (uint32_t)(snbp >> (16 * l)) & 0xFFFFu;
+            if (j == 0xFFFFu) return 0u;
+            const uint32_t ori = nibble(M[j * a.nw + w], q);
+            touched += (uint32_t)__popc(ori);
+            return facing_colours(ori, qc, l);
+        };
+        f0 = side(0);
+        f1 = side(1);
+        f2 = side(2);
+        f3 = side(3);
+        lb = (uint32_t)T[q] - touched;
     }
 
     // Slot context shared by every candidate of slot i: its free neighbour
@@ -535,6 +580,11 @@ struct Wave {
         }
         if (lane == 0) scratch[1] = n_risk;
         __syncthreads();
+        if (!scoring) {
+            uint32_t lb_unused;
+            rk_q = lane < n_risk ? (uint32_t)risk[lane] : 0xFFFFFFFFu;
+            payload(i, rk_q, lb_unused, rk_f0, rk_f1, rk_f2, rk_f3);
+        }
         if (scoring) {
             // Per available piece q: lb = T(q) less its options in slot i and
             // in the free neighbours (what q keeps after the worst move into
@@ -565,27 +615,9 @@ struct Wave {
             key_lo = smallest64(kl, lane);
             key_hi = smallest64(kh, lane);
             // payload of each listed piece
-            auto payload = [&](uint32_t key, uint32_t &lb, uint32_t &f0, uint32_t &f1, uint32_t &f2, uint32_t &f3) {
-                lb = f0 = f1 = f2 = f3 = 0;
-                if (key == 0xFFFFFFFFu) return;
-                const uint32_t q = key & 0xFF, w = q >> 4, qc = pcol[q];
-                uint32_t touched = nib(M[i * a.nw + w], q);
-                auto side = [&](uint32_t l) {
-                    const uint32_t j = (uint32_t)(snbp >> (16 * l)) & 0xFFFFu;
-                    if (j == 0xFFFFu) return 0u;
-                    const uint32_t ori = nibble(M[j * a.nw + w], q);
-                    touched += (uint32_t)__popc(ori);
-                    return facing_colours(ori, qc, l);
-                };
-                f0 = side(0);
-                f1 = side(1);
-                f2 = side(2);
-                f3 = side(3);
-                lb = (uint32_t)T[q] - touched;
-            };
             uint32_t lb_unused;
-            payload(key_lo, lb_unused, lo_f0, lo_f1, lo_f2, lo_f3);
-            payload(key_hi, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3);
+            payload(i, key_lo, lb_unused, lo_f0, lo_f1, lo_f2, lo_f3);
+            payload(i, key_hi, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3);
         }
     }
 
