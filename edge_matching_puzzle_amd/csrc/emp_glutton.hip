@@ -189,8 +189,13 @@ struct Wave {
     // for the piece of each listed key: the colours it shows toward the slot
     // from each free neighbour (facing_colours), and lb for the upper list
     uint32_t lo_f0, lo_f1, lo_f2, lo_f3, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3;
-    // slot context: free neighbour slot of side l in bits 16l..16l+15 (0xFFFF: none)
-    uint64_t snbp;
+    // slot context: free neighbour slot of side l in bits 16l..16l+15
+    // (0xFFFF: none), that neighbour's option count (pc) and single piece
+    // (sp) likewise; the slot's own pc
+    uint64_t snbp, pcnp, spnp;
+    uint32_t pc_i, sp_i;
+    // the current mask word of each free neighbour (0: none)
+    uint64_t nbw0, nbw1, nbw2, nbw3;
     // slot context: the first 64 at-risk pieces, one per lane, with the
     // colours they show toward the slot (facing_colours per side)
     uint32_t rk_q, rk_f0, rk_f1, rk_f2, rk_f3;
@@ -224,6 +229,19 @@ struct Wave {
         return l == 0 ? -(int)a.W : (l == 1 ? 1 : (l == 2 ? (int)a.W : -1));
     }
     __device__ uint32_t n() const { return a.S - level; }
+    __device__ uint64_t nbw(uint32_t l) const { return l == 0 ? nbw0 : (l == 1 ? nbw1 : (l == 2 ? nbw2 : nbw3)); }
+    __device__ uint32_t field(uint64_t packed, uint32_t l) const { return (uint32_t)(packed >> (16 * l)) & 0xFFFFu; }
+    // mask word w of the slot context's free neighbours
+    __device__ void neighbour_words(uint32_t w) {
+        uint32_t j = field(snbp, 0);
+        nbw0 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        j = field(snbp, 1);
+        nbw1 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        j = field(snbp, 2);
+        nbw2 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        j = field(snbp, 3);
+        nbw3 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+    }
     // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
     __device__ void count(uint64_t evaluated, bool scoring) {
         for (int o = 32; o > 0; o >>= 1) evaluated += __shfl_xor(evaluated, o);
@@ -333,8 +351,7 @@ struct Wave {
         const uint32_t pw = p >> 4;
         // Constrained neighbours packed in registers: slot of side l in bits
         // 16l..16l+15 (0xFFFF: none), colour the candidate shows toward it in
-        // bits 8l..8l+7.  Rolled loops over the packed words keep the code
-        // small (an unrolled form is slower: more code, more branches).
+        // bits 8l..8l+7.
         uint64_t nbp = ~0ull;
         uint32_t csp = 0;
         uint32_t rsum = 0, nb_pc_minus = 0;
@@ -346,33 +363,35 @@ struct Wave {
         // the options of the neighbour matching the candidate's side, from
         // the slot context, less those of piece p itself.  (A coloured side
         // never faces the frame, so the neighbour lookup needs no edge test.)
+        // Unrolled and branch-free: the four cnt reads go out together.
+        const uint32_t ncol = a.max_colour + 1;
+        bool invalid = false;
+#pragma unroll
         for (uint32_t l = 0; l < 4; ++l) {
             const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
-            const uint32_t j = (uint32_t)(snbp >> (16 * l)) & 0xFFFFu;
-            if (!col) {
-                bounded &= j == 0xFFFFu;
-                continue;
-            }
-            if (j == 0xFFFFu) continue;
-            const uint32_t rp =
-                cnt[l * (a.max_colour + 1) + col] - facing(nibble(M[j * a.nw + pw], p), pc_p, l, col);
-            if (!rp) return false;
-            nbp &= ~(0xFFFFull << (16 * l)) | ((uint64_t)j << (16 * l));
-            csp |= col << (8 * l);
-            rsum += rp;
-            nb_pc_minus += pc[j] - nib(M[j * a.nw + pw], p);
-            if (score) {
-                mn = rp < mn ? rp : mn;
-                mx = rp > mx ? rp : mx;
+            const uint32_t j = field(snbp, l);
+            const bool con = col != 0 && j != 0xFFFFu;
+            bounded &= col != 0 || j == 0xFFFFu;
+            const uint32_t ori = nibble(nbw(l), p);
+            const uint32_t rp = cnt[l * ncol + col] - facing(ori, pc_p, l, col);
+            if (con) {
+                invalid |= rp == 0;
+                nbp &= ~(0xFFFFull << (16 * l)) | ((uint64_t)j << (16 * l));
+                csp |= col << (8 * l);
+                rsum += rp;
+                nb_pc_minus += field(pcnp, l) - (uint32_t)__popc(ori);
+                if (score) {
+                    mn = rp < mn ? rp : mn;
+                    mx = rp > mx ? rp : mx;
+                }
             }
         }
-        auto nbs = [&](uint32_t l) { return (uint32_t)(nbp >> (16 * l)) & 0xFFFFu; };
+        if (invalid) return false;
+        auto nbs = [&](uint32_t l) { return field(nbp, l); };
         // Other free slots: invalid when piece p is all one of them holds.
-        uint32_t own_single = sp[i] == p;
-        for (uint32_t l = 0; l < 4; ++l) {
-            const uint32_t j = nbs(l);
-            own_single += j != 0xFFFFu && sp[j] == p;
-        }
+        uint32_t own_single = sp_i == p;
+#pragma unroll
+        for (uint32_t l = 0; l < 4; ++l) own_single += nbs(l) != 0xFFFFu && field(spnp, l) == p;
         if (single[p] > own_single) return false;
         // Options piece q loses by the move: its orientations in slot i and
         // those the colour constraints remove at the neighbours.
@@ -501,7 +520,7 @@ struct Wave {
             mn = pmn < mn ? pmn : mn;
             mx = pmx > mx ? pmx : mx;
             const uint32_t cnt_ip = nib(M[i * a.nw + pw], p);
-            total = 2 * (PC - T[p] - pc[i] + cnt_ip - nb_pc_minus) + 2 * rsum;
+            total = 2 * (PC - T[p] - pc_i + cnt_ip - nb_pc_minus) + 2 * rsum;
             minmax = (mx << 16) + mn;
         }
         return true;
@@ -545,9 +564,16 @@ struct Wave {
             snb[lane] = (inside && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
         }
         __syncthreads();
-        snbp = 0;
+        snbp = pcnp = spnp = 0;
 #pragma unroll
-        for (uint32_t l = 0; l < 4; ++l) snbp |= (uint64_t)(snb[l] & 0xFFFFu) << (16 * l);
+        for (uint32_t l = 0; l < 4; ++l) {
+            const uint32_t j = snb[l];
+            snbp |= (uint64_t)(j & 0xFFFFu) << (16 * l);
+            pcnp |= (uint64_t)(j != 0xFFFFFFFFu ? pc[j] : 0u) << (16 * l);
+            spnp |= (uint64_t)(j != 0xFFFFFFFFu ? sp[j] : 0xFFFFu) << (16 * l);
+        }
+        pc_i = pc[i];
+        sp_i = sp[i];
         for (uint32_t e = lane; e < 4 * nc; e += kWave) {
             const uint32_t l = e / nc, c = e % nc, j = snb[l];
             uint32_t v = 0;
@@ -634,6 +660,8 @@ struct Wave {
                 slot_context(i);
                 for (uint32_t w = 0; w < a.nw; ++w) {
                     const uint64_t v = M[i * a.nw + w];
+                    if (!v) continue;
+                    neighbour_words(w);
                     if (!((v >> lane) & 1)) continue;
                     const uint32_t bit = w * 64 + lane;
                     uint32_t t, mm;
@@ -667,6 +695,8 @@ struct Wave {
             slot_context(j, true);
             for (uint32_t w = 0; w < a.nw; ++w) {
                 const uint64_t v = M[j * a.nw + w];
+                if (!v) continue;
+                neighbour_words(w);
                 if (!((v >> lane) & 1)) continue;
                 const uint32_t bit = w * 64 + lane;
                 uint32_t total = 0, mm = 0;
