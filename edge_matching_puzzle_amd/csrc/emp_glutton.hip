@@ -173,7 +173,7 @@ struct Wave {
     uint32_t *pcol;     // per piece N | E << 8 | S << 16 | W << 24
     uint64_t *avail;
     uint32_t *scratch;
-    uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q), value | slot << 16
+    uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q) (stats())
     uint16_t *cnt;      // slot context: [side l][colour c] options of the side-l neighbour matching c
     uint16_t *risk;     // slot context: critical pieces whose options all lie in the slot and its neighbours
     uint32_t *snb;      // slot context: free neighbour slot on each side (0xFFFFFFFF: none)
@@ -326,16 +326,30 @@ struct Wave {
         const bool any_empty = __ballot(empty) != 0;
         __syncthreads();
         if (extremes) {
+            // Per piece q: the min of pc_j - cnt_j(q) over free slots j, the
+            // slot s holding it, and the min over the free slots farther
+            // than 2 steps (Manhattan) from s; likewise for the max.  Packed
+            // value | s << 11 | far value << 19 (far min 0x7FF: none).
             for (uint32_t q = lane; q < a.P; q += kWave) {
-                uint32_t mnv = 0xFFFFu, mns = 0, mxv = 0, mxs = 0;
+                uint32_t mnv = 0x7FFu, mns = 0, mxv = 0, mxs = 0;
                 const uint32_t w = q >> 4;
                 for (uint32_t j = 0; j < nn; ++j) {
                     const uint32_t v = pc[j] - nib(M[j * a.nw + w], q);
                     if (v < mnv) { mnv = v; mns = j; }
                     if (v > mxv || j == 0) { mxv = v; mxs = j; }
                 }
-                lo[q] = mnv | mns << 16;
-                hi[q] = mxv | mxs << 16;
+                const uint32_t pl = pos_of_slot[mns], ph = pos_of_slot[mxs];
+                const int xl = (int)(pl % a.W), yl = (int)(pl / a.W), xh = (int)(ph % a.W), yh = (int)(ph / a.W);
+                uint32_t mn2 = 0x7FFu, mx2 = 0;
+                for (uint32_t j = 0; j < nn; ++j) {
+                    const uint32_t pos = pos_of_slot[j];
+                    const int x = (int)(pos % a.W), y = (int)(pos / a.W);
+                    const uint32_t v = pc[j] - nib(M[j * a.nw + w], q);
+                    if (abs(x - xl) + abs(y - yl) > 2) mn2 = v < mn2 ? v : mn2;
+                    if (abs(x - xh) + abs(y - yh) > 2) mx2 = v > mx2 ? v : mx2;
+                }
+                lo[q] = mnv | mns << 11 | mn2 << 19;
+                hi[q] = mxv | mxs << 11 | mx2 << 19;
             }
             __syncthreads();
         }
@@ -413,18 +427,34 @@ struct Wave {
             auto excluded = [&](uint32_t j) {
                 return j == i || j == nbs(0) || j == nbs(1) || j == nbs(2) || j == nbs(3);
             };
-            const uint32_t l = lo[p], h = hi[p];
-            if (!excluded(l >> 16) && !excluded(h >> 16)) {
-                if (nn > 1) {
-                    mn = (l & 0xFFFF) < mn ? (l & 0xFFFF) : mn;
-                    mx = (h & 0xFFFF) > mx ? (h & 0xFFFF) : mx;
+            // When the extreme's slot s is excluded, every excluded slot lies
+            // within 2 steps of s: the far value covers the slots beyond, the
+            // 13 cells around s are visited.
+            auto near = [&](uint32_t s, auto fold) {
+                const uint32_t ps = pos_of_slot[s];
+                const int xs = (int)(ps % a.W), ys = (int)(ps / a.W), H = (int)(a.P / a.W);
+                for (int dy = -2; dy <= 2; ++dy)
+                    for (int dx = -2 + abs(dy); dx <= 2 - abs(dy); ++dx) {
+                        const int x = xs + dx, y = ys + dy;
+                        if (x < 0 || x >= (int)a.W || y < 0 || y >= H) continue;
+                        const uint32_t j = slot_of_pos[y * (int)a.W + x];
+                        if (j >= nn || excluded(j)) continue;
+                        fold(pc[j] - nib(M[j * a.nw + pw], p));
+                    }
+            };
+            if (nn > 1) {
+                const uint32_t l = lo[p], h = hi[p];
+                if (!excluded((l >> 11) & 0xFF)) {
+                    mn = (l & 0x7FF) < mn ? (l & 0x7FF) : mn;
+                } else {
+                    if ((l >> 19) != 0x7FFu) mn = (l >> 19) < mn ? (l >> 19) : mn;
+                    near((l >> 11) & 0xFF, [&](uint32_t v) { mn = v < mn ? v : mn; });
                 }
-            } else {
-                for (uint32_t j = 0; j < nn; ++j) {
-                    if (excluded(j)) continue;
-                    const uint32_t v = pc[j] - nib(M[j * a.nw + pw], p);
-                    mn = v < mn ? v : mn;
-                    mx = v > mx ? v : mx;
+                if (!excluded((h >> 11) & 0xFF)) {
+                    mx = (h & 0x7FF) > mx ? (h & 0x7FF) : mx;
+                } else {
+                    mx = (h >> 19) > mx ? (h >> 19) : mx;
+                    near((h >> 11) & 0xFF, [&](uint32_t v) { mx = v > mx ? v : mx; });
                 }
             }
         }
