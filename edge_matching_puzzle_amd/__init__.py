@@ -97,6 +97,7 @@ SIGNATURES = {
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
     "emp_hip_stream_copy": (_I32, [_P, _U64, _P]),
+    "emp_hip_peaks": (_I32, [_P, _P, _P]),
     "emp_hip_mlp_train": (_I32, [_P, _P, _P, _U64, _P, _P, _P]),
     "emp_hip_mlp_score": (_I32, [_P, _P, _P, _U64, _P]),
     "emp_hip_mlp_gradients": (_I32, [_P, _P, _P, _P, _U64, _P, _P]),
@@ -231,6 +232,13 @@ class Context:
         ms = ctypes.c_double()
         _check(lib().emp_hip_stream_copy(self._ctx, nbytes, ctypes.byref(ms)))
         return ms.value
+
+    def peaks(self):
+        """Measured LDS read bandwidth (GB/s) and integer VALU throughput
+        (G lane-ops/s) of the device."""
+        lds, valu = ctypes.c_double(), ctypes.c_double()
+        _check(lib().emp_hip_peaks(self._ctx, ctypes.byref(lds), ctypes.byref(valu)))
+        return {"lds_gbs": lds.value, "valu_gops": valu.value}
 
     def mlp_train(self, features, targets, params, epochs=4, batch=1024, lr=1e-3, seed=1):
         """Adam on the scorer's MSE; returns (params, loss per epoch)."""

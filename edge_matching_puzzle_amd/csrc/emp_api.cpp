@@ -460,6 +460,88 @@ __global__ void stream_copy_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
 
 }  // namespace
 
+namespace {
+
+// LDS read stream: each lane reads 16 B per ds_read_b128, a wave a
+// contiguous 1 KiB of a 32 KiB block buffer (no bank conflicts, constant
+// offsets from a base that moves every iteration); the XOR of everything read
+// decides one store, so no read can be dropped.
+constexpr uint32_t kLdsWords = 2048;   // uint4 entries
+__global__ void __launch_bounds__(256) lds_stream_kernel(uint32_t iters, uint32_t *sink) {
+    __shared__ uint4 buf[kLdsWords];
+    for (uint32_t i = threadIdx.x; i < kLdsWords; i += blockDim.x) buf[i] = make_uint4(i, 3 * i, 5 * i, 7 * i);
+    __syncthreads();
+    uint32_t a0 = 0, a1 = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+        // index <= 1023 + 7 * 128 = 1919 < kLdsWords; the base moves every
+        // iteration, so no read is loop-invariant
+        const uint4 *base = buf + ((threadIdx.x + it * 64) & (kLdsWords / 2 - 1));
+#pragma unroll
+        for (uint32_t r = 0; r < 8; ++r) {
+            const uint4 v = base[r * 128];
+            a0 ^= v.x ^ v.y;
+            a1 ^= v.z ^ v.w;
+        }
+    }
+    if ((a0 ^ a1) == 0x9E3779B9u) sink[0] = threadIdx.x;
+}
+
+// Integer VALU stream: 8 independent chains of x = popcount(x & m) + x
+// (v_and_b32 and v_bcnt_u32_b32: 2 lane-ops per step).
+__global__ void __launch_bounds__(256) valu_stream_kernel(uint32_t iters, uint32_t seed, uint32_t *sink) {
+    const uint32_t m = seed * 0x9E3779B9u | 1u;
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = seed ^ (threadIdx.x * 0x85EBCA6Bu + (uint32_t)j);
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = (uint32_t)__popc(x[j] & m) + x[j];
+    }
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v ^= x[j];
+    if (v == 0x9E3779B9u) sink[0] = threadIdx.x;
+}
+
+}  // namespace
+
+int emp_hip_peaks(emp_hip_context *context, double *lds_gbs, double *valu_gops) {
+    return guarded([&]() -> int {
+        if (!context || !lds_gbs || !valu_gops) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        uint32_t *sink = context->workspace.get<uint32_t>("peak_sink", 1);
+        const uint32_t blocks = context->dev.compute_units * 8, threads = 256;
+        const uint32_t lds_iters = 4096, valu_iters = 4096;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail(EMP_HIP_DEVICE_ERROR, "hipEventCreate failed");
+        float t_lds = 0, t_valu = 0;
+        hipLaunchKernelGGL(lds_stream_kernel, dim3(blocks), dim3(threads), 0, context->stream, lds_iters, sink);
+        (void)hipEventRecord(e0, context->stream);
+        hipLaunchKernelGGL(lds_stream_kernel, dim3(blocks), dim3(threads), 0, context->stream, lds_iters, sink);
+        (void)hipEventRecord(e1, context->stream);
+        hipError_t err = hipStreamSynchronize(context->stream);
+        (void)hipEventElapsedTime(&t_lds, e0, e1);
+        if (err == hipSuccess) {
+            hipLaunchKernelGGL(valu_stream_kernel, dim3(blocks), dim3(threads), 0, context->stream, valu_iters, 7u, sink);
+            (void)hipEventRecord(e0, context->stream);
+            hipLaunchKernelGGL(valu_stream_kernel, dim3(blocks), dim3(threads), 0, context->stream, valu_iters, 7u, sink);
+            (void)hipEventRecord(e1, context->stream);
+            err = hipStreamSynchronize(context->stream);
+            (void)hipEventElapsedTime(&t_valu, e0, e1);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (err != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, hipGetErrorString(err));
+        const double n = (double)blocks * threads;
+        *lds_gbs = n * lds_iters * 8.0 * 16.0 / (t_lds * 1e6);
+        *valu_gops = n * valu_iters * 4.0 * 8.0 * 2.0 / (t_valu * 1e6);
+        return EMP_HIP_OK;
+    });
+}
+
 int emp_hip_stream_copy(emp_hip_context *context, uint64_t bytes, double *ms) {
     return guarded([&]() -> int {
         if (!context || !bytes || bytes % 16) return fail(EMP_HIP_BAD_ARGUMENT, "bytes must be a positive multiple of 16");

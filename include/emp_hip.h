@@ -269,6 +269,13 @@ int emp_hip_probe(emp_hip_context *context, const emp_hip_puzzle *puzzle, const 
  * kernel give the scale of the same counters on the search kernels. */
 int emp_hip_stream_copy(emp_hip_context *context, uint64_t bytes, double *ms);
 
+/* Measured peaks of the two resources the search kernels are bound by: LDS
+ * read bandwidth (GB/s, a conflict-free ds_read_b128 stream on every CU) and
+ * 32-bit integer VALU throughput (G lane-ops/s, independent v_and_b32 /
+ * v_bcnt_u32_b32 chains), each over a second timed launch.  A measurement
+ * utility with no counterpart in the reference, for the roofline fractions. */
+int emp_hip_peaks(emp_hip_context *context, double *lds_gbs, double *valu_gops);
+
 /* ------------------------------------------------------------------------
  * Candidate scorer (no counterpart in the reference: it only orders the
  * candidates of the search and never removes one).  A 126-256-256-1
