@@ -66,7 +66,8 @@ struct GlArgs {
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
 struct GlLayout {
-    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, risk, pcol, avail, scratch, cnt, snb, lo, hi, total;
+    uint32_t masks, pos_of_slot, slot_of_pos, pc, sp, T, single, crit, risk, pcol, avail, scratch, sets, cnt, snb, lo, hi,
+        total;
     __host__ __device__ static GlLayout make(uint32_t S, uint32_t P, uint32_t nw, uint32_t max_colour) {
         GlLayout l;
         uint32_t o = 0;
@@ -82,7 +83,8 @@ struct GlLayout {
         l.risk = take(256 * 2);
         l.pcol = take(P * 4);
         l.avail = take(4 * 8);
-        l.scratch = take(128);
+        l.scratch = take(64);
+        l.sets = take(6 * 32);
         l.cnt = take(4 * (max_colour + 1) * 2);
         l.snb = take(4 * 4);
         l.lo = take(256 * 4);
@@ -173,6 +175,11 @@ struct Wave {
     uint32_t *pcol;     // per piece N | E << 8 | S << 16 | W << 24
     uint64_t *avail;
     uint32_t *scratch;
+    // 256-bit sets: slots changed in the previous / current removal round,
+    // pieces that lost options in the previous / current round, single
+    // pieces of the slots changed in the previous round, critical pieces
+    uint64_t *ch_prev, *ch_cur, *d_prev, *d_cur, *e_single, *crit_set;
+    bool zero_piece;    // an available piece without option (stats())
     uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q) (stats())
     uint16_t *cnt;      // slot context: [side l][colour c] options of the side-l neighbour matching c
     uint16_t *risk;     // slot context: critical pieces whose options all lie in the slot and its neighbours
@@ -214,6 +221,12 @@ struct Wave {
         played = a.played + (size_t)wave_id * a.S;
         avail = reinterpret_cast<uint64_t *>(lds + L.avail);
         scratch = reinterpret_cast<uint32_t *>(lds + L.scratch);
+        ch_prev = reinterpret_cast<uint64_t *>(lds + L.sets);
+        ch_cur = ch_prev + 4;
+        d_prev = ch_prev + 8;
+        d_cur = ch_prev + 12;
+        e_single = ch_prev + 16;
+        crit_set = ch_prev + 20;
         lo = reinterpret_cast<uint32_t *>(lds + L.lo);
         cnt = reinterpret_cast<uint16_t *>(lds + L.cnt);
         risk = reinterpret_cast<uint16_t *>(lds + L.risk);
@@ -307,6 +320,7 @@ struct Wave {
         // options: a move removes at most 4 options from each of 5 cells, so
         // only those pieces can be left without option.
         uint32_t n_crit = 0;
+        bool zero = false;
         for (uint32_t base = 0; base < a.P; base += kWave) {
             const uint32_t q = base + lane;
             uint32_t t = 0;
@@ -319,7 +333,12 @@ struct Wave {
             const uint64_t m = __ballot(c);
             if (c) crit[n_crit + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = (uint16_t)q;
             n_crit += (uint32_t)__popcll(m);
+            if (lane == 0) crit_set[base / kWave] = m;
+            zero |= __ballot(c && t == 0) != 0;
         }
+        for (uint32_t base = a.P & ~(uint32_t)(kWave - 1); base < 256; base += kWave)
+            if (lane == 0 && base >= a.P) crit_set[base / kWave] = 0;
+        zero_piece = zero;
         if (lane == 0) scratch[0] = n_crit;
         for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
         PC = local;
@@ -677,16 +696,85 @@ struct Wave {
         }
     }
 
+    // Whether slot i may hold a candidate made invalid by the changes of
+    // the previous round (and of this one so far), given that every
+    // candidate of slot i was valid when it was last evaluated.  A
+    // candidate's validity reads the masks of slot i and its free
+    // neighbours (colour counts, single pieces, at-risk pieces), single[]
+    // and the totals T of critical pieces; so slot i is affected only when
+    // it or a free neighbour lost options, when a critical piece that lost
+    // options elsewhere has options there (it may now be at risk), or when
+    // a slot changed into holding only a piece that slot i holds.
+    __device__ bool affected(uint32_t i) {
+        const uint32_t pos = pos_of_slot[i], x = pos % a.W;
+        auto changed = [&](uint32_t j) { return ((ch_prev[j >> 6] | ch_cur[j >> 6]) >> (j & 63)) & 1; };
+        if (changed(i)) return true;
+        uint32_t nb[4];
+#pragma unroll
+        for (uint32_t l = 0; l < 4; ++l) {
+            const bool inside = l == 0 ? pos >= a.W : (l == 1 ? x + 1 < a.W : (l == 2 ? pos + a.W < a.P : x > 0));
+            const uint32_t q = (uint32_t)((int)pos + off(l));
+            nb[l] = (inside && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
+            if (nb[l] != 0xFFFFFFFFu && changed(nb[l])) return true;
+        }
+        // pieces: lane w takes mask word w (16 pieces); a piece bit spread
+        // to its 4 orientation bits
+        auto spread = [](uint64_t v16) {
+            uint64_t v = v16 & 0xFFFF;
+            v = (v | (v << 24)) & 0x000000FF000000FFull;
+            v = (v | (v << 12)) & 0x000F000F000F000Full;
+            v = (v | (v << 6)) & 0x0303030303030303ull;
+            v = (v | (v << 3)) & 0x1111111111111111ull;
+            return v * 0xF;
+        };
+        bool hit = false;
+        for (uint32_t w = lane; w < a.nw; w += kWave) {
+            const uint32_t sh = 16 * (w & 3);
+            const uint64_t dm = spread(((d_prev[w >> 2] & crit_set[w >> 2]) >> sh));
+            const uint64_t em = spread(e_single[w >> 2] >> sh);
+            const uint64_t own = M[i * a.nw + w];
+            uint64_t near = own;
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l)
+                if (nb[l] != 0xFFFFFFFFu) near |= M[nb[l] * a.nw + w];
+            hit |= (near & dm) != 0 || (own & em) != 0;
+        }
+        return __ballot(hit) != 0;
+    }
+
     // remove_invalid_transitions: false when a slot runs empty.  Slot by
-    // slot; lane b takes bit b of each word of the slot's mask.
+    // slot; lane b takes bit b of each word of the slot's mask.  The first
+    // round evaluates every slot; later rounds only the slots the changes
+    // since their last evaluation can affect (affected()), so the round
+    // that ends the loop still finds every candidate valid on the final
+    // masks.
     __device__ bool fixpoint() {
         uint64_t evaluated = 0;
+        bool first = true;
         while (true) {
             uint32_t PC;
             if (!stats(PC)) return false;
-            bool removed = false;
             const uint32_t nn = n();
+            // round start: the current round's changes become the previous
+            // round's; single pieces of the slots they touched
+            if (lane < 4) {
+                ch_prev[lane] = ch_cur[lane];
+                ch_cur[lane] = 0;
+                d_prev[lane] = d_cur[lane];
+                d_cur[lane] = 0;
+                e_single[lane] = 0;
+            }
+            __syncthreads();
+            if (!first)
+                for (uint32_t j = lane; j < nn; j += kWave)
+                    if (((ch_prev[j >> 6] >> (j & 63)) & 1) && sp[j] != 0xFFFF)
+                        atomicOr(reinterpret_cast<unsigned long long *>(&e_single[sp[j] >> 6]), 1ull << (sp[j] & 63));
+            __syncthreads();
+            // a piece already without option makes every candidate invalid
+            const bool all = first || zero_piece;
+            bool removed = false;
             for (uint32_t i = 0; i < nn; ++i) {
+                if (!all && !affected(i)) continue;
                 slot_context(i);
                 for (uint32_t w = 0; w < a.nw; ++w) {
                     const uint64_t v = M[i * a.nw + w];
@@ -698,11 +786,15 @@ struct Wave {
                     ++evaluated;
                     if (!evaluate(i, bit >> 2, bit & 3, false, PC, t, mm)) {
                         atomicAnd(reinterpret_cast<unsigned long long *>(&M[i * a.nw + w]), ~(1ull << lane));
+                        atomicOr(reinterpret_cast<unsigned long long *>(&ch_cur[i >> 6]), 1ull << (i & 63));
+                        const uint32_t pp = bit >> 2;
+                        atomicOr(reinterpret_cast<unsigned long long *>(&d_cur[pp >> 6]), 1ull << (pp & 63));
                         removed = true;
                     }
                 }
                 __syncthreads();
             }
+            first = false;
             if (lane == 0) atomicAdd(&a.evals[2], 1ull);
             if (!__ballot(removed)) break;
         }
