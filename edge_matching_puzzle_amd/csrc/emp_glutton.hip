@@ -375,148 +375,173 @@ struct Wave {
         return !any_empty;
     }
 
-    // Evaluates candidate (slot i, piece p, orientation o) against the live
-    // masks.  Returns validity; with score set also total and min / max.
-    __device__ bool evaluate(uint32_t i, uint32_t p, uint32_t o, bool score, uint32_t PC,
-                             uint32_t &total, uint32_t &minmax) {
-        const uint32_t nn = n();
-        const uint32_t pc_p = pcol[p];
-        const uint32_t pw = p >> 4;
-        // Constrained neighbours packed in registers: slot of side l in bits
-        // 16l..16l+15 (0xFFFF: none), colour the candidate shows toward it in
-        // bits 8l..8l+7.
-        uint64_t nbp = ~0ull;
-        uint32_t csp = 0;
-        uint32_t rsum = 0, nb_pc_minus = 0;
-        uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    // The neighbour part of candidate (slot i, piece p, orientation o).
+    struct Cand {
+        // constrained neighbours: slot of side l in bits 16l..16l+15 (0xFFFF:
+        // none); colour the candidate shows toward it in bits 8l..8l+7, and
+        // that colour repeated in the 4 bytes of b[l] (0: none)
+        uint64_t nbp;
+        uint32_t csp, b0, b1, b2, b3;
+        uint32_t rsum, nb_pc_minus;   // sum of r_l; sum of pc_jl - cnt_jl(p)
+        uint32_t mn, mx;              // extremes of r_l
         // every free neighbour constrained (the upper bounds of the slot
         // context assume it; a frame colour only ever faces the frame)
-        bool bounded = true;
-        // Colour-constrained free neighbours (apply_color_constraints :528-612):
-        // the options of the neighbour matching the candidate's side, from
-        // the slot context, less those of piece p itself.  (A coloured side
-        // never faces the frame, so the neighbour lookup needs no edge test.)
-        // Unrolled and branch-free: the four cnt reads go out together.
+        bool bounded;
+    };
+
+    // Colour-constrained free neighbours (apply_color_constraints :528-612):
+    // the options of the neighbour matching the candidate's side, from the
+    // slot context, less those of piece p itself; then the other slots holding
+    // only piece p.  False when either leaves a slot without option.
+    // Unrolled and branch-free: the four cnt reads go out together.
+    __device__ bool neighbours(uint32_t i, uint32_t p, uint32_t o, Cand &cd) {
+        const uint32_t pc_p = pcol[p];
         const uint32_t ncol = a.max_colour + 1;
+        cd.nbp = ~0ull;
+        cd.csp = 0;
+        cd.rsum = cd.nb_pc_minus = 0;
+        cd.mn = 0xFFFFFFFFu;
+        cd.mx = 0;
+        cd.bounded = true;
         bool invalid = false;
 #pragma unroll
         for (uint32_t l = 0; l < 4; ++l) {
             const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
             const uint32_t j = field(snbp, l);
             const bool con = col != 0 && j != 0xFFFFu;
-            bounded &= col != 0 || j == 0xFFFFu;
+            cd.bounded &= col != 0 || j == 0xFFFFu;
             const uint32_t ori = nibble(nbw(l), p);
             const uint32_t rp = cnt[l * ncol + col] - facing(ori, pc_p, l, col);
             if (con) {
                 invalid |= rp == 0;
-                nbp &= ~(0xFFFFull << (16 * l)) | ((uint64_t)j << (16 * l));
-                csp |= col << (8 * l);
-                rsum += rp;
-                nb_pc_minus += field(pcnp, l) - (uint32_t)__popc(ori);
-                if (score) {
-                    mn = rp < mn ? rp : mn;
-                    mx = rp > mx ? rp : mx;
-                }
+                cd.nbp &= ~(0xFFFFull << (16 * l)) | ((uint64_t)j << (16 * l));
+                cd.csp |= col << (8 * l);
+                cd.rsum += rp;
+                cd.nb_pc_minus += field(pcnp, l) - (uint32_t)__popc(ori);
+                cd.mn = rp < cd.mn ? rp : cd.mn;
+                cd.mx = rp > cd.mx ? rp : cd.mx;
             }
         }
+        cd.b0 = (cd.csp & 0xFF) * 0x01010101u;
+        cd.b1 = ((cd.csp >> 8) & 0xFF) * 0x01010101u;
+        cd.b2 = ((cd.csp >> 16) & 0xFF) * 0x01010101u;
+        cd.b3 = (cd.csp >> 24) * 0x01010101u;
         if (invalid) return false;
-        auto nbs = [&](uint32_t l) { return field(nbp, l); };
         // Other free slots: invalid when piece p is all one of them holds.
         uint32_t own_single = sp_i == p;
 #pragma unroll
-        for (uint32_t l = 0; l < 4; ++l) own_single += nbs(l) != 0xFFFFu && field(spnp, l) == p;
-        if (single[p] > own_single) return false;
-        // Options piece q loses by the move: its orientations in slot i and
-        // those the colour constraints remove at the neighbours.
-        auto lost = [&](uint32_t q) {
-            const uint32_t w = q >> 4;
-            uint32_t d = nib(M[i * a.nw + w], q);
-            const uint32_t qc = pcol[q];
-            for (uint32_t l = 0; l < 4; ++l) {
-                const uint32_t j = nbs(l);
-                if (j == 0xFFFFu) continue;
-                const uint32_t ori = nibble(M[j * a.nw + w], q);
-                d += (uint32_t)__popc(ori) - facing(ori, qc, l, (csp >> (8 * l)) & 0xFF);
-            }
-            return d;
-        };
-        if (score) {
-            // Free slots other than i and the constrained neighbours: the
-            // per-piece extremes, unless they sit in one of those slots.
-            auto excluded = [&](uint32_t j) {
-                return j == i || j == nbs(0) || j == nbs(1) || j == nbs(2) || j == nbs(3);
-            };
-            // When the extreme's slot s is excluded, every excluded slot lies
-            // within 2 steps of s: the far value covers the slots beyond, the
-            // 13 cells around s are visited.
-            auto near = [&](uint32_t s, auto fold) {
-                const uint32_t ps = pos_of_slot[s];
-                const int xs = (int)(ps % a.W), ys = (int)(ps / a.W), H = (int)(a.P / a.W);
-                for (int dy = -2; dy <= 2; ++dy)
-                    for (int dx = -2 + abs(dy); dx <= 2 - abs(dy); ++dx) {
-                        const int x = xs + dx, y = ys + dy;
-                        if (x < 0 || x >= (int)a.W || y < 0 || y >= H) continue;
-                        const uint32_t j = slot_of_pos[y * (int)a.W + x];
-                        if (j >= nn || excluded(j)) continue;
-                        fold(pc[j] - nib(M[j * a.nw + pw], p));
-                    }
-            };
-            if (nn > 1) {
-                const uint32_t l = lo[p], h = hi[p];
-                if (!excluded((l >> 11) & 0xFF)) {
-                    mn = (l & 0x7FF) < mn ? (l & 0x7FF) : mn;
-                } else {
-                    if ((l >> 19) != 0x7FFu) mn = (l >> 19) < mn ? (l >> 19) : mn;
-                    near((l >> 11) & 0xFF, [&](uint32_t v) { mn = v < mn ? v : mn; });
-                }
-                if (!excluded((h >> 11) & 0xFF)) {
-                    mx = (h & 0x7FF) > mx ? (h & 0x7FF) : mx;
-                } else {
-                    mx = (h >> 19) > mx ? (h >> 19) : mx;
-                    near((h >> 11) & 0xFF, [&](uint32_t v) { mx = v > mx ? v : mx; });
-                }
-            }
+        for (uint32_t l = 0; l < 4; ++l) own_single += field(cd.nbp, l) != 0xFFFFu && field(spnp, l) == p;
+        return single[p] <= own_single;
+    }
+
+    // Options piece q loses by the move: its orientations in slot i and
+    // those the colour constraints remove at the neighbours.
+    __device__ uint32_t lost(uint32_t i, const Cand &cd, uint32_t q) const {
+        const uint32_t w = q >> 4;
+        uint32_t d = nib(M[i * a.nw + w], q);
+        const uint32_t qc = pcol[q];
+        for (uint32_t l = 0; l < 4; ++l) {
+            const uint32_t j = field(cd.nbp, l);
+            if (j == 0xFFFFu) continue;
+            const uint32_t ori = nibble(M[j * a.nw + w], q);
+            d += (uint32_t)__popc(ori) - facing(ori, qc, l, (cd.csp >> (8 * l)) & 0xFF);
         }
-        // With every free neighbour constrained, T(q) - lost(q) = lb(q) + the
-        // options of q at the neighbours that show the candidate's colour
-        // toward slot i (byte compares against the listed facing colours).
-        const uint32_t b0 = (csp & 0xFF) * 0x01010101u, b1 = ((csp >> 8) & 0xFF) * 0x01010101u;
-        const uint32_t b2 = ((csp >> 16) & 0xFF) * 0x01010101u, b3 = (csp >> 24) * 0x01010101u;
-        auto matches = [&](uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
-            return (b0 ? zero_bytes(f0 ^ b0) : 0u) + (b1 ? zero_bytes(f1 ^ b1) : 0u) +
-                   (b2 ? zero_bytes(f2 ^ b2) : 0u) + (b3 ? zero_bytes(f3 ^ b3) : 0u);
-        };
-        if (!score) {
-            // Only the critical pieces at risk in this slot (lb = 0) can be
-            // left without option (slot context).
-            const uint32_t nc = scratch[1];
-            uint32_t k = 0;
-            if (bounded)
-                for (; k < nc && k < kWave; ++k) {
-                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)rk_q, (int)k);
-                    if (q == p) continue;
-                    if (!matches(__builtin_amdgcn_readlane((int)rk_f0, (int)k),
-                                 __builtin_amdgcn_readlane((int)rk_f1, (int)k),
-                                 __builtin_amdgcn_readlane((int)rk_f2, (int)k),
-                                 __builtin_amdgcn_readlane((int)rk_f3, (int)k)))
-                        return false;
-                }
-            for (; k < nc; ++k) {
-                const uint32_t q = risk[k];
+        return d;
+    }
+
+    // With every free neighbour constrained, T(q) - lost(q) = lb(q) + the
+    // options of q at the neighbours that show the candidate's colour toward
+    // slot i (byte compares against the listed facing colours).
+    __device__ static uint32_t matches(const Cand &cd, uint32_t f0, uint32_t f1, uint32_t f2, uint32_t f3) {
+        return (cd.b0 ? zero_bytes(f0 ^ cd.b0) : 0u) + (cd.b1 ? zero_bytes(f1 ^ cd.b1) : 0u) +
+               (cd.b2 ? zero_bytes(f2 ^ cd.b2) : 0u) + (cd.b3 ? zero_bytes(f3 ^ cd.b3) : 0u);
+    }
+
+    // Validity of candidate (slot i, piece p, orientation o) against the live
+    // masks (remove_invalid_transitions).
+    __device__ bool valid(uint32_t i, uint32_t p, uint32_t o) {
+        Cand cd;
+        if (!neighbours(i, p, o, cd)) return false;
+        // Only the critical pieces at risk in this slot (lb = 0) can be left
+        // without option (slot context).
+        const uint32_t nc = scratch[1];
+        uint32_t k = 0;
+        if (cd.bounded)
+            for (; k < nc && k < kWave; ++k) {
+                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)rk_q, (int)k);
                 if (q == p) continue;
-                if ((uint32_t)T[q] == lost(q)) return false;
+                if (!matches(cd, __builtin_amdgcn_readlane((int)rk_f0, (int)k),
+                             __builtin_amdgcn_readlane((int)rk_f1, (int)k),
+                             __builtin_amdgcn_readlane((int)rk_f2, (int)k),
+                             __builtin_amdgcn_readlane((int)rk_f3, (int)k)))
+                    return false;
             }
-            return true;
+        for (; k < nc; ++k) {
+            const uint32_t q = risk[k];
+            if (q == p) continue;
+            if ((uint32_t)T[q] == lost(i, cd, q)) return false;
+        }
+        return true;
+    }
+
+    // Score total of a candidate whose neighbour part is cd (the algebraic
+    // form of get_best_candidate's sum, see the file header).
+    __device__ uint32_t score_total(uint32_t i, uint32_t p, uint32_t PC, const Cand &cd) const {
+        return 2 * (PC - T[p] - pc_i + nib(M[i * a.nw + (p >> 4)], p) - cd.nb_pc_minus) + 2 * cd.rsum;
+    }
+
+    // Min / max of the per-slot and per-piece option counts after the move,
+    // (max << 16) + min; false when the move leaves a piece without option.
+    // Needs the slot's scoring lists (score_lists()).
+    __device__ bool score_minmax(uint32_t i, uint32_t p, const Cand &cd, uint32_t &minmax) {
+        const uint32_t nn = n();
+        const uint32_t pw = p >> 4;
+        uint32_t mn = cd.mn, mx = cd.mx;
+        // Free slots other than i and the constrained neighbours: the
+        // per-piece extremes, unless they sit in one of those slots.
+        auto excluded = [&](uint32_t j) {
+            return j == i || j == field(cd.nbp, 0) || j == field(cd.nbp, 1) || j == field(cd.nbp, 2) ||
+                   j == field(cd.nbp, 3);
+        };
+        // When the extreme's slot s is excluded, every excluded slot lies
+        // within 2 steps of s: the far value covers the slots beyond, the 13
+        // cells around s are visited.
+        auto near = [&](uint32_t s, auto fold) {
+            const uint32_t ps = pos_of_slot[s];
+            const int xs = (int)(ps % a.W), ys = (int)(ps / a.W), H = (int)(a.P / a.W);
+            for (int dy = -2; dy <= 2; ++dy)
+                for (int dx = -2 + abs(dy); dx <= 2 - abs(dy); ++dx) {
+                    const int x = xs + dx, y = ys + dy;
+                    if (x < 0 || x >= (int)a.W || y < 0 || y >= H) continue;
+                    const uint32_t j = slot_of_pos[y * (int)a.W + x];
+                    if (j >= nn || excluded(j)) continue;
+                    fold(pc[j] - nib(M[j * a.nw + pw], p));
+                }
+        };
+        if (nn > 1) {
+            const uint32_t l = lo[p], h = hi[p];
+            if (!excluded((l >> 11) & 0xFF)) {
+                mn = (l & 0x7FF) < mn ? (l & 0x7FF) : mn;
+            } else {
+                if ((l >> 19) != 0x7FFu) mn = (l >> 19) < mn ? (l >> 19) : mn;
+                near((l >> 11) & 0xFF, [&](uint32_t v) { mn = v < mn ? v : mn; });
+            }
+            if (!excluded((h >> 11) & 0xFF)) {
+                mx = (h & 0x7FF) > mx ? (h & 0x7FF) : mx;
+            } else {
+                mx = (h >> 19) > mx ? (h >> 19) : mx;
+                near((h >> 11) & 0xFF, [&](uint32_t v) { mx = v > mx ? v : mx; });
+            }
         }
         // Option counts of the other available pieces after the move: T(q)
         // less what the move takes; their min (0: invalid move) and max.
         // Pieces are visited in the order of the slot context's bounds, until
-        // the next bound cannot improve the extreme; past the 64 listed
-        // pieces (or when a free neighbour is left unconstrained, which the
-        // bounds and the listed colours do not cover), every available piece.
+        // the next bound cannot change the extreme; past the 64 listed pieces
+        // (or when a free neighbour is left unconstrained, which the bounds and
+        // the listed colours do not cover), every available piece.
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
         bool open = true;
-        for (uint32_t r = 0; r < kWave && bounded; ++r) {
+        for (uint32_t r = 0; r < kWave && cd.bounded; ++r) {
             const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_lo, (int)r);
             if (k == 0xFFFFFFFFu || (k >> 8) >= pmn) {
                 open = false;
@@ -524,7 +549,7 @@ struct Wave {
             }
             const uint32_t q = k & 0xFF;
             if (q == p) continue;
-            const uint32_t c = (k >> 8) + matches(__builtin_amdgcn_readlane((int)lo_f0, (int)r),
+            const uint32_t c = (k >> 8) + matches(cd, __builtin_amdgcn_readlane((int)lo_f0, (int)r),
                                                   __builtin_amdgcn_readlane((int)lo_f1, (int)r),
                                                   __builtin_amdgcn_readlane((int)lo_f2, (int)r),
                                                   __builtin_amdgcn_readlane((int)lo_f3, (int)r));
@@ -533,15 +558,15 @@ struct Wave {
         if (open)
             for (uint32_t q = 0; q < a.P; ++q)
                 if (q != p && piece_avail(q)) {
-                    const uint32_t c = (uint32_t)T[q] - lost(q);
+                    const uint32_t c = (uint32_t)T[q] - lost(i, cd, q);
                     pmn = c < pmn ? c : pmn;
                 }
         if (pmn == 0) return false;
         // The max in steps of 4 listed pieces (independent, so their reads
-        // and compares overlap); a piece past the stopping bound cannot
-        // raise the max, so taking whole steps changes nothing.
-        open = !bounded;
-        if (bounded)
+        // and compares overlap); a piece past the stopping bound cannot raise
+        // the max, so taking whole steps changes nothing.
+        open = !cd.bounded;
+        if (cd.bounded)
             for (uint32_t r = 0; r < kWave; r += 4) {
                 uint32_t last = 0;
 #pragma unroll
@@ -550,7 +575,7 @@ struct Wave {
                     last = k;
                     if (k == 0xFFFFFFFFu || (k & 0xFF) == p) continue;
                     const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)(r + e)) +
-                                       matches(__builtin_amdgcn_readlane((int)hi_f0, (int)(r + e)),
+                                       matches(cd, __builtin_amdgcn_readlane((int)hi_f0, (int)(r + e)),
                                                __builtin_amdgcn_readlane((int)hi_f1, (int)(r + e)),
                                                __builtin_amdgcn_readlane((int)hi_f2, (int)(r + e)),
                                                __builtin_amdgcn_readlane((int)hi_f3, (int)(r + e)));
@@ -562,16 +587,12 @@ struct Wave {
         if (open)
             for (uint32_t q = 0; q < a.P; ++q)
                 if (q != p && piece_avail(q)) {
-                    const uint32_t c = (uint32_t)T[q] - lost(q);
+                    const uint32_t c = (uint32_t)T[q] - lost(i, cd, q);
                     pmx = c > pmx ? c : pmx;
                 }
-        if (score) {
-            mn = pmn < mn ? pmn : mn;
-            mx = pmx > mx ? pmx : mx;
-            const uint32_t cnt_ip = nib(M[i * a.nw + pw], p);
-            total = 2 * (PC - T[p] - pc_i + cnt_ip - nb_pc_minus) + 2 * rsum;
-            minmax = (mx << 16) + mn;
-        }
+        mn = pmn < mn ? pmn : mn;
+        mx = pmx > mx ? pmx : mx;
+        minmax = (mx << 16) + mn;
         return true;
     }
 
@@ -632,6 +653,10 @@ struct Wave {
             }
             cnt[e] = (uint16_t)v;
         }
+        if (scoring) {
+            __syncthreads();
+            return;
+        }
         // A candidate of slot i can leave a piece without option only if
         // every option of that piece lies in slot i or its free neighbours.
         const uint32_t n_crit = scratch[0];
@@ -655,45 +680,45 @@ struct Wave {
         }
         if (lane == 0) scratch[1] = n_risk;
         __syncthreads();
-        if (!scoring) {
-            uint32_t lb_unused;
-            rk_q = lane < n_risk ? (uint32_t)risk[lane] : 0xFFFFFFFFu;
-            payload(i, rk_q, lb_unused, rk_f0, rk_f1, rk_f2, rk_f3);
-        }
-        if (scoring) {
-            // Per available piece q: lb = T(q) less its options in slot i and
-            // in the free neighbours (what q keeps after the worst move into
-            // slot i); ub = lb plus, per free neighbour, the most options
-            // there that one colour toward slot i can leave (what q keeps
-            // after the best move).  A move's effect on q lies in [lb, ub].
-            uint32_t kl[4], kh[4];
+        uint32_t lb_unused;
+        rk_q = lane < n_risk ? (uint32_t)risk[lane] : 0xFFFFFFFFu;
+        payload(i, rk_q, lb_unused, rk_f0, rk_f1, rk_f2, rk_f3);
+    }
+
+    // Scoring lists of slot i (after its slot context), for score_minmax().
+    // Per available piece q: lb = T(q) less its options in slot i and in the
+    // free neighbours (what q keeps after the worst move into slot i); ub =
+    // lb plus, per free neighbour, the most options there that one colour
+    // toward slot i can leave (what q keeps after the best move).  A move's
+    // effect on q lies in [lb, ub].  Wave-wide: every lane calls it.
+    __device__ void score_lists(uint32_t i) {
+        uint32_t kl[4], kh[4];
 #pragma unroll
-            for (uint32_t r = 0; r < 4; ++r) {
-                const uint32_t q = r * 64 + lane;
-                kl[r] = kh[r] = 0xFFFFFFFFu;
-                if (q < a.P && piece_avail(q)) {
-                    const uint32_t w = q >> 4, qc = pcol[q];
-                    uint32_t touched = nib(M[i * a.nw + w], q), keep = 0;
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t q = r * 64 + lane;
+            kl[r] = kh[r] = 0xFFFFFFFFu;
+            if (q < a.P && piece_avail(q)) {
+                const uint32_t w = q >> 4, qc = pcol[q];
+                uint32_t touched = nib(M[i * a.nw + w], q), keep = 0;
 #pragma unroll
-                    for (uint32_t l = 0; l < 4; ++l) {
-                        const uint32_t j = snb[l];
-                        if (j == 0xFFFFFFFFu) continue;
-                        const uint32_t ori = nibble(M[j * a.nw + w], q);
-                        touched += (uint32_t)__popc(ori);
-                        keep += most_facing(ori, qc, l);
-                    }
-                    const uint32_t lb = (uint32_t)T[q] - touched;
-                    kl[r] = lb << 8 | q;
-                    kh[r] = (0xFFFFu - (lb + keep)) << 8 | q;
+                for (uint32_t l = 0; l < 4; ++l) {
+                    const uint32_t j = snb[l];
+                    if (j == 0xFFFFFFFFu) continue;
+                    const uint32_t ori = nibble(M[j * a.nw + w], q);
+                    touched += (uint32_t)__popc(ori);
+                    keep += most_facing(ori, qc, l);
                 }
+                const uint32_t lb = (uint32_t)T[q] - touched;
+                kl[r] = lb << 8 | q;
+                kh[r] = (0xFFFFu - (lb + keep)) << 8 | q;
             }
-            key_lo = smallest64(kl, lane);
-            key_hi = smallest64(kh, lane);
-            // payload of each listed piece
-            uint32_t lb_unused;
-            payload(i, key_lo, lb_unused, lo_f0, lo_f1, lo_f2, lo_f3);
-            payload(i, key_hi, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3);
         }
+        key_lo = smallest64(kl, lane);
+        key_hi = smallest64(kh, lane);
+        // payload of each listed piece
+        uint32_t lb_unused;
+        payload(i, key_lo, lb_unused, lo_f0, lo_f1, lo_f2, lo_f3);
+        payload(i, key_hi, hi_lb, hi_f0, hi_f1, hi_f2, hi_f3);
     }
 
     // Whether slot i may hold a candidate made invalid by the changes of
@@ -782,9 +807,8 @@ struct Wave {
                     neighbour_words(w);
                     if (!((v >> lane) & 1)) continue;
                     const uint32_t bit = w * 64 + lane;
-                    uint32_t t, mm;
                     ++evaluated;
-                    if (!evaluate(i, bit >> 2, bit & 3, false, PC, t, mm)) {
+                    if (!valid(i, bit >> 2, bit & 3)) {
                         atomicAnd(reinterpret_cast<unsigned long long *>(&M[i * a.nw + w]), ~(1ull << lane));
                         atomicOr(reinterpret_cast<unsigned long long *>(&ch_cur[i >> 6]), 1ull << (i & 63));
                         const uint32_t pp = bit >> 2;
@@ -803,7 +827,12 @@ struct Wave {
         return stats(PC);
     }
 
-    // get_best_candidate: returns (slot << 16) | bit, or 0xFFFFFFFF.
+    // get_best_candidate: returns (slot << 16) | bit, or 0xFFFFFFFF.  The
+    // order is (total, (max << 16) + min), so the min / max (walks and slot
+    // extremes, and the slot's scoring lists) are only computed for
+    // candidates whose total reaches the best total found so far: a lower
+    // total cannot win whatever its min / max.  At the last level, where the
+    // last valid candidate wins, every candidate is evaluated in full.
     __device__ uint32_t best() {
         uint32_t PC;
         stats(PC, true);
@@ -811,19 +840,37 @@ struct Wave {
         uint64_t best_key = 0;
         uint32_t best_idx = 0xFFFFFFFFu;
         bool have = false;
+        uint32_t wave_total = 0;   // best total of the wave after the previous slots
         uint64_t evaluated = 0;
         const uint32_t nn = n();
         for (uint32_t j = 0; j < nn; ++j) {
             slot_context(j, true);
+            bool lists = false;
             for (uint32_t w = 0; w < a.nw; ++w) {
                 const uint64_t v = M[j * a.nw + w];
                 if (!v) continue;
                 neighbour_words(w);
-                if (!((v >> lane) & 1)) continue;
                 const uint32_t bit = w * 64 + lane;
-                uint32_t total = 0, mm = 0;
-                ++evaluated;
-                if (!evaluate(j, bit >> 2, bit & 3, true, PC, total, mm)) continue;
+                const uint32_t p = bit >> 2;
+                Cand cd;
+                uint32_t total = 0;
+                bool need = false;
+                if ((v >> lane) & 1) {
+                    ++evaluated;
+                    if (neighbours(j, p, bit & 3, cd)) {
+                        total = score_total(j, p, PC, cd);
+                        const uint32_t thr = have && ((uint32_t)(best_key >> 32)) > wave_total
+                                                 ? (uint32_t)(best_key >> 32) : wave_total;
+                        need = last_level || total >= thr;
+                    }
+                }
+                if (!__ballot(need)) continue;
+                if (!lists) {
+                    score_lists(j);
+                    lists = true;
+                }
+                uint32_t mm = 0;
+                if (!need || !score_minmax(j, p, cd, mm)) continue;
                 const uint64_t key = last_level ? 0 : (((uint64_t)total << 32) | mm);
                 const uint32_t idx = (j << 16) | bit;
                 // higher key wins; on equal keys the earlier candidate, except
@@ -837,6 +884,12 @@ struct Wave {
                 }
             }
             __syncthreads();
+            uint32_t t = have ? (uint32_t)(best_key >> 32) : 0u;
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint32_t ot = (uint32_t)__shfl_xor((int)t, o);
+                t = ot > t ? ot : t;
+            }
+            wave_total = t;
         }
         count(evaluated, true);
         for (int o = 32; o > 0; o >>= 1) {
