@@ -833,6 +833,12 @@ struct Wave {
     // candidates whose total reaches the best total found so far: a lower
     // total cannot win whatever its min / max.  At the last level, where the
     // last valid candidate wins, every candidate is evaluated in full.
+    // A candidate of slot j scores at most 2 (PC - pc_j) (the options of the
+    // neighbours and of the piece elsewhere only lower it), so slots whose
+    // bound is below the best total so far are skipped, and the 64 slots
+    // with the fewest options -- the highest bounds -- go first.  The
+    // comparison carries the (slot, bit) tie-break, so the order of the
+    // slots does not change the choice.
     __device__ uint32_t best() {
         uint32_t PC;
         stats(PC, true);
@@ -841,9 +847,29 @@ struct Wave {
         uint32_t best_idx = 0xFFFFFFFFu;
         bool have = false;
         uint32_t wave_total = 0;   // best total of the wave after the previous slots
+        bool have_any = false;     // some lane holds a valid candidate
         uint64_t evaluated = 0;
         const uint32_t nn = n();
-        for (uint32_t j = 0; j < nn; ++j) {
+        uint32_t ks[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint32_t j = r * kWave + lane;
+            ks[r] = j < nn ? ((uint32_t)pc[j] << 8 | j) : 0xFFFFFFFFu;
+        }
+        const uint32_t skey = smallest64(ks, lane);
+        const uint32_t klast = (uint32_t)__builtin_amdgcn_readlane((int)skey, kWave - 1);
+        // steps 0..63: the listed slots; then every other slot in index order
+        for (uint32_t step = 0; step < kWave + nn; ++step) {
+            uint32_t j;
+            if (step < kWave) {
+                const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)skey, (int)step);
+                if (k == 0xFFFFFFFFu) continue;
+                j = k & 0xFF;
+            } else {
+                j = step - kWave;
+                if (klast == 0xFFFFFFFFu || ((uint32_t)pc[j] << 8 | j) <= klast) continue;
+            }
+            if (!last_level && have_any && 2 * (PC - pc[j]) < wave_total) continue;
             slot_context(j, true);
             bool lists = false;
             for (uint32_t w = 0; w < a.nw; ++w) {
@@ -890,6 +916,7 @@ struct Wave {
                 t = ot > t ? ot : t;
             }
             wave_total = t;
+            have_any = __ballot(have) != 0;
         }
         count(evaluated, true);
         for (int o = 32; o > 0; o >>= 1) {
