@@ -644,12 +644,23 @@ struct Wave {
         }
         pc_i = pc[i];
         sp_i = sp[i];
+        // cnt[l][c] = popcount(M[j_l] & cm[c][l]); the 16 words (nw <= 16)
+        // are read without a guard (index clamped, extra words masked) so
+        // all the global reads of a lane are in flight together.
         for (uint32_t e = lane; e < 4 * nc; e += kWave) {
             const uint32_t l = e / nc, c = e % nc, j = snb[l];
             uint32_t v = 0;
             if (c && j != 0xFFFFFFFFu) {
                 const uint64_t *cm = a.cm + ((size_t)(c - 1) * 4 + l) * a.nw;
-                for (uint32_t w = 0; w < a.nw; ++w) v += (uint32_t)__popcll(M[j * a.nw + w] & cm[w]);
+                const uint64_t *mj = M + j * a.nw;
+                uint64_t x[16];
+#pragma unroll
+                for (uint32_t w = 0; w < 16; ++w) {
+                    const uint32_t ww = w < a.nw ? w : 0;
+                    x[w] = cm[ww] & mj[ww];
+                }
+#pragma unroll
+                for (uint32_t w = 0; w < 16; ++w) v += w < a.nw ? (uint32_t)__popcll(x[w]) : 0u;
             }
             cnt[e] = (uint16_t)v;
         }
