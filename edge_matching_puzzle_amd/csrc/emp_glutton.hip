@@ -83,8 +83,8 @@ struct GlLayout {
         l.risk = take(256 * 2);
         l.pcol = take(P * 4);
         l.avail = take(4 * 8);
-        l.scratch = take(64);
-        l.sets = take(6 * 32);
+        l.scratch = take(16);
+        l.sets = take(6 * 32 + 8 * 8);
         l.cnt = take(4 * (max_colour + 1) * 2);
         l.snb = take(4 * 4);
         l.lo = take(256 * 4);
@@ -180,6 +180,11 @@ struct Wave {
     // pieces of the slots changed in the previous round, critical pieces
     uint64_t *ch_prev, *ch_cur, *d_prev, *d_cur, *e_single, *crit_set;
     bool zero_piece;    // an available piece without option (stats())
+    // [0..3] pieces q with single[q] > 0 (stats()); [4..7] removal slot
+    // context, colours below 64: per side, the colours whose neighbour count
+    // is at most 4 (only those can leave the neighbour without option: a
+    // candidate holds at most 4 of them)
+    uint64_t *single_set, *thin;
     uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q) (stats())
     uint16_t *cnt;      // slot context: [side l][colour c] options of the side-l neighbour matching c
     uint16_t *risk;     // slot context: critical pieces whose options all lie in the slot and its neighbours
@@ -227,6 +232,8 @@ struct Wave {
         d_cur = ch_prev + 12;
         e_single = ch_prev + 16;
         crit_set = ch_prev + 20;
+        single_set = ch_prev + 24;
+        thin = ch_prev + 28;
         lo = reinterpret_cast<uint32_t *>(lds + L.lo);
         cnt = reinterpret_cast<uint16_t *>(lds + L.cnt);
         risk = reinterpret_cast<uint16_t *>(lds + L.risk);
@@ -340,6 +347,12 @@ struct Wave {
             if (lane == 0 && base >= a.P) crit_set[base / kWave] = 0;
         zero_piece = zero;
         if (lane == 0) scratch[0] = n_crit;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            const uint64_t m = __ballot(r * kWave + lane < a.P && single[r * kWave + lane] != 0);
+            if (lane == 0) single_set[r] = m;
+        }
         for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o);
         PC = local;
         const bool any_empty = __ballot(empty) != 0;
@@ -460,11 +473,34 @@ struct Wave {
     // Validity of candidate (slot i, piece p, orientation o) against the live
     // masks (remove_invalid_transitions).
     __device__ bool valid(uint32_t i, uint32_t p, uint32_t o) {
+        const uint32_t nc = scratch[1];
+        if (a.max_colour < 64) {
+            // Fast form: only the thin colours of a side need the count; a
+            // piece without a single-piece slot passes the single check; with
+            // no piece at risk and every free neighbour constrained nothing
+            // else can fail.
+            const uint32_t pc_p = pcol[p];
+            const uint32_t ncol = a.max_colour + 1;
+            bool invalid = false, bounded = true;
+            uint32_t own_single = sp_i == p;
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l) {
+                const uint32_t col = (pc_p >> (8 * ((l + o) & 3))) & 0xFF;
+                const uint32_t j = field(snbp, l);
+                const bool con = col != 0 && j != 0xFFFFu;
+                bounded &= col != 0 || j == 0xFFFFu;
+                own_single += con && field(spnp, l) == p;
+                if (con && ((thin[l] >> col) & 1))
+                    invalid |= cnt[l * ncol + col] == facing(nibble(nbw(l), p), pc_p, l, col);
+            }
+            if (invalid) return false;
+            if (((single_set[p >> 6] >> (p & 63)) & 1) && single[p] > own_single) return false;
+            if (nc == 0 && bounded) return true;
+        }
         Cand cd;
         if (!neighbours(i, p, o, cd)) return false;
         // Only the critical pieces at risk in this slot (lb = 0) can be left
         // without option (slot context).
-        const uint32_t nc = scratch[1];
         uint32_t k = 0;
         if (cd.bounded)
             for (; k < nc && k < kWave; ++k) {
@@ -667,6 +703,14 @@ struct Wave {
         if (scoring) {
             __syncthreads();
             return;
+        }
+        __syncthreads();
+        if (a.max_colour < 64) {
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l) {
+                const uint64_t m = __ballot(lane < nc && cnt[l * nc + lane] <= 4);
+                if (lane == 0) thin[l] = m;
+            }
         }
         // A candidate of slot i can leave a piece without option only if
         // every option of that piece lies in slot i or its free neighbours.
@@ -1064,6 +1108,8 @@ struct Wave {
 // Status codes of one search.
 constexpr uint32_t kSolved = 0, kExhausted = 1, kLimit = 2, kBadPrefix = 3, kInternal = 4;
 
+// One wave per SIMD (the LDS holds four searches per CU), so the whole
+// register file is available.
 __global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     Wave wv(a, lds, blockIdx.x);
