@@ -251,16 +251,33 @@ struct Wave {
     __device__ uint32_t n() const { return a.S - level; }
     __device__ uint64_t nbw(uint32_t l) const { return l == 0 ? nbw0 : (l == 1 ? nbw1 : (l == 2 ? nbw2 : nbw3)); }
     __device__ uint32_t field(uint64_t packed, uint32_t l) const { return (uint32_t)(packed >> (16 * l)) & 0xFFFFu; }
-    // mask word w of the slot context's free neighbours
-    __device__ void neighbour_words(uint32_t w) {
+    // Lane w holds mask word w of slot i and of its free neighbours (nw <=
+    // 64), so the word loops read them from registers.
+    uint64_t my_mi, my_n0, my_n1, my_n2, my_n3;
+    __device__ void slot_words(uint32_t i) {
+        const uint32_t wl = lane < a.nw ? lane : 0;
+        my_mi = M[i * a.nw + wl];
         uint32_t j = field(snbp, 0);
-        nbw0 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        my_n0 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
         j = field(snbp, 1);
-        nbw1 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        my_n1 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
         j = field(snbp, 2);
-        nbw2 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        my_n2 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
         j = field(snbp, 3);
-        nbw3 = j != 0xFFFFu ? M[j * a.nw + w] : 0;
+        my_n3 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
+    }
+    static __device__ uint64_t lane_word(uint64_t v, uint32_t w) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)w);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)w);
+        return (uint64_t)hi << 32 | lo;
+    }
+    // word w of slot i (as loaded by slot_words) and of its neighbours
+    __device__ uint64_t slot_word(uint32_t w) {
+        nbw0 = lane_word(my_n0, w);
+        nbw1 = lane_word(my_n1, w);
+        nbw2 = lane_word(my_n2, w);
+        nbw3 = lane_word(my_n3, w);
+        return lane_word(my_mi, w);
     }
     // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
     __device__ void count(uint64_t evaluated, bool scoring) {
@@ -856,10 +873,10 @@ struct Wave {
             for (uint32_t i = 0; i < nn; ++i) {
                 if (!all && !affected(i)) continue;
                 slot_context(i);
+                slot_words(i);
                 for (uint32_t w = 0; w < a.nw; ++w) {
-                    const uint64_t v = M[i * a.nw + w];
+                    const uint64_t v = slot_word(w);
                     if (!v) continue;
-                    neighbour_words(w);
                     if (!((v >> lane) & 1)) continue;
                     const uint32_t bit = w * 64 + lane;
                     ++evaluated;
@@ -926,11 +943,11 @@ struct Wave {
             }
             if (!last_level && have_any && 2 * (PC - pc[j]) < wave_total) continue;
             slot_context(j, true);
+            slot_words(j);
             bool lists = false;
             for (uint32_t w = 0; w < a.nw; ++w) {
-                const uint64_t v = M[j * a.nw + w];
+                const uint64_t v = slot_word(w);
                 if (!v) continue;
-                neighbour_words(w);
                 const uint32_t bit = w * 64 + lane;
                 const uint32_t p = bit >> 2;
                 Cand cd;
