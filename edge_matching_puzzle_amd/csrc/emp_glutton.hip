@@ -110,6 +110,16 @@ __device__ __forceinline__ uint32_t facing(uint32_t ori, uint32_t qcol, uint32_t
 
 __device__ __forceinline__ uint32_t nibble(uint64_t v, uint32_t q) { return (uint32_t)(v >> (4 * (q & 15))) & 0xF; }
 
+// The 16 pieces of a mask word that have an option in it, as 16 bits.
+__device__ __forceinline__ uint64_t word_pieces(uint64_t v) {
+    uint64_t x = (v | (v >> 1) | (v >> 2) | (v >> 3)) & 0x1111111111111111ull;
+    x = (x | (x >> 3)) & 0x0303030303030303ull;
+    x = (x | (x >> 6)) & 0x000F000F000F000Full;
+    x = (x | (x >> 12)) & 0x000000FF000000FFull;
+    x = (x | (x >> 24)) & 0xFFFFull;
+    return x;
+}
+
 // Largest facing(ori, qcol, l, col) over colours col: the most options in
 // `ori` that one colour toward the slot on side l can leave.
 __device__ __forceinline__ uint32_t most_facing(uint32_t ori, uint32_t qcol, uint32_t l) {
@@ -180,6 +190,11 @@ struct Wave {
     // pieces of the slots changed in the previous round, critical pieces
     uint64_t *ch_prev, *ch_cur, *d_prev, *d_cur, *e_single, *crit_set;
     bool zero_piece;    // an available piece without option (stats())
+    // the last move played by apply() (position, piece); push_known when the
+    // masks were a fixpoint right before it (a search move, not a prefix move
+    // or a pop), so the next removal rounds start from its changes only
+    uint32_t push_pos, push_piece;
+    bool push_known;
     // [0..3] pieces q with single[q] > 0 (stats()); [4..7] removal slot
     // context, colours below 64: per side, the colours whose neighbour count
     // is at most 4 (only those can leave the neighbour without option: a
@@ -310,6 +325,7 @@ struct Wave {
             avail[lane] = v;
         }
         level = max = base = 0;
+        push_known = false;
         __syncthreads();
     }
 
@@ -839,6 +855,66 @@ struct Wave {
         return __ballot(hit) != 0;
     }
 
+    // First removal round after a search move (piece p at position s) from
+    // masks that were a fixpoint: a candidate of slot i can only have lost
+    // its validity when slot i lies within 2 steps of s (the neighbours of s
+    // took its colours, s left the free slots), when a critical piece whose
+    // total dropped (it had options at s or at a neighbour of s) has options
+    // in slot i or its neighbours, when slot i holds a piece that is the
+    // single piece of a slot, or when a free neighbour that lost options of p
+    // now counts at most 4 options showing one of p's colours toward slot i
+    // (the only counts p's removal lowered; a candidate holds at most 4 of
+    // them).  The slots that lost options of p are in ch_prev.
+    __device__ bool push_affected(uint32_t i) {
+        const uint32_t pos = pos_of_slot[i], x = pos % a.W;
+        {
+            const int dx = (int)x - (int)(push_pos % a.W), dy = (int)(pos / a.W) - (int)(push_pos / a.W);
+            if (abs(dx) + abs(dy) <= 2) return true;
+        }
+        uint64_t nbk = ~0ull;   // free neighbour slot of side l in bits 16l..16l+15
+#pragma unroll
+        for (uint32_t l = 0; l < 4; ++l) {
+            const bool inside = l == 0 ? pos >= a.W : (l == 1 ? x + 1 < a.W : (l == 2 ? pos + a.W < a.P : x > 0));
+            const uint32_t q = (uint32_t)((int)pos + off(l));
+            if (inside && is_free(q)) nbk &= ~(0xFFFFull << (16 * l)) | ((uint64_t)slot_of_pos[q] << (16 * l));
+        }
+        auto spread = [](uint64_t v16) {
+            uint64_t v = v16 & 0xFFFF;
+            v = (v | (v << 24)) & 0x000000FF000000FFull;
+            v = (v | (v << 12)) & 0x000F000F000F000Full;
+            v = (v | (v << 6)) & 0x0303030303030303ull;
+            v = (v | (v << 3)) & 0x1111111111111111ull;
+            return v * 0xF;
+        };
+        bool hit = false;
+        for (uint32_t w = lane; w < a.nw; w += kWave) {
+            const uint32_t sh = 16 * (w & 3);
+            const uint64_t dm = spread((d_prev[w >> 2] & crit_set[w >> 2]) >> sh);
+            const uint64_t em = spread(e_single[w >> 2] >> sh);
+            const uint64_t own = M[i * a.nw + w];
+            uint64_t near = own;
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l) {
+                const uint32_t j = field(nbk, l);
+                if (j != 0xFFFFu) near |= M[j * a.nw + w];
+            }
+            hit |= (near & dm) != 0 || (own & em) != 0;
+        }
+        // lanes: side l, colour index ci of p, word group wg (4 words)
+        const uint32_t l = lane >> 4, ci = (lane >> 2) & 3, wg = lane & 3;
+        const uint32_t col = (pcol[push_piece] >> (8 * ci)) & 0xFF;
+        uint32_t j = field(nbk, l);
+        if (j != 0xFFFFu && !((ch_prev[j >> 6] >> (j & 63)) & 1)) j = 0xFFFFu;
+        uint32_t v = 0;
+        if (j != 0xFFFFu && col)
+            for (uint32_t w = 4 * wg; w < 4 * wg + 4 && w < a.nw; ++w)
+                v += (uint32_t)__popcll(M[j * a.nw + w] & a.cm[((size_t)(col - 1) * 4 + l) * a.nw + w]);
+        v += (uint32_t)__shfl_xor((int)v, 1);
+        v += (uint32_t)__shfl_xor((int)v, 2);
+        hit |= wg == 0 && j != 0xFFFFu && col && v <= 4;
+        return __ballot(hit) != 0;
+    }
+
     // remove_invalid_transitions: false when a slot runs empty.  Slot by
     // slot; lane b takes bit b of each word of the slot's mask.  The first
     // round evaluates every slot; later rounds only the slots the changes
@@ -862,16 +938,17 @@ struct Wave {
                 e_single[lane] = 0;
             }
             __syncthreads();
-            if (!first)
+            const bool pushed = first && push_known;
+            if (!first || pushed)
                 for (uint32_t j = lane; j < nn; j += kWave)
-                    if (((ch_prev[j >> 6] >> (j & 63)) & 1) && sp[j] != 0xFFFF)
+                    if ((pushed || ((ch_prev[j >> 6] >> (j & 63)) & 1)) && sp[j] != 0xFFFF)
                         atomicOr(reinterpret_cast<unsigned long long *>(&e_single[sp[j] >> 6]), 1ull << (sp[j] & 63));
             __syncthreads();
             // a piece already without option makes every candidate invalid
-            const bool all = first || zero_piece;
+            const bool all = (first && !push_known) || zero_piece;
             bool removed = false;
             for (uint32_t i = 0; i < nn; ++i) {
-                if (!all && !affected(i)) continue;
+                if (!all && !(pushed ? push_affected(i) : affected(i))) continue;
                 slot_context(i);
                 slot_words(i);
                 for (uint32_t w = 0; w < a.nw; ++w) {
@@ -891,6 +968,7 @@ struct Wave {
                 __syncthreads();
             }
             first = false;
+            push_known = false;
             if (lane == 0) atomicAdd(&a.evals[2], 1ull);
             if (!__ballot(removed)) break;
         }
@@ -1053,6 +1131,7 @@ struct Wave {
         }
         slot = __shfl(slot, 0);
         level -= 1;
+        push_known = false;
         __syncthreads();
         const uint64_t *src = stack + level_offset(level);
         for (uint32_t k = lane; k < n() * a.nw; k += kWave) M[k] = src[k];
@@ -1071,6 +1150,14 @@ struct Wave {
         uint64_t *dst = stack + level_offset(level);
         for (uint32_t k = lane; k < nn * a.nw; k += kWave) dst[k] = M[k];
         const uint32_t pos = pos_of_slot[slot];
+        // pieces whose totals drop: those with options in the played slot
+        for (uint32_t w = lane; w < a.nw; w += kWave) {
+            const uint64_t pm = word_pieces(M[slot * a.nw + w]);
+            if (pm) atomicOr(reinterpret_cast<unsigned long long *>(&d_cur[w >> 2]), pm << (16 * (w & 3)));
+        }
+        push_pos = pos;
+        push_piece = piece;
+        push_known = true;
         const uint32_t pw = piece >> 4;
         const uint64_t keep_p = ~(0xFull << (4 * (piece & 15)));
         bool empty = false;
@@ -1079,7 +1166,11 @@ struct Wave {
             for (uint32_t k = lane; k < (nn - 1) * a.nw; k += kWave) {
                 const uint32_t j = k / a.nw, w = k % a.nw;
                 uint64_t v = (j == slot) ? M[(nn - 1) * a.nw + w] : M[k];
-                if (w == pw) v &= keep_p;
+                if (w == pw) {
+                    // slots that lose options of the played piece
+                    if (v & ~keep_p) atomicOr(reinterpret_cast<unsigned long long *>(&ch_cur[j >> 6]), 1ull << (j & 63));
+                    v &= keep_p;
+                }
                 M[k] = v;
             }
             __syncthreads();
@@ -1105,9 +1196,13 @@ struct Wave {
             const uint64_t *c = a.cm + ((size_t)(col - 1) * 4 + l) * a.nw;
             uint64_t any = 0;
             for (uint32_t w = lane; w < a.nw; w += kWave) {
-                const uint64_t v = M[j * a.nw + w] & c[w];
+                const uint64_t old = M[j * a.nw + w];
+                const uint64_t v = old & c[w];
                 M[j * a.nw + w] = v;
                 any |= v;
+                // pieces that lose options here
+                const uint64_t pm = word_pieces(old & ~v);
+                if (pm) atomicOr(reinterpret_cast<unsigned long long *>(&d_cur[w >> 2]), pm << (16 * (w & 3)));
             }
             emptied |= __ballot(any != 0) == 0;
             __syncthreads();
@@ -1151,6 +1246,8 @@ __global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
                 if (ok) dead = !wv.apply(slot, bit);
             }
         }
+        // the masks before the last prefix move were no fixpoint
+        wv.push_known = false;
         uint64_t pushes = 0;
         if (!ok) status = kBadPrefix;
         else if (dead) status = kExhausted;
