@@ -47,6 +47,7 @@ constexpr int kWave = 64;
 
 struct GlArgs {
     uint32_t P, W, S, nw, max_colour;
+    uint32_t max_mult;           // most sides of one piece showing one (non-frame) colour
     const uint64_t *cm;          // [((c - 1) * 4 + side) * nw + w]
     const uint32_t *piece_col;   // [P] N | E << 8 | S << 16 | W << 24
     const uint64_t *init_masks;  // [P][nw] initial capability per position
@@ -197,8 +198,9 @@ struct Wave {
     bool push_known;
     // [0..3] pieces q with single[q] > 0 (stats()); [4..7] removal slot
     // context, colours below 64: per side, the colours whose neighbour count
-    // is at most 4 (only those can leave the neighbour without option: a
-    // candidate holds at most 4 of them)
+    // is at most max_mult (only those can leave the neighbour without
+    // option: a candidate holds at most max_mult of them, one per side of its
+    // piece showing the colour)
     uint64_t *single_set, *thin;
     uint32_t *lo, *hi;  // per piece q: min / max over free slots j of pc_j - cnt_j(q) (stats())
     uint16_t *cnt;      // slot context: [side l][colour c] options of the side-l neighbour matching c
@@ -741,7 +743,7 @@ struct Wave {
         if (a.max_colour < 64) {
 #pragma unroll
             for (uint32_t l = 0; l < 4; ++l) {
-                const uint64_t m = __ballot(lane < nc && cnt[l * nc + lane] <= 4);
+                const uint64_t m = __ballot(lane < nc && cnt[l * nc + lane] <= a.max_mult);
                 if (lane == 0) thin[l] = m;
             }
         }
@@ -862,9 +864,9 @@ struct Wave {
     // total dropped (it had options at s or at a neighbour of s) has options
     // in slot i or its neighbours, when slot i holds a piece that is the
     // single piece of a slot, or when a free neighbour that lost options of p
-    // now counts at most 4 options showing one of p's colours toward slot i
-    // (the only counts p's removal lowered; a candidate holds at most 4 of
-    // them).  The slots that lost options of p are in ch_prev.
+    // now counts at most max_mult options showing one of p's colours toward
+    // slot i (the only counts p's removal lowered; a candidate holds at most
+    // max_mult of them).  The slots that lost options of p are in ch_prev.
     __device__ bool push_affected(uint32_t i) {
         const uint32_t pos = pos_of_slot[i], x = pos % a.W;
         {
@@ -911,7 +913,7 @@ struct Wave {
                 v += (uint32_t)__popcll(M[j * a.nw + w] & a.cm[((size_t)(col - 1) * 4 + l) * a.nw + w]);
         v += (uint32_t)__shfl_xor((int)v, 1);
         v += (uint32_t)__shfl_xor((int)v, 2);
-        hit |= wg == 0 && j != 0xFFFFu && col && v <= 4;
+        hit |= wg == 0 && j != 0xFFFFu && col && v <= a.max_mult;
         return __ballot(hit) != 0;
     }
 
@@ -1404,6 +1406,14 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     a.S = S;
     a.nw = t.nw;
     a.max_colour = t.max_colour;
+    a.max_mult = 1;
+    for (uint32_t q = 0; q < t.P; ++q)
+        for (uint32_t s1 = 0; s1 < 4; ++s1) {
+            const uint32_t c1 = (t.piece_col[q] >> (8 * s1)) & 0xFF;
+            uint32_t m = 0;
+            for (uint32_t s2 = 0; s2 < 4; ++s2) m += c1 && ((t.piece_col[q] >> (8 * s2)) & 0xFF) == c1;
+            a.max_mult = std::max(a.max_mult, m);
+        }
     a.cm = d_cm;
     a.piece_col = d_pcol;
     a.init_masks = d_init;
