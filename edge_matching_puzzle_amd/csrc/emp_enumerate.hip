@@ -585,17 +585,17 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const uint32_t xab = cur_ab;
             fresh = false;
             const uint64_t d = tv.nbr(tk);
+            // take the placed piece (a no-op AND unless descending) before the
+            // next level's candidates read the available bits; a dead end
+            // gives it back below
+            A(xab, xw) &= ~xm;
             uint64_t cc[W];
             candidates<W>(tv, tk, d, tl,
                           [&](uint32_t off, uint32_t) {
                               return (uint32_t)*reinterpret_cast<const STK *>(
                                   reinterpret_cast<const uint8_t *>(stk + lane) + off);
                           },
-                          [&](uint64_t dd, int w) {
-                              const uint32_t ab = desc_abase(dd);
-                              const uint64_t v = A(ab, (uint32_t)w);
-                              return (ab == xab && (uint32_t)w == xw) ? (v & ~xm) : v;
-                          },
+                          [&](uint64_t dd, int w) { return A(desc_abase(dd), (uint32_t)w); },
                           cc, RANKED ? rank_base + (size_t)tk * rank_stride : nullptr);
             // A placement whose next level has no candidate is a dead end:
             // the lane stays at level k with the rest of c.
@@ -604,8 +604,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 #pragma unroll
             for (int w = 0; w < W; ++w)
                 c[w] = stay ? (c[w] & ~((w == (t >> 6)) ? tbit : 0ull)) : cc[w];
-            // commit the take when descending (a no-op AND otherwise)
-            A(xab, xw) &= (down && !stay) ? ~xm : ~0ull;
+            // a dead end gives the piece back (a no-op OR otherwise)
+            A(xab, xw) |= stay ? xm : 0ull;
             k = stay ? k : tk;
             last = stay ? t : tl;
             cur_ab = stay ? cur_ab : desc_abase(d);
