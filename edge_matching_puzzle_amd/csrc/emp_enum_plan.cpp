@@ -103,6 +103,43 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
         h.off_rot[k] = off;
         off += (4 * n_kind[k] + 1) / 2;
     }
+    // Identical-piece chains: the pieces an appearance class spans, lowest
+    // first.  Pieces showing one look show the same kept looks (they are
+    // rotations of each other with the same symmetry), so every class of a
+    // piece names the same chain.
+    std::vector<int> succ[3], pred[3];
+    for (int k = 0; k < 3; ++k) {
+        succ[k].assign(n_kind[k], -1);
+        pred[k].assign(n_kind[k], -1);
+        for (const std::vector<uint64_t> &mask : dup[k]) {
+            std::vector<int> pieces;
+            for (uint32_t w = 0; w < R; ++w)
+                for (uint64_t v = mask[w]; v; v &= v - 1) {
+                    const int p = (int)((w * 64 + __builtin_ctzll(v)) >> 2);
+                    if (pieces.empty() || pieces.back() != p) pieces.push_back(p);
+                }
+            for (size_t i = 0; i + 1 < pieces.size(); ++i) {
+                const int p = pieces[i], q = pieces[i + 1];
+                if ((succ[k][p] >= 0 && succ[k][p] != q) || (pred[k][q] >= 0 && pred[k][q] != p))
+                    throw std::logic_error("appearance classes of one piece span different pieces");
+                succ[k][p] = q;
+                pred[k][q] = p;
+            }
+        }
+    }
+    const uint32_t abase_of[3] = {0, h.words[0], h.words[0] + h.words[1]};
+    if (h.avail_words * 16 > 4096) throw std::domain_error("identical-piece chains index at most 4096 piece slots");
+    std::vector<uint32_t> pred_list;
+    for (int k = 2; k >= 0; --k)
+        for (int ki = (int)n_kind[k] - 1; ki >= 0; --ki)
+            if (pred[k][ki] >= 0)
+                pred_list.push_back((abase_of[k] * 16 + (uint32_t)ki) | (abase_of[k] * 16 + (uint32_t)pred[k][ki]) << 16);
+    h.off_succ = off;
+    off += (h.avail_words * 16 * 2 + 7) / 8;
+    h.off_pred = off;
+    h.npred = (uint32_t)pred_list.size();
+    off += (h.npred * 4 + 7) / 8;
+
     uint32_t col_bytes[3], col_total = 0;
     for (int k = 0; k < 3; ++k) {
         col_bytes[k] = col_total;
@@ -165,6 +202,19 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
                 b[h.off_dup[k] + h.ndup[k] * R + w] |= dup[k][d][w];
                 for (uint64_t v = dup[k][d][w]; v; v &= v - 1) cls[w * 64 + __builtin_ctzll(v)] = (uint16_t)d;
             }
+    }
+
+    {
+        uint16_t *sc = reinterpret_cast<uint16_t *>(b + h.off_succ);
+        for (int k = 0; k < 3; ++k)
+            for (uint32_t ki = 0; ki < n_kind[k]; ++ki) {
+                if (succ[k][ki] < 0) continue;
+                const uint32_t q = (uint32_t)succ[k][ki];
+                const uint32_t vn = (uint32_t)(b[h.off_valid[k] + (4 * q >> 6)] >> (4 * q & 63)) & 0xF;
+                sc[abase_of[k] * 16 + ki] = (uint16_t)((abase_of[k] * 16 + q) | vn << 12);
+            }
+        uint32_t *pl = reinterpret_cast<uint32_t *>(b + h.off_pred);
+        for (uint32_t i = 0; i < h.npred; ++i) pl[i] = pred_list[i];
     }
 
     for (uint32_t w = 0; w < 16; ++w) b[h.off_ones + w] = ~0ull;

@@ -63,6 +63,15 @@ struct EnumTableHeader {
     uint32_t off_ones;           // 16 all-ones words, then 16 all-zeros words
     uint32_t off_cls[3];         // u16 per transition: its appearance class, 0xFFFF for none
     uint32_t off_col;            // colour bytes: per kind, per side, one byte per transition
+    // Identical pieces.  A piece slot is abase(K) * 16 + kind_index, abase(K)
+    // being the offset of kind K's words in a lane's available-transition
+    // region (the slot's transitions are the 4 bits at slot * 4 of the region).
+    // Pieces of one kind that share their appearance classes form a chain,
+    // lowest kind_index first.
+    uint32_t off_succ;           // u16 per slot: 0, or the next piece of the chain's slot | its valid orientations << 12
+    uint32_t off_pred;           // u32 per chained piece after the first, highest slot first: slot | previous piece's slot << 16
+    uint32_t npred;              // entries at off_pred
+    uint32_t pad;
 };
 static_assert(sizeof(EnumTableHeader) % 8 == 0, "header must be u64 aligned");
 constexpr uint32_t kHeaderWords = sizeof(EnumTableHeader) / 8;

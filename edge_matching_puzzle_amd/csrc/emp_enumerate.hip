@@ -64,7 +64,8 @@ struct DfsArgs {
     uint16_t *out_stack;
     uint32_t out_cap;
     unsigned int *out_count;
-    unsigned int *flags;           // [0] output overflow, [1] splits, [2] items, [3] drain
+    unsigned int *flags;           // [0] output overflow, [1] splits, [2] items, [3] drain,
+                                   // [4] identical pieces out of order (never cleared per pass)
     // per puzzle
     unsigned long long *nodes;
     unsigned long long *solutions;
@@ -159,6 +160,7 @@ struct TableView {
     uint32_t n0, n1, n2, d0, d1, d2;
     uint32_t cm0, cm1, cm2, va0, va1, va2, du0, du1, du2, ro0, ro1, ro2;
     uint32_t off_base, off_nbr, off_ones, off_zero, cl0, cl1, cl2, off_col;
+    uint32_t off_succ, off_pred, npred;
 
     __device__ __forceinline__ void load(const uint64_t *blob) {
         b = blob;
@@ -178,12 +180,15 @@ struct TableView {
         off_zero = h[29] + 16;
         cl0 = h[30]; cl1 = h[31]; cl2 = h[32];
         off_col = h[33];
+        off_succ = h[34];
+        off_pred = h[35];
+        npred = h[36];
     }
     // One table for the whole grid: keep its fields in scalar registers.
     __device__ __forceinline__ void make_uniform() {
         uint32_t *f[] = {&P, &ncol, &n0, &n1, &n2, &d0, &d1, &d2, &cm0, &cm1, &cm2, &va0, &va1, &va2,
                          &du0, &du1, &du2, &ro0, &ro1, &ro2, &off_base, &off_nbr, &off_ones, &off_zero,
-                         &cl0, &cl1, &cl2, &off_col};
+                         &cl0, &cl1, &cl2, &off_col, &off_succ, &off_pred, &npred};
 #pragma unroll
         for (uint32_t *x : f) *x = __builtin_amdgcn_readfirstlane(*x);
     }
@@ -206,9 +211,16 @@ struct TableView {
     __device__ __forceinline__ uint32_t rot(int kind, uint32_t t) const {
         return reinterpret_cast<const uint32_t *>(b + sel3(kind, ro0, ro1, ro2))[t];
     }
+    // identical-piece chain entry of a piece slot (abase * 16 + kind_index)
+    __device__ __forceinline__ uint32_t succ(uint32_t slot) const {
+        return reinterpret_cast<const uint16_t *>(b + off_succ)[slot];
+    }
+    __device__ __forceinline__ uint32_t pred_entry(uint32_t i) const {
+        return reinterpret_cast<const uint32_t *>(b + off_pred)[i];
+    }
 };
 
-static_assert(sizeof(EnumTableHeader) == 34 * 4, "TableView::load reads EnumTableHeader by field index");
+static_assert(sizeof(EnumTableHeader) == 38 * 4, "TableView::load reads EnumTableHeader by field index");
 
 __device__ __forceinline__ int desc_kind(uint64_t d) { return (int)(d >> 3) & 3; }
 // offset of the position's kind in a lane's available-transition region
@@ -243,7 +255,13 @@ __device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
 // get_avail(d, w): available transition word w of the kind of the position
 // whose record word is d (only words below the kind's natural count matter:
 // the base mask is zero above).
-template <int W, typename GetChoice, typename GetAvail>
+//
+// Without CLASSES the caller's available bits already hide every piece whose
+// chain holds a lower available piece (see the chain entries of the table):
+// identical pieces that are both available are candidates together or not at
+// all, as their transitions show the same looks, so hiding the higher ones
+// from the available bits removes exactly what the class reduction removes.
+template <int W, bool CLASSES = true, typename GetChoice, typename GetAvail>
 __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint64_t d, int last,
                                            GetChoice get_choice, GetAvail get_avail, uint64_t (&c)[W],
                                            const uint16_t *rank_row = nullptr) {
@@ -268,22 +286,24 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
     }
     // Appearance classes present among the candidates, lowest first; most
     // candidate sets meet none, so the loop usually does not run.
-    const uint64_t *uni = tv.b + desc_union(d);
-    uint64_t rem[W];
+    if (CLASSES) {
+        const uint64_t *uni = tv.b + desc_union(d);
+        uint64_t rem[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) rem[w] = c[w] & uni[w];
-    while (__ballot(any_bit<W>(rem)) != 0) {
-        const int b = first_bit<W>(rem);
-        const uint64_t *dm = b >= 0 ? tv.b + tv.off_dup(K) + tv.cls(K, (uint32_t)b) * W : tv.b + tv.off_zero;
-        bool seen = false;
+        for (int w = 0; w < W; ++w) rem[w] = c[w] & uni[w];
+        while (__ballot(any_bit<W>(rem)) != 0) {
+            const int b = first_bit<W>(rem);
+            const uint64_t *dm = b >= 0 ? tv.b + tv.off_dup(K) + tv.cls(K, (uint32_t)b) * W : tv.b + tv.off_zero;
+            bool seen = false;
 #pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const uint64_t m = c[w] & dm[w];
-            // keep the transitions of the lowest piece of the class
-            const uint64_t low = m ? (0xFull << (__builtin_ctzll(m) & 60)) : 0ull;
-            c[w] &= ~(m & ~(seen ? 0ull : low));
-            seen |= m != 0;
-            rem[w] &= ~dm[w];
+            for (int w = 0; w < W; ++w) {
+                const uint64_t m = c[w] & dm[w];
+                // keep the transitions of the lowest piece of the class
+                const uint64_t low = m ? (0xFull << (__builtin_ctzll(m) & 60)) : 0ull;
+                c[w] &= ~(m & ~(seen ? 0ull : low));
+                seen |= m != 0;
+                rem[w] &= ~dm[w];
+            }
         }
     }
     // Only transitions above `last` (last = -1 keeps all).
@@ -517,6 +537,20 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                             const uint32_t bb = t & ~3u;
                             A(desc_abase(tv.nbr(j)), bb >> 6) &= ~(0xFull << (bb & 63));
                         }
+                        // Chained pieces behind an available lower piece are
+                        // hidden (highest slot first, so each test reads its
+                        // lower piece before that piece is itself hidden).  A
+                        // placed piece whose lower piece is still available
+                        // was never a candidate: the item is reported.
+                        for (uint32_t i = 0; i < tv.npred; ++i) {
+                            const uint32_t e = tv.pred_entry(i);
+                            const uint32_t r = e & 0xFFFF, q = e >> 16;
+                            uint64_t &rv = avail[(r >> 4) * kWave + lane];
+                            const uint64_t rm = 0xFull << ((r & 15) * 4);
+                            const bool lower = ((avail[(q >> 4) * kWave + lane] >> ((q & 15) * 4)) & 0xF) != 0;
+                            if (lower && !(rv & rm)) atomicOr(&a.flags[4], 1u);
+                            rv &= lower ? ~rm : ~0ull;
+                        }
                         atomicMin(&a.first_tick[puzzle], (unsigned long long)wall_clock64());
                         atomicAdd(&a.flags[2], 1u);
                         active = true;
@@ -563,10 +597,26 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const uint64_t dprev = tv.nbr(km1);
             // S(k) is dead for every lane that does not place.
             S(k) = (STK)t;
-            // pop: level k - 1 gets its piece back (a no-op OR otherwise)
+            // The piece that changes hands, by piece slot: a pop gives level
+            // k - 1's piece back and hides the next piece of its chain again;
+            // a placement takes t's piece (before the next level's candidates
+            // read the available bits) and shows the next piece of its chain.
+            // Other lanes apply empty masks to slot 0.
+            const bool move = pop || down;
+            const uint32_t slot = pop ? desc_abase(dprev) * 16 + (held >> 2)
+                                      : (down ? cur_ab * 16 + ((uint32_t)t >> 2) : 0u);
+            const uint64_t pm = move ? (0xFull << ((slot & 15) * 4)) : 0ull;
+            const uint32_t se = move ? tv.succ(slot) : 0u;
+            const uint32_t ss = se & 0xFFF;
+            const uint64_t sm = (uint64_t)(se >> 12) << ((ss & 15) * 4);
             {
-                const uint32_t bb = held & ~3u;
-                A(desc_abase(dprev), bb >> 6) |= pop ? (0xFull << (bb & 63)) : 0ull;
+                // masked merges: the piece's bits become `pop`, the next
+                // piece's bits become `!pop`
+                const uint64_t to_piece = pop ? ~0ull : 0ull;
+                uint64_t &pv = avail[(slot >> 4) * kWave + lane];
+                pv = (pv & ~pm) | (pm & to_piece);
+                uint64_t &sv = avail[(ss >> 4) * kWave + lane];
+                sv = (sv & ~sm) | (sm & ~to_piece);
             }
             if (sol) {
                 const uint64_t sidx = atomicAdd(&a.solutions[puzzle], 1ull);
@@ -580,17 +630,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             ++steps;
             const uint32_t tk = pop ? km1 : (down ? k + 1 : k);
             const int tl = pop ? (int)held : (down ? -1 : (sol ? t : last));
-            const uint32_t xw = down ? ((uint32_t)t >> 6) : 0u;
-            const uint64_t xm = down ? (0xFull << ((uint32_t)t & 60)) : 0ull;
-            const uint32_t xab = cur_ab;
             fresh = false;
             const uint64_t d = tv.nbr(tk);
-            // take the placed piece (a no-op AND unless descending) before the
-            // next level's candidates read the available bits; a dead end
-            // gives it back below
-            A(xab, xw) &= ~xm;
             uint64_t cc[W];
-            candidates<W>(tv, tk, d, tl,
+            candidates<W, false>(tv, tk, d, tl,
                           [&](uint32_t off, uint32_t) {
                               return (uint32_t)*reinterpret_cast<const STK *>(
                                   reinterpret_cast<const uint8_t *>(stk + lane) + off);
@@ -604,8 +647,14 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 #pragma unroll
             for (int w = 0; w < W; ++w)
                 c[w] = stay ? (c[w] & ~((w == (t >> 6)) ? tbit : 0ull)) : cc[w];
-            // a dead end gives the piece back (a no-op OR otherwise)
-            A(xab, xw) |= stay ? xm : 0ull;
+            // a dead end gives the piece back and hides its chain's next
+            // piece again (no-ops otherwise)
+            {
+                uint64_t &pv = avail[(slot >> 4) * kWave + lane];
+                pv |= stay ? pm : 0ull;
+                uint64_t &sv = avail[(ss >> 4) * kWave + lane];
+                sv &= stay ? ~sm : ~0ull;
+            }
             k = stay ? k : tk;
             last = stay ? t : tl;
             cur_ab = stay ? cur_ab : desc_abase(d);
@@ -1105,6 +1154,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             ++stats.dfs_launches;
             stats.splits += counters[3];
             stats.items_processed += counters[4];
+            if (counters[6])
+                throw std::invalid_argument("a situation places an identical piece while a lower-numbered twin is "
+                                            "still available (such a piece is never a candidate)");
         }
         n_items = std::min<uint64_t>(counters[1], cap);
         cur = 1 - cur;
