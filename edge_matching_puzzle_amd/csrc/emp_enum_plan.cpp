@@ -219,6 +219,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
 
     for (uint32_t w = 0; w < 16; ++w) b[h.off_ones + w] = ~0ull;
     const uint32_t sb = stack_bytes ? stack_bytes : (R <= 4 ? 1 : 2);
+    h.stack_bytes = sb;
     {
         uint8_t *col = reinterpret_cast<uint8_t *>(b + h.off_col);
         for (int k = 0; k < 3; ++k)
@@ -252,8 +253,8 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
             const uint64_t stk_off = (uint64_t)code * 64 * sb;
             const uint64_t col_off = col_bytes[nk] + facing * 4 * n_kind[nk];
             const uint64_t cm_block = h.off_cm[k] + (uint64_t)s * h.ncol * R;
-            if (stk_off > 0xFFFF || cm_block >= (1u << 23)) throw std::domain_error("table offsets exceed their fields");
-            rec_out[1 + n] = stk_off | col_off << 16 | cm_block << 32 | (uint64_t)code << 55;
+            if (stk_off > 0xFFFF || cm_block >= (1u << 29)) throw std::domain_error("table offsets exceed their fields");
+            rec_out[1 + n] = stk_off | col_off << 16 | (cm_block * 8) << 32;
             ++n;
         }
         const uint64_t abase = k == 0 ? 0 : (k == 1 ? h.words[0] : h.words[0] + h.words[1]);
@@ -321,9 +322,9 @@ void moves_to_stack(const EnumTable &t, const Puzzle &pz, const uint32_t *moves,
         for (uint32_t w = 0; w < R; ++w) c[w] = b[h.off_base + (size_t)i * R + w] & avail[K][w];
         for (uint32_t q = 0, n = (uint32_t)(rec[0] & 7); q < n; ++q) {
             const uint64_t e = rec[1 + q];
-            const uint32_t j = (uint32_t)(e >> 55);
+            const uint32_t j = ((uint32_t)e & 0xFFFF) / (64 * h.stack_bytes);
             const uint32_t col = colour[((uint32_t)e >> 16) + stack[j]];
-            const uint64_t *row = b + ((e >> 32) & 0x7FFFFF) + (size_t)col * R;
+            const uint64_t *row = b + (e >> 35) + (size_t)col * R;
             for (uint32_t w = 0; w < R; ++w) c[w] &= row[w];
         }
         for (uint32_t d = 0; d < h.ndup[K]; ++d) {

@@ -39,9 +39,9 @@ constexpr uint32_t kUnconstrained = 0x1FF;
 //               (index * 64 lanes * bytes per transition)
 //   bits 16-31  byte offset in the colour bytes of the row (neighbour kind,
 //               side facing this position)
-//   bits 32-54  u64 offset of the colour-mask block of (this kind, the side
-//               facing the neighbour): off_cm[K] + side * ncol * row
-//   bits 55-63  its strategy index
+//   bits 32-63  byte offset of the colour-mask block of (this kind, the side
+//               facing the neighbour): 8 * (off_cm[K] + side * ncol * row)
+// (the neighbour's strategy index is the stack offset / (64 * stack_bytes))
 // Frame sides are folded into the position's base mask instead.
 constexpr uint32_t kNbrWords = 5;
 
@@ -71,7 +71,7 @@ struct EnumTableHeader {
     uint32_t off_succ;           // u16 per slot: 0, or the next piece of the chain's slot | its valid orientations << 12
     uint32_t off_pred;           // u32 per chained piece after the first, highest slot first: slot | previous piece's slot << 16
     uint32_t npred;              // entries at off_pred
-    uint32_t pad;
+    uint32_t stack_bytes;        // bytes per transition in the stacks the neighbour records address
 };
 static_assert(sizeof(EnumTableHeader) % 8 == 0, "header must be u64 aligned");
 constexpr uint32_t kHeaderWords = sizeof(EnumTableHeader) / 8;

@@ -160,7 +160,7 @@ struct TableView {
     uint32_t n0, n1, n2, d0, d1, d2;
     uint32_t cm0, cm1, cm2, va0, va1, va2, du0, du1, du2, ro0, ro1, ro2;
     uint32_t off_base, off_nbr, off_ones, off_zero, cl0, cl1, cl2, off_col;
-    uint32_t off_succ, off_pred, npred;
+    uint32_t off_succ, off_pred, npred, stk_shift;
 
     __device__ __forceinline__ void load(const uint64_t *blob) {
         b = blob;
@@ -183,12 +183,13 @@ struct TableView {
         off_succ = h[34];
         off_pred = h[35];
         npred = h[36];
+        stk_shift = h[37] == 2 ? 7u : 6u;
     }
     // One table for the whole grid: keep its fields in scalar registers.
     __device__ __forceinline__ void make_uniform() {
         uint32_t *f[] = {&P, &ncol, &n0, &n1, &n2, &d0, &d1, &d2, &cm0, &cm1, &cm2, &va0, &va1, &va2,
                          &du0, &du1, &du2, &ro0, &ro1, &ro2, &off_base, &off_nbr, &off_ones, &off_zero,
-                         &cl0, &cl1, &cl2, &off_col, &off_succ, &off_pred, &npred};
+                         &cl0, &cl1, &cl2, &off_col, &off_succ, &off_pred, &npred, &stk_shift};
 #pragma unroll
         for (uint32_t *x : f) *x = __builtin_amdgcn_readfirstlane(*x);
     }
@@ -228,13 +229,19 @@ __device__ __forceinline__ uint32_t desc_abase(uint64_t d) { return (uint32_t)(d
 // u64 offset of the kind's class-union row in the table
 __device__ __forceinline__ uint32_t desc_union(uint64_t d) { return (uint32_t)(d >> 32); }
 
+// Lowest set bit of c, or -1: the first non-zero word is selected, then one
+// count of trailing zeros.
 template <int W>
 __device__ __forceinline__ int first_bit(const uint64_t (&c)[W]) {
-    int t = -1;
+    uint64_t x = c[0];
+    int base = 0;
 #pragma unroll
-    for (int w = W - 1; w >= 0; --w)
-        if (c[w]) t = w * 64 + __builtin_ctzll(c[w]);
-    return t;
+    for (int w = 1; w < W; ++w) {
+        const bool z = x == 0;
+        x = z ? c[w] : x;
+        base = z ? w * 64 : base;
+    }
+    return x ? base + (int)__builtin_ctzll(x) : -1;
 }
 
 template <int W>
@@ -251,7 +258,9 @@ __device__ __forceinline__ bool any_bit(const uint64_t (&c)[W]) {
 // only the lowest piece present keeps its transitions; then only transitions
 // above `last`.  The class reduction is applied to the full candidate set, as
 // the reference's identical-piece masks accumulate from the first candidate
-// on.  get_choice(stack_byte_offset, j): transition placed at index j;
+// on.  get_choice(stack_byte_offset): transition placed at the index whose
+// level starts at that byte offset of a wave's stack (index * 64 lanes *
+// stack_bytes);
 // get_avail(d, w): available transition word w of the kind of the position
 // whose record word is d (only words below the kind's natural count matter:
 // the base mask is zero above).
@@ -279,8 +288,10 @@ __device__ __forceinline__ void candidates(const TableView &tv, uint32_t k, uint
         if (__ballot(on) == 0) break;
         const uint64_t e = on ? rec[i] : 0ull;
         const uint32_t lo = (uint32_t)e, hi = (uint32_t)(e >> 32);
-        const uint32_t col = tv.colour((lo >> 16) + get_choice(lo & 0xFFFF, hi >> 23));
-        const uint64_t *row = on ? tv.b + (hi & 0x7FFFFF) + col * W : tv.b + tv.off_ones;
+        const uint32_t col = tv.colour((lo >> 16) + get_choice(lo & 0xFFFF));
+        // hi: byte offset of the colour-mask block in the table
+        const uint64_t *row = on ? reinterpret_cast<const uint64_t *>(reinterpret_cast<const uint8_t *>(tv.b) + hi) + col * W
+                                 : tv.b + tv.off_ones;
 #pragma unroll
         for (int w = 0; w < W; ++w) c[w] &= row[w];
     }
@@ -355,7 +366,7 @@ __global__ void expand_kernel(DfsArgs a) {
     }
     uint64_t c[W];
     const uint16_t *rank_row = RANKED ? a.ranks + a.rank_off[puzzle] + (size_t)kmin * a.rank_stride[puzzle] : nullptr;
-    candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t, uint32_t j) { return (uint32_t)stack[j]; },
+    candidates<W>(tv, kmin, tv.nbr(kmin), last, [&](uint32_t off) { return (uint32_t)stack[off >> tv.stk_shift]; },
                   [&](uint64_t d, int w) {
                       const int kk = desc_kind(d);
                       uint64_t v = 0;
@@ -634,7 +645,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const uint64_t d = tv.nbr(tk);
             uint64_t cc[W];
             candidates<W, false>(tv, tk, d, tl,
-                          [&](uint32_t off, uint32_t) {
+                          [&](uint32_t off) {
                               return (uint32_t)*reinterpret_cast<const STK *>(
                                   reinterpret_cast<const uint8_t *>(stk + lane) + off);
                           },
