@@ -144,10 +144,22 @@ def bench_c3(args, ctx):
     for i in range(args.warmup):
         ctx.glutton_max(w, h, c, prefixes=rows[:256], push_budget=2)
         print("warmup step %d done" % i, file=sys.stderr, flush=True)
+    chunk = args.chunk if args.chunk > 0 else len(rows)
     t0 = time.perf_counter()
     pushes = 0
     for _ in range(args.steps):
-        res, boards, st = ctx.glutton_max(w, h, c, prefixes=rows, push_budget=args.push_budget)
+        # One launch per chunk of prefixes (long runs print as they go).
+        res, st = [], {}
+        for lo in range(0, len(rows), chunk):
+            r, _, s = ctx.glutton_max(w, h, c, prefixes=rows[lo:lo + chunk], push_budget=args.push_budget)
+            res += r
+            for key in ("pushes", "kernel_ms", "evaluations", "score_evaluations", "rounds"):
+                st[key] = st.get(key, 0) + s[key]
+            st["waves"], st["lds_bytes"] = s["waves"], s["lds_bytes"]
+            if chunk < len(rows):
+                print("prefixes %d..%d: %d pushes, max depth %d" % (lo, lo + len(r) - 1, s["pushes"],
+                                                                    max(x["max_depth"] for x in r)),
+                      file=sys.stderr, flush=True)
         pushes += st["pushes"]
         print("step done: %d pushes, %.1f ms kernel, %.3g evaluations" % (st["pushes"], st["kernel_ms"], st["evaluations"]),
               file=sys.stderr, flush=True)
@@ -231,6 +243,7 @@ def main():
     ap.add_argument("--node-limit", type=int, default=200_000_000, help="C2 nodes per instance")
     ap.add_argument("--prefixes", type=int, default=4096, help="C3 prefixes")
     ap.add_argument("--push-budget", type=int, default=16, help="C3 pushes per prefix")
+    ap.add_argument("--chunk", type=int, default=0, help="C3 prefixes per launch (0: all in one launch)")
     ap.add_argument("--autotune", action="store_true", help="tune the C1 launch before timing")
     ap.add_argument("--cpu-nodes", type=int, default=600_000_000,
                     help="nodes of the CPU baseline sample (about 15 s on one core)")
