@@ -621,13 +621,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const uint32_t ss = se & 0xFFF;
             const uint64_t sm = (uint64_t)(se >> 12) << ((ss & 15) * 4);
             {
-                // masked merges: the piece's bits become `pop`, the next
-                // piece's bits become `!pop`
-                const uint64_t to_piece = pop ? ~0ull : 0ull;
+                // masked merges (bit-field inserts): the piece's bits become
+                // `pop`, the next piece's bits become `!pop`
+                const uint64_t to_piece = pop ? ~0ull : 0ull, to_next = pop ? 0ull : ~0ull;
                 uint64_t &pv = avail[(slot >> 4) * kWave + lane];
-                pv = (pv & ~pm) | (pm & to_piece);
+                pv ^= (pv ^ to_piece) & pm;
                 uint64_t &sv = avail[(ss >> 4) * kWave + lane];
-                sv = (sv & ~sm) | (sm & ~to_piece);
+                sv ^= (sv ^ to_next) & sm;
             }
             if (sol) {
                 const uint64_t sidx = atomicAdd(&a.solutions[puzzle], 1ull);
