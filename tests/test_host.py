@@ -136,3 +136,15 @@ def test_spiral_prefixes_are_valid_and_distinct():
                 assert look[1] == cells[pos + 1][3]
             if pos + w in cells:
                 assert look[2] == cells[pos + w][0]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before
+    touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr + r.stdout
+    assert r.stdout.strip() == ""
