@@ -953,19 +953,68 @@ struct Wave {
                 if (!all && !(pushed ? push_affected(i) : affected(i))) continue;
                 slot_context(i);
                 slot_words(i);
-                for (uint32_t w = 0; w < a.nw; ++w) {
-                    const uint64_t v = slot_word(w);
-                    if (!v) continue;
-                    if (!((v >> lane) & 1)) continue;
-                    const uint32_t bit = w * 64 + lane;
-                    ++evaluated;
-                    if (!valid(i, bit >> 2, bit & 3)) {
-                        atomicAnd(reinterpret_cast<unsigned long long *>(&M[i * a.nw + w]), ~(1ull << lane));
-                        atomicOr(reinterpret_cast<unsigned long long *>(&ch_cur[i >> 6]), 1ull << (i & 63));
-                        const uint32_t pp = bit >> 2;
-                        atomicOr(reinterpret_cast<unsigned long long *>(&d_cur[pp >> 6]), 1ull << (pp & 63));
-                        removed = true;
+                // The slot's candidates in batches of up to 64, whole mask
+                // words at a time: bit b of word w goes to list entry fill +
+                // (its rank among the word's bits), lane j evaluates entry j.
+                // Word by word is the order that counts: a word's candidates
+                // see the removals of the words before it (validity compares
+                // round-start totals with live masks, so the order changes
+                // the result).  A batch therefore keeps its results up to the
+                // first word with a removal and starts again after it.  The
+                // list lives in the per-piece extremes area, which best()
+                // rewrites before reading it.
+                for (uint32_t w0 = 0; w0 < a.nw;) {
+                    uint32_t fill = 0, w = w0;
+                    for (; w < a.nw; ++w) {
+                        const uint64_t v = lane_word(my_mi, w);
+                        const uint32_t c = (uint32_t)__popcll(v);
+                        if (fill + c > kWave) break;
+                        if ((v >> lane) & 1) {
+                            const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(v >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)v, 0u));
+                            lo[fill + r] = w * 64 + lane;
+                        }
+                        fill += c;
                     }
+                    if (fill == 0) break;   // the remaining words are empty
+                    __syncthreads();
+                    uint32_t bit = 0;
+                    bool bad = false;
+                    if (lane < fill) {
+                        bit = lo[lane];
+                        // neighbour words of the candidate's word (the
+                        // neighbours' masks do not change while slot i is
+                        // evaluated)
+                        const uint32_t bw = bit >> 6;
+                        uint32_t j = field(snbp, 0);
+                        nbw0 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
+                        j = field(snbp, 1);
+                        nbw1 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
+                        j = field(snbp, 2);
+                        nbw2 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
+                        j = field(snbp, 3);
+                        nbw3 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
+                        bad = !valid(i, bit >> 2, bit & 3);
+                    }
+                    __syncthreads();
+                    const uint64_t bads = __ballot(bad);
+                    // words up to the first with a removal are final
+                    const uint32_t last_w = bads ? (uint32_t)__builtin_amdgcn_readlane(
+                                                       (int)(bit >> 6), (int)__builtin_ctzll(bads))
+                                                 : w - 1;
+                    if (lane < fill && (bit >> 6) <= last_w) {
+                        ++evaluated;
+                        if (bad) {
+                            atomicAnd(reinterpret_cast<unsigned long long *>(&M[i * a.nw + (bit >> 6)]),
+                                      ~(1ull << (bit & 63)));
+                            atomicOr(reinterpret_cast<unsigned long long *>(&ch_cur[i >> 6]), 1ull << (i & 63));
+                            const uint32_t pp = bit >> 2;
+                            atomicOr(reinterpret_cast<unsigned long long *>(&d_cur[pp >> 6]), 1ull << (pp & 63));
+                            removed = true;
+                        }
+                    }
+                    w0 = last_w + 1;
+                    __syncthreads();
                 }
                 __syncthreads();
             }
@@ -1224,7 +1273,7 @@ constexpr uint32_t kSolved = 0, kExhausted = 1, kLimit = 2, kBadPrefix = 3, kInt
 
 // One wave per SIMD (the LDS holds four searches per CU), so the whole
 // register file is available.
-__global__ void __launch_bounds__(64) glutton_kernel(GlArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(GlArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     Wave wv(a, lds, blockIdx.x);
     while (true) {
