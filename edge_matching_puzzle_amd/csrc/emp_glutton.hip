@@ -1074,15 +1074,43 @@ struct Wave {
             slot_context(j, true);
             slot_words(j);
             bool lists = false;
-            for (uint32_t w = 0; w < a.nw; ++w) {
-                const uint64_t v = slot_word(w);
-                if (!v) continue;
-                const uint32_t bit = w * 64 + lane;
-                const uint32_t p = bit >> 2;
+            // The slot's candidates packed 64 per pass, as in the removal
+            // rounds (scoring changes nothing, and each lane keeps its best
+            // under the (key, slot, bit) order, so the packing does not change
+            // the choice).  The list lives in the at-risk list, which scoring
+            // does not use and the next slot context rewrites.
+            for (uint32_t w0 = 0; w0 < a.nw;) {
+                uint32_t fill = 0, w = w0;
+                for (; w < a.nw; ++w) {
+                    const uint64_t v = lane_word(my_mi, w);
+                    const uint32_t c = (uint32_t)__popcll(v);
+                    if (fill + c > kWave) break;
+                    if ((v >> lane) & 1) {
+                        const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(v >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)v, 0u));
+                        risk[fill + r] = (uint16_t)(w * 64 + lane);
+                    }
+                    fill += c;
+                }
+                if (fill == 0) break;   // the remaining words are empty
+                w0 = w;
+                __syncthreads();
+                uint32_t bit = 0, p = 0;
                 Cand cd;
                 uint32_t total = 0;
                 bool need = false;
-                if ((v >> lane) & 1) {
+                if (lane < fill) {
+                    bit = risk[lane];
+                    p = bit >> 2;
+                    const uint32_t bw = bit >> 6;
+                    uint32_t jn = field(snbp, 0);
+                    nbw0 = jn != 0xFFFFu ? M[jn * a.nw + bw] : 0ull;
+                    jn = field(snbp, 1);
+                    nbw1 = jn != 0xFFFFu ? M[jn * a.nw + bw] : 0ull;
+                    jn = field(snbp, 2);
+                    nbw2 = jn != 0xFFFFu ? M[jn * a.nw + bw] : 0ull;
+                    jn = field(snbp, 3);
+                    nbw3 = jn != 0xFFFFu ? M[jn * a.nw + bw] : 0ull;
                     ++evaluated;
                     if (neighbours(j, p, bit & 3, cd)) {
                         total = score_total(j, p, PC, cd);
@@ -1091,6 +1119,7 @@ struct Wave {
                         need = last_level || total >= thr;
                     }
                 }
+                __syncthreads();
                 if (!__ballot(need)) continue;
                 if (!lists) {
                     score_lists(j);
