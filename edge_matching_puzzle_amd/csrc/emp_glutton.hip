@@ -857,6 +857,49 @@ struct Wave {
         return __ballot(hit) != 0;
     }
 
+    // affected() for slots i0 .. i0 + 3 at once (bit k: slot i0 + k; lanes
+    // 16k .. 16k + 15 take the mask words of slot i0 + k; nw <= 16).
+    __device__ uint32_t affected4(uint32_t i0) {
+        const uint32_t k = lane >> 4, w = lane & 15, i = i0 + k;
+        const bool in = i < n();
+        auto changed = [&](uint32_t j) { return ((ch_prev[j >> 6] | ch_cur[j >> 6]) >> (j & 63)) & 1; };
+        bool hit = false;
+        uint32_t nb[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+        if (in) {
+            const uint32_t pos = pos_of_slot[i], x = pos % a.W;
+            hit = changed(i);
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l) {
+                const bool inside = l == 0 ? pos >= a.W : (l == 1 ? x + 1 < a.W : (l == 2 ? pos + a.W < a.P : x > 0));
+                const uint32_t q = (uint32_t)((int)pos + off(l));
+                nb[l] = (inside && is_free(q)) ? (uint32_t)slot_of_pos[q] : 0xFFFFFFFFu;
+                hit |= nb[l] != 0xFFFFFFFFu && changed(nb[l]);
+            }
+        }
+        if (in && !hit && w < a.nw) {
+            auto spread = [](uint64_t v16) {
+                uint64_t v = v16 & 0xFFFF;
+                v = (v | (v << 24)) & 0x000000FF000000FFull;
+                v = (v | (v << 12)) & 0x000F000F000F000Full;
+                v = (v | (v << 6)) & 0x0303030303030303ull;
+                v = (v | (v << 3)) & 0x1111111111111111ull;
+                return v * 0xF;
+            };
+            const uint32_t sh = 16 * (w & 3);
+            const uint64_t dm = spread(((d_prev[w >> 2] & crit_set[w >> 2]) >> sh));
+            const uint64_t em = spread(e_single[w >> 2] >> sh);
+            const uint64_t own = M[i * a.nw + w];
+            uint64_t near = own;
+#pragma unroll
+            for (uint32_t l = 0; l < 4; ++l)
+                if (nb[l] != 0xFFFFFFFFu) near |= M[nb[l] * a.nw + w];
+            hit = (near & dm) != 0 || (own & em) != 0;
+        }
+        const uint64_t m = __ballot(hit);
+        return ((m & 0xFFFFull) != 0) | ((m >> 16 & 0xFFFFull) != 0) << 1 | ((m >> 32 & 0xFFFFull) != 0) << 2 |
+               ((m >> 48) != 0) << 3;
+    }
+
     // First removal round after a search move (piece p at position s) from
     // masks that were a fixpoint: a candidate of slot i can only have lost
     // its validity when slot i lies within 2 steps of s (the neighbours of s
@@ -949,8 +992,21 @@ struct Wave {
             // a piece already without option makes every candidate invalid
             const bool all = (first && !push_known) || zero_piece;
             bool removed = false;
+            // later rounds: affected() four slots at a time, again after a
+            // slot that removed candidates (its changes may affect the rest)
+            const bool packed = !all && !pushed && a.nw <= 16;
+            uint32_t aff = 0, aff_base = 0, aff_n = 0;
             for (uint32_t i = 0; i < nn; ++i) {
-                if (!all && !(pushed ? push_affected(i) : affected(i))) continue;
+                if (packed) {
+                    if (i >= aff_base + aff_n) {
+                        aff = affected4(i);
+                        aff_base = i;
+                        aff_n = 4;
+                    }
+                    if (!((aff >> (i - aff_base)) & 1)) continue;
+                } else if (!all && !(pushed ? push_affected(i) : affected(i))) {
+                    continue;
+                }
                 slot_context(i);
                 slot_words(i);
                 // The slot's candidates in batches of up to 64, whole mask
@@ -1017,6 +1073,7 @@ struct Wave {
                     __syncthreads();
                 }
                 __syncthreads();
+                if (packed && ((ch_cur[i >> 6] >> (i & 63)) & 1)) aff_n = 0;
             }
             first = false;
             push_known = false;
