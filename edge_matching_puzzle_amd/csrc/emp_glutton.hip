@@ -361,16 +361,57 @@ struct Wave {
         // Per-piece totals, and the list of available pieces with at most 20
         // options: a move removes at most 4 options from each of 5 cells, so
         // only those pieces can be left without option.
+        // T: lane (word w, slot group g) sums the per-piece option counts
+        // of word w over the slots j = g mod 4, as nibble popcounts in 8-bit
+        // fields (even and odd pieces; at most 63 slots before the fields
+        // go into 16-bit sums), then the four groups are added across lanes.
+        {
+            const uint32_t w = lane & 15, g = lane >> 4;
+            uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;   // 16-bit sums of pieces 16w + 4h + r
+            if (w < a.nw) {
+                for (uint32_t j0 = g; j0 < nn; j0 += 4 * 63) {
+                    uint64_t ev = 0, od = 0;
+                    const uint32_t j1 = j0 + 4 * 63 < nn ? j0 + 4 * 63 : nn;
+                    for (uint32_t j = j0; j < j1; j += 4) {
+                        const uint64_t v = M[j * a.nw + w];
+                        uint64_t c = v - ((v >> 1) & 0x5555555555555555ull);
+                        c = (c & 0x3333333333333333ull) + ((c >> 2) & 0x3333333333333333ull);
+                        ev += c & 0x0F0F0F0F0F0F0F0Full;
+                        od += (c >> 4) & 0x0F0F0F0F0F0F0F0Full;
+                    }
+                    s0 += ev & 0x00FF00FF00FF00FFull;
+                    s2 += (ev >> 8) & 0x00FF00FF00FF00FFull;
+                    s1 += od & 0x00FF00FF00FF00FFull;
+                    s3 += (od >> 8) & 0x00FF00FF00FF00FFull;
+                }
+            }
+            auto sum_groups = [&](uint64_t v) {
+                for (int o = 16; o <= 32; o <<= 1) {
+                    const uint32_t lo32 = (uint32_t)__shfl_xor((int)(uint32_t)v, o);
+                    const uint32_t hi32 = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), o);
+                    v += (uint64_t)hi32 << 32 | lo32;
+                }
+                return v;
+            };
+            s0 = sum_groups(s0);
+            s1 = sum_groups(s1);
+            s2 = sum_groups(s2);
+            s3 = sum_groups(s3);
+            // lane (w, g) writes pieces 16w + 4g .. 16w + 4g + 3
+            if (w < a.nw) {
+                const uint32_t q = 16 * w + 4 * g, sh = 16 * g;
+                T[q] = (uint16_t)(s0 >> sh);
+                T[q + 1] = (uint16_t)(s1 >> sh);
+                T[q + 2] = (uint16_t)(s2 >> sh);
+                T[q + 3] = (uint16_t)(s3 >> sh);
+            }
+        }
+        __syncthreads();
         uint32_t n_crit = 0;
         bool zero = false;
         for (uint32_t base = 0; base < a.P; base += kWave) {
             const uint32_t q = base + lane;
-            uint32_t t = 0;
-            if (q < a.P) {
-                const uint32_t w = q >> 4;
-                for (uint32_t j = 0; j < nn; ++j) t += nib(M[j * a.nw + w], q);
-                T[q] = (uint16_t)t;
-            }
+            const uint32_t t = q < a.P ? (uint32_t)T[q] : 0u;
             const bool c = q < a.P && t <= 20 && piece_avail(q);
             const uint64_t m = __ballot(c);
             if (c) crit[n_crit + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = (uint16_t)q;
