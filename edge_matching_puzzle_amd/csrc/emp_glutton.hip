@@ -310,6 +310,26 @@ struct Wave {
         return ((size_t)a.S * (a.S + 1) - (size_t)(a.S - l) * (a.S - l + 1)) / 2 * a.nw;
     }
 
+    // This lane's colour-mask rows for the colour-count entries lane and
+    // lane + 64 (entry e: side e / nc, colour e % nc); zero past nw words
+    // and for colour 0.
+    uint64_t cmr0[16], cmr1[16];
+    __device__ void load_colour_rows() {
+        const uint32_t nc = a.max_colour + 1;
+#pragma unroll
+        for (uint32_t r = 0; r < 2; ++r) {
+            const uint32_t e = r * kWave + lane, l = e / nc, c = e % nc;
+            const bool use = e < 4 * nc && c != 0;
+            const uint64_t *cm = a.cm + (use ? ((size_t)(c - 1) * 4 + l) * a.nw : 0);
+#pragma unroll
+            for (uint32_t w = 0; w < 16; ++w) {
+                const uint64_t v = use && w < a.nw ? cm[w] : 0ull;
+                if (r == 0) cmr0[w] = v;
+                else cmr1[w] = v;
+            }
+        }
+    }
+
     __device__ void init() {
         for (uint32_t k = lane; k < a.S * a.nw; k += kWave) M[k] = a.init_masks[k];
         for (uint32_t j = lane; j < a.S; j += kWave) {
@@ -756,10 +776,29 @@ struct Wave {
         }
         pc_i = pc[i];
         sp_i = sp[i];
-        // cnt[l][c] = popcount(M[j_l] & cm[c][l]); the 16 words (nw <= 16)
-        // are read without a guard (index clamped, extra words masked) so
-        // all the global reads of a lane are in flight together.
-        for (uint32_t e = lane; e < 4 * nc; e += kWave) {
+        // cnt[l][c] = popcount(M[j_l] & cm[c][l]).  Entries lane and lane + 64
+        // take their colour-mask rows from registers (loaded once per
+        // wave); further entries (more than 31 colours) read them, 16 words
+        // without a guard (index clamped, extra words masked) so all the
+        // global reads of a lane are in flight together.
+#pragma unroll
+        for (uint32_t r = 0; r < 2; ++r) {
+            const uint32_t e = r * kWave + lane;
+            if (e < 4 * nc) {
+                const uint32_t l = e / nc, c = e % nc, j = snb[l];
+                uint32_t v = 0;
+                if (c && j != 0xFFFFFFFFu) {
+                    const uint64_t *mj = M + j * a.nw;
+#pragma unroll
+                    for (uint32_t w = 0; w < 16; ++w) {
+                        const uint32_t ww = w < a.nw ? w : 0;
+                        v += (uint32_t)__popcll((r == 0 ? cmr0[w] : cmr1[w]) & mj[ww]);
+                    }
+                }
+                cnt[e] = (uint16_t)v;
+            }
+        }
+        for (uint32_t e = 2 * kWave + lane; e < 4 * nc; e += kWave) {
             const uint32_t l = e / nc, c = e % nc, j = snb[l];
             uint32_t v = 0;
             if (c && j != 0xFFFFFFFFu) {
@@ -1403,6 +1442,7 @@ constexpr uint32_t kSolved = 0, kExhausted = 1, kLimit = 2, kBadPrefix = 3, kInt
 __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(GlArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     Wave wv(a, lds, blockIdx.x);
+    wv.load_colour_rows();
     while (true) {
         uint32_t item = 0;
         if (wv.lane == 0) item = atomicAdd(a.next_item, 1u);
