@@ -268,33 +268,18 @@ struct Wave {
     __device__ uint32_t n() const { return a.S - level; }
     __device__ uint64_t nbw(uint32_t l) const { return l == 0 ? nbw0 : (l == 1 ? nbw1 : (l == 2 ? nbw2 : nbw3)); }
     __device__ uint32_t field(uint64_t packed, uint32_t l) const { return (uint32_t)(packed >> (16 * l)) & 0xFFFFu; }
-    // Lane w holds mask word w of slot i and of its free neighbours (nw <=
-    // 64), so the word loops read them from registers.
-    uint64_t my_mi, my_n0, my_n1, my_n2, my_n3;
+    // Lane w holds mask word w of slot i (nw <= 64); the candidate packing
+    // reads the words with readlane.  The neighbours' words (nbw0..3) are
+    // read per candidate from the masks.
+    uint64_t my_mi;
     __device__ void slot_words(uint32_t i) {
         const uint32_t wl = lane < a.nw ? lane : 0;
         my_mi = M[i * a.nw + wl];
-        uint32_t j = field(snbp, 0);
-        my_n0 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
-        j = field(snbp, 1);
-        my_n1 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
-        j = field(snbp, 2);
-        my_n2 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
-        j = field(snbp, 3);
-        my_n3 = j != 0xFFFFu ? M[j * a.nw + wl] : 0;
     }
     static __device__ uint64_t lane_word(uint64_t v, uint32_t w) {
         const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)w);
         const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)w);
         return (uint64_t)hi << 32 | lo;
-    }
-    // word w of slot i (as loaded by slot_words) and of its neighbours
-    __device__ uint64_t slot_word(uint32_t w) {
-        nbw0 = lane_word(my_n0, w);
-        nbw1 = lane_word(my_n1, w);
-        nbw2 = lane_word(my_n2, w);
-        nbw3 = lane_word(my_n3, w);
-        return lane_word(my_mi, w);
     }
     // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
     __device__ void count(uint64_t evaluated, bool scoring) {
