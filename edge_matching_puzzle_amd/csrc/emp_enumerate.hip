@@ -582,6 +582,16 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             finish();
             continue;
         }
+        // An item that could not split (the output queue was full) runs on
+        // in this pass; it still honours its puzzle's node limit, checked
+        // once per round against the puzzle's count plus its own.
+        if (!may_split && a.node_limit[puzzle] &&
+            __hip_atomic_load(&a.nodes[puzzle], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + nodes >=
+                a.node_limit[puzzle]) {
+            atomicOr(&a.truncated[puzzle], 1u);
+            finish();
+            continue;
+        }
         const bool drain = __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
         const uint64_t budget = drain ? a.drain_budget : a.step_budget;
         for (uint32_t r = 0; r < a.steps_per_round; ++r) {

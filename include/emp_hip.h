@@ -107,7 +107,8 @@ typedef struct {
                                   split, 0 = default                           */
     uint64_t target_items;     /* prefix-split until this many items, 0 = auto  */
     uint64_t node_limit;       /* per puzzle: stop taking its work items once
-                                  this many nodes were counted, 0 = none      */
+                                  this many nodes were counted (an item that
+                                  can no longer split stops too), 0 = none    */
     uint64_t max_solutions;    /* capacity of each boards[i] in solutions       */
     uint32_t use_graph;        /* capture each pass in a hipGraph               */
     uint32_t queue_items;      /* work queue capacity in items, 0 = sized from
