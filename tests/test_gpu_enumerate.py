@@ -215,3 +215,19 @@ def test_graph_replays_match_eager(gpu_context):
                 assert _sorted_boards(g["boards"]) == _sorted_boards(e["boards"])
         for (pw, ph, pc), e in zip(puzzles, eager):
             _check_same(e, o.enumerate_(pw, ph, pc, max_solutions=4096))
+
+
+def test_node_limit_holds_when_no_item_can_split(gpu_context):
+    """A work queue that only holds the root items: no item can split, and each
+    still stops at its puzzle's node limit.  The device deadline bounds the call
+    should that check fail, so a regression fails here instead of running on."""
+    from edge_matching_puzzle_amd.workloads import c2_puzzles
+    puzzles = [p for p in c2_puzzles(256) if p[0] >= 9][:8]   # trees far beyond the limit
+    assert len(puzzles) == 8
+    limit = 200_000
+    print("case: node limit with a full queue", flush=True)
+    res, st = gpu_context.enumerate(puzzles, max_solutions=4, node_limit=limit, step_budget=1 << 10,
+                                    queue_items=len(puzzles), time_limit_ms=30_000.0)
+    assert st["total_ms"] < 15_000
+    assert sum(not r["complete"] for r in res) >= 1
+    assert all(r["n_nodes"] >= limit for r in res if not r["complete"])
