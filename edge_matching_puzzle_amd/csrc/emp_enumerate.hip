@@ -730,6 +730,49 @@ __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
     if (i < n) p[i] = v;
 }
 
+// Work items grouped by puzzle (a counting sort on the header's puzzle
+// field), so that the waves of a CU work on few puzzles and their tables
+// stay in the caches.  Order within a puzzle is arbitrary; the enumeration
+// does not depend on the order of its items.
+__global__ void items_per_puzzle(const uint64_t *hdr, uint32_t n_items, unsigned int *count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_items) atomicAdd(&count[hdr_puzzle(hdr[i])], 1u);
+}
+
+// In-place exclusive prefix sum of n counts, one block of 1024 threads.
+__global__ void __launch_bounds__(1024) exclusive_scan_u32(unsigned int *v, uint32_t n) {
+    __shared__ unsigned int part[1024];
+    unsigned int carry = 0;
+    for (uint32_t base = 0; base < n; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const unsigned int x = i < n ? v[i] : 0u;
+        part[threadIdx.x] = x;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024; o <<= 1) {
+            const unsigned int y = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (i < n) v[i] = carry + part[threadIdx.x] - x;
+        const unsigned int total = part[1023];
+        __syncthreads();
+        carry += total;
+    }
+}
+
+__global__ void scatter_items(const uint64_t *hdr, const uint16_t *stack, uint32_t n_items, uint32_t stride,
+                              unsigned int *offset, uint64_t *out_hdr, uint16_t *out_stack) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const uint64_t h = hdr[i];
+    const uint32_t o = atomicAdd(&offset[hdr_puzzle(h)], 1u);
+    out_hdr[o] = h;
+    const uint16_t *src = stack + (size_t)i * stride;
+    uint16_t *dst = out_stack + (size_t)o * stride;
+    for (uint32_t j = 0; j < stride; ++j) dst[j] = src[j];
+}
+
 template <int W>
 void launch_expand(const DfsArgs &a, hipStream_t s) {
     const uint32_t threads = 256, blocks = (a.n_in + threads - 1) / threads;
@@ -983,6 +1026,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned long long *d_limit = ws.get<unsigned long long>("limit", n);
     unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
     unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
+    // batches read their tables through the caches: group the items by
+    // puzzle before each pass
+    const bool group_items = !lds_table && n > 1;
+    unsigned int *d_group = group_items ? ws.get<unsigned int>("group_count", n) : nullptr;
+    uint64_t *d_group_hdr = group_items ? ws.get<uint64_t>("group_hdr", cap) : nullptr;
+    uint16_t *d_group_stack = group_items ? ws.get<uint16_t>("group_stack", (size_t)cap * stride) : nullptr;
     unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
     unsigned int *d_depth = ws.get<unsigned int>("max_depth", n);
     const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
@@ -1128,6 +1177,18 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         // Expand while the queue is short of the target and expansions still
         // multiply it (resumed searches pass through an expansion unchanged).
         const bool expand = n_items < target && expansions < max_expansions && expansion_grows;
+        if (!expand && group_items && n_items > 1) {
+            const uint32_t ni = (uint32_t)n_items, g = (ni + 255) / 256;
+            hipLaunchKernelGGL(fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, d_group, n, 0u);
+            hipLaunchKernelGGL(items_per_puzzle, dim3(g), dim3(256), 0, stream, d_hdr[cur], ni, d_group);
+            hipLaunchKernelGGL(exclusive_scan_u32, dim3(1), dim3(1024), 0, stream, d_group, n);
+            hipLaunchKernelGGL(scatter_items, dim3(g), dim3(256), 0, stream, d_hdr[cur], d_stack[cur], ni, stride,
+                               d_group, d_group_hdr, d_group_stack);
+            EMP_CHECK(hipGetLastError());
+            EMP_CHECK(hipMemcpyAsync(d_hdr[cur], d_group_hdr, (size_t)ni * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_stack[cur], d_group_stack, (size_t)ni * stride * 2, hipMemcpyDeviceToDevice,
+                                     stream));
+        }
         if (!cfg.use_graph || expand)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
         if (expand) {
