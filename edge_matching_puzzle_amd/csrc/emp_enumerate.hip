@@ -1026,9 +1026,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned long long *d_limit = ws.get<unsigned long long>("limit", n);
     unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
     unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
-    // batches read their tables through the caches: group the items by
-    // puzzle before each pass
-    const bool group_items = !lds_table && n > 1;
+    // Batches of several tables read them through the caches: group the
+    // items by puzzle before each pass (roots of one table gain nothing).
+    // Not under a node limit: a puzzle's count grows only as its items
+    // finish, and grouped items are taken together, so the limit would be
+    // overshot many times over (48 node-limited C2 instances: 8x the nodes).
+    const bool group_items = !lds_table && n_tab > 1 && cfg.node_limit == 0;
     unsigned int *d_group = group_items ? ws.get<unsigned int>("group_count", n) : nullptr;
     uint64_t *d_group_hdr = group_items ? ws.get<uint64_t>("group_hdr", cap) : nullptr;
     uint16_t *d_group_stack = group_items ? ws.get<uint16_t>("group_stack", (size_t)cap * stride) : nullptr;
