@@ -155,6 +155,27 @@ def test_batch_of_uneven_sizes(gpu_context):
         _check_same(r, o.enumerate_(w, h, c, max_solutions=4096))
 
 
+@pytest.mark.parametrize("order,seed,shapes", [
+    ("spiral", 11, [(3, 3), (4, 3), (4, 4), (5, 4), (5, 5), (6, 4)]),
+    ("basic", 12, [(3, 4), (4, 3), (4, 5), (3, 5), (5, 4), (4, 4)])])
+def test_batch_of_identical_piece_puzzles(gpu_context, order, seed, shapes):
+    """One launch of 16 few-colour puzzles of mixed shapes: tables read
+    through the cache, work items grouped by puzzle, chains of identical
+    pieces in most tables (up to 4664 solutions and 5e5 nodes a puzzle)."""
+    rng = np.random.default_rng(seed)
+    puzzles = []
+    for s in range(16):
+        w, h = shapes[int(rng.integers(0, len(shapes)))]
+        B = int(rng.integers(1, 3))
+        I = int(rng.integers(2, 4))
+        puzzles.append((w, h, emp.generate(2000 + s, w, h, B, I)[0]))
+    print("case: identical-piece batch", order, flush=True)
+    res, _ = gpu_context.enumerate(puzzles, max_solutions=1 << 13, order=order)
+    oo = o.SPIRAL if order == "spiral" else o.BASIC
+    for (w, h, c), r in zip(puzzles, res):
+        _check_same(r, o.enumerate_(w, h, c, max_solutions=1 << 13, order=oo))
+
+
 def test_c1_full_size_against_golden(gpu_context, golden_dir):
     """The bench workload at full size (8x8, 4 border / 8 interior colours,
     seed 7: 3.7e11 nodes, about 15 s on the GPU) against the oracle's result
