@@ -62,6 +62,20 @@ def test_prefixed_searches(gpu_context, w, B, I, seed, length, budget):
         _check(res[i], boards[i], _oracle(w, w, c, row, budget))
 
 
+@pytest.mark.parametrize("w,h,B,I,seed,budget", [(6, 4, 2, 3, 1, 300), (7, 5, 2, 3, 2, 300),
+                                                 (8, 5, 3, 4, 3, 200), (5, 3, 1, 2, 4, 300),
+                                                 (9, 6, 2, 4, 5, 150), (4, 7, 2, 3, 6, 300),
+                                                 (6, 6, 1, 2, 7, 300), (10, 4, 2, 3, 8, 200)])
+def test_rectangles_and_few_colours(gpu_context, w, h, B, I, seed, budget):
+    """Wide and tall boards with few colours: many identical pieces, so the
+    most-constrained choice meets many ties.  Six solve, two stop at the
+    push budget."""
+    c, _ = emp.generate(seed, w, h, B, I)
+    print("case: glutton rectangle", w, h, B, I, seed, budget, flush=True)
+    (r,), boards, _ = gpu_context.glutton_max(w, h, c, push_budget=budget)
+    _check(r, boards[0], o.glutton_max(w, h, c, push_limit=budget))
+
+
 def test_bad_prefix_is_reported(gpu_context):
     w = h = 5
     c, _ = emp.generate(3, w, h, 3, 4)
