@@ -40,6 +40,22 @@ def test_synthetic_against_oracle(gpu_context, w, B, I, seed, order):
     assert list(r["board"]) == list(ref["board"])
 
 
+@pytest.mark.parametrize("w,h,B,I,seed,order", [(6, 4, 2, 3, 1, "spiral"), (5, 3, 1, 2, 2, "basic"),
+                                                (4, 6, 2, 3, 3, "basic"), (7, 4, 2, 4, 4, "spiral"),
+                                                (3, 7, 2, 3, 5, "basic"), (5, 4, 1, 3, 6, "spiral")])
+def test_rectangles_against_oracle(gpu_context, w, h, B, I, seed, order):
+    """Wide and tall few-colour boards; 5x3 with one border colour has no
+    solution, so its trace is the whole exhausted search (121,556 entries)."""
+    c, _ = emp.generate(seed, w, h, B, I)
+    print("case: backtrack rectangle", w, h, B, I, seed, order, flush=True)
+    r = gpu_context.backtrack_trace(w, h, c, order=order, trace_capacity=1 << 18)
+    ref = o.backtracker(w, h, c, order=o.BASIC if order == "basic" else o.SPIRAL, trace_capacity=1 << 18)
+    assert r["found"] == ref["found"]
+    assert r["trace_length"] == ref["trace_length"]
+    assert (r["trace"] == ref["trace"]).all()
+    assert list(r["board"]) == list(ref["board"])
+
+
 def test_selection_limit(gpu_context):
     c, _ = emp.generate(1, 6, 6, 3, 6)
     r = gpu_context.backtrack_trace(6, 6, c, max_selections=10)
