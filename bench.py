@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -149,10 +151,11 @@ def bench_c3(args, ctx):
     pushes = 0
     for _ in range(args.steps):
         # One launch per chunk of prefixes (long runs print as they go).
-        res, st = [], {}
+        res, st, boards = [], {}, []
         for lo in range(0, len(rows), chunk):
-            r, _, s = ctx.glutton_max(w, h, c, prefixes=rows[lo:lo + chunk], push_budget=args.push_budget)
+            r, b, s = ctx.glutton_max(w, h, c, prefixes=rows[lo:lo + chunk], push_budget=args.push_budget)
             res += r
+            boards.append(b)
             for key in ("pushes", "kernel_ms", "evaluations", "score_evaluations", "rounds"):
                 st[key] = st.get(key, 0) + s[key]
             st["waves"], st["lds_bytes"] = s["waves"], s["lds_bytes"]
@@ -160,11 +163,15 @@ def bench_c3(args, ctx):
                 print("prefixes %d..%d: %d pushes, max depth %d" % (lo, lo + len(r) - 1, s["pushes"],
                                                                     max(x["max_depth"] for x in r)),
                       file=sys.stderr, flush=True)
+        boards = np.concatenate(boards)
         pushes += st["pushes"]
         print("step done: %d pushes, %.1f ms kernel, %.3g evaluations" % (st["pushes"], st["kernel_ms"], st["evaluations"]),
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     depths = [r["max_depth"] for r in res]
+    invalid = _invalid_glutton_boards(w, h, c, rows, res, boards)
+    if invalid:
+        raise SystemExit("glutton_max returned %d invalid boards (first: prefix %d)" % (len(invalid), invalid[0]))
     return {
         "metric": METRIC, "value": pushes / elapsed, "unit": "glutton nodes (pushes)/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
@@ -172,6 +179,7 @@ def bench_c3(args, ctx):
         "config": {"workload": "C3 glutton_max: 16x16, 5 border / 17 interior colours, seed 3, centre hint + 24 "
                                "spiral moves, %d prefixes, %d pushes each" % (len(rows), args.push_budget),
                    "max_depth": max(depths), "prefixes_depth_ge_200": sum(1 for d in depths if d >= 200),
+                   "valid_boards": len(depths),
                    "mean_depth": sum(depths) / len(depths), "evaluations_per_s": st["evaluations"] / (st["kernel_ms"] * 1e-3),
                    "evaluations_per_push": st["evaluations"] / max(st["pushes"], 1),
                    "scoring_share": st["score_evaluations"] / max(st["evaluations"], 1),
@@ -180,6 +188,18 @@ def bench_c3(args, ctx):
         "roofline": None,
         "cpu_baseline": None if args.no_cpu else _cpu_c3(w, h, c, rows[0]),
     }
+
+
+def _invalid_glutton_boards(w, h, c, rows, res, boards):
+    """Indices of deepest situations that break the puzzle's edge rules, drop
+    a prefix move or disagree with the reported depth / status (checked from
+    the puzzle alone, tests/board_check.py)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from board_check import invalid_boards
+    import edge_matching_puzzle_amd as emp
+    depth = np.array([r["max_depth"] for r in res])
+    solved = np.array([r["status"] == emp.GLUTTON_SOLVED for r in res])
+    return [int(i) for i in invalid_boards(w, h, c, boards, prefixes=rows, depths=depth, solved=solved)]
 
 
 def _cpu_c3(w, h, c, row):

@@ -1,13 +1,17 @@
 """GPU glutton_max (through the C ABI) against the CPU oracle's restatement.
 
 Bar: same status, same max depth, same number of pushes and the same board
-(the solution, or the deepest situation) for every search.
+(the solution, or the deepest situation) for every search; and every board
+valid by the puzzle's edge rules alone (tests/board_check.py: edges match,
+frame sides face the frame, each piece once, the prefix kept, depth = pieces
+on the board, solved = full), which does not rest on the oracle.
 """
 import numpy as np
 import pytest
 
 import edge_matching_puzzle_amd as emp
 import oracle_bindings as o
+from board_check import board_problems, invalid_boards
 
 pytestmark = pytest.mark.gpu
 
@@ -19,6 +23,15 @@ def _unpack(row):
 def _oracle(w, h, c, row, budget):
     moves = [v for t in _unpack(row) for v in t]
     return o.glutton_max(w, h, c, prefix=moves, push_limit=budget)
+
+
+def _valid(w, h, c, gpu, board, prefix=()):
+    problems = board_problems(w, h, c, board, prefix_moves=prefix, depth=gpu["max_depth"],
+                              solved=gpu["status"] == emp.GLUTTON_SOLVED)
+    assert not problems, problems[:4]
+    if gpu["max_depth"] < len(prefix):
+        # a prefix whose last moves leave a cell without option is not searched
+        assert gpu["status"] == emp.GLUTTON_EXHAUSTED and gpu["pushes"] == 0
 
 
 def _check(gpu, board, ref):
@@ -37,6 +50,7 @@ def test_known_answers(gpu_context, example_puzzles, expected_stdout, name):
     (r,), boards, st = gpu_context.glutton_max(w, h, c)
     assert r["status"] == emp.GLUTTON_SOLVED
     assert emp.situation_string(w, h, boards[0]) in expected_stdout["CUDA_glutton_max_" + name]["strings"]
+    _valid(w, h, c, r, boards[0])
     _check(r, boards[0], o.glutton_max(w, h, c))
 
 
@@ -46,6 +60,7 @@ def test_synthetic_empty_start(gpu_context):
         c, _ = emp.generate(seed, w, h, 3, 4)
         print("case: glutton 5x5 seed", seed, flush=True)
         (r,), boards, _ = gpu_context.glutton_max(w, h, c, push_budget=400)
+        _valid(w, h, c, r, boards[0])
         _check(r, boards[0], o.glutton_max(w, h, c, push_limit=400))
 
 
@@ -59,6 +74,7 @@ def test_prefixed_searches(gpu_context, w, B, I, seed, length, budget):
     print("case: glutton prefixed", w, B, I, seed, len(rows), flush=True)
     res, boards, st = gpu_context.glutton_max(w, w, c, prefixes=rows, push_budget=budget)
     for i, row in enumerate(rows):
+        _valid(w, w, c, res[i], boards[i], row)
         _check(res[i], boards[i], _oracle(w, w, c, row, budget))
 
 
@@ -73,6 +89,7 @@ def test_rectangles_and_few_colours(gpu_context, w, h, B, I, seed, budget):
     c, _ = emp.generate(seed, w, h, B, I)
     print("case: glutton rectangle", w, h, B, I, seed, budget, flush=True)
     (r,), boards, _ = gpu_context.glutton_max(w, h, c, push_budget=budget)
+    _valid(w, h, c, r, boards[0])
     _check(r, boards[0], o.glutton_max(w, h, c, push_limit=budget))
 
 
@@ -85,19 +102,53 @@ def test_bad_prefix_is_reported(gpu_context):
     assert r["status"] == emp.GLUTTON_BAD_PREFIX
 
 
-def test_c3_shape_against_golden(gpu_context, golden_dir):
-    """16x16, 5 border / 17 interior colours, centre hint + 24 spiral moves,
-    two pushes; expected results from tests/golden/make_glutton_16x16_golden.py
-    (the oracle needs about half a minute per push at this size)."""
+def _golden_16x16(golden_dir):
     import json
     import os
-    g = json.load(open(os.path.join(golden_dir, "glutton_16x16_seed3.json")))
+    return json.load(open(os.path.join(golden_dir, "glutton_16x16_seed3.json")))
+
+
+def test_c3_shape_against_golden(gpu_context, golden_dir):
+    """16x16, 5 border / 17 interior colours, centre hint + 24 spiral moves:
+    the first 16 start situations of the benchmark, 16 pushes each; expected
+    results from tests/golden/make_glutton_16x16_golden.py (the oracle needs
+    20-40 s per push at this size)."""
+    g = _golden_16x16(golden_dir)
     w = h = 16
     c, _ = emp.generate(g["seed"], w, h, g["border_colours"], g["interior_colours"])
     rows = np.array([s["prefix"] for s in g["searches"]], dtype=np.uint32)
+    assert len(rows) >= 8 and g["push_budget"] >= 16
     print("case: glutton 16x16 golden", flush=True)
     res, boards, st = gpu_context.glutton_max(w, h, c, prefixes=rows, push_budget=g["push_budget"])
     for i, s in enumerate(g["searches"]):
         ref = {"max_depth": s["max_depth"], "pushes": s["pushes"], "solved": s["solved"],
                "status": o.LIMIT_REACHED if s["budget_reached"] else o.OK, "board": s["board"]}
+        _valid(w, h, c, res[i], boards[i], rows[i])
         _check(res[i], boards[i], ref)
+
+
+def test_c3_shape_property_run(gpu_context, golden_dir):
+    """The benchmark's C3 launch (4,096 start situations, 16 pushes each):
+    every deepest situation is valid, holds its whole prefix and at least
+    one push beyond it or reports the prefix dead, and the searches the
+    golden covers give the golden's results inside the full launch."""
+    from edge_matching_puzzle_amd.workloads import c3_prefixes, c3_puzzle
+    w, h, c, _ = c3_puzzle()
+    rows = c3_prefixes(4096)
+    print("case: glutton 16x16 property run, %d prefixes" % len(rows), flush=True)
+    res, boards, st = gpu_context.glutton_max(w, h, c, prefixes=rows, push_budget=16)
+    depth = np.array([r["max_depth"] for r in res])
+    status = np.array([r["status"] for r in res])
+    pushes = np.array([r["pushes"] for r in res])
+    bad = invalid_boards(w, h, c, boards, prefixes=rows, depths=depth, solved=status == emp.GLUTTON_SOLVED)
+    assert len(bad) == 0, [board_problems(w, h, c, boards[i], prefix_moves=rows[i]) for i in bad[:3]]
+    dead = depth < rows.shape[1]
+    assert ((status[dead] == emp.GLUTTON_EXHAUSTED) & (pushes[dead] == 0)).all()
+    assert (status != emp.GLUTTON_BAD_PREFIX).all()
+    assert ((status[~dead] == emp.GLUTTON_BUDGET) | (status[~dead] == emp.GLUTTON_EXHAUSTED)).all()
+    assert (pushes <= 16).all() and st["pushes"] == int(pushes.sum())
+    g = _golden_16x16(golden_dir)
+    for i, s in enumerate(g["searches"]):
+        assert list(rows[i]) == s["prefix"]
+        assert (depth[i], int(pushes[i])) == (s["max_depth"], s["pushes"])
+        assert list(boards[i]) == s["board"]
