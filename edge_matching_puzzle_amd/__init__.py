@@ -52,7 +52,8 @@ class EnumStats(ctypes.Structure):
                 ("expand_launches", ctypes.c_uint64), ("items_processed", ctypes.c_uint64),
                 ("splits", ctypes.c_uint64), ("max_words", ctypes.c_uint32),
                 ("lds_tables", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
-                ("threads", ctypes.c_uint32), ("candidate_sets", ctypes.c_uint64)]
+                ("threads", ctypes.c_uint32), ("candidate_sets", ctypes.c_uint64),
+                ("group_ms", ctypes.c_double)]
 
 
 class GluttonOptions(ctypes.Structure):
@@ -163,6 +164,13 @@ def validate(width, height, colours):
     colours = np.ascontiguousarray(colours, dtype=np.uint32)
     pz = Puzzle(width, height, colours.ctypes.data)
     _check(lib().emp_hip_validate_puzzle(ctypes.byref(pz)))
+
+
+def kind_counts(colours):
+    """Pieces of each kind (centre, border, corner: the number of frame
+    sides, include/emp_piece.h:126-131)."""
+    zeros = (np.asarray(colours) == 0).sum(axis=1)
+    return [int((zeros == k).sum()) for k in range(3)]
 
 
 def pack_move(pos, piece, orientation):
@@ -381,6 +389,10 @@ class Context:
             rk = [np.ascontiguousarray(r, dtype=np.uint16) for r in ranks]
             if len(rk) != n:
                 raise ValueError("one rank table per puzzle")
+            for i, ((w, h, c), r) in enumerate(zip(puzzles, rk)):
+                T = 4 * max(kind_counts(c))
+                if r.shape != (w * h, T):
+                    raise ValueError("rank table %d has shape %s, expected (%d, %d)" % (i, r.shape, w * h, T))
             rank_ptrs = (ctypes.c_void_p * n)(*[r.ctypes.data for r in rk])
             _check(lib().emp_hip_enumerate_ranked(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt), rank_ptrs,
                                                   ctypes.byref(res), board_ptrs, ctypes.byref(st)))

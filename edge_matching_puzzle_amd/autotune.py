@@ -16,9 +16,9 @@ DEFAULT_SPACE = {
     "waves_per_block": [2, 4],
     "steps_per_round": [16, 32, 64],
     "target_per_lane": [2, 4, 8],
-    # 2^14 splits so often under a node limit that the limit is overshot by far
-    # (4096 C2 instances: 104 s against 2.6 s at 2^16)
-    "step_budget": [1 << 16, 1 << 18],
+    # an item over its budget always requeues what remains (only the split-off
+    # second half needs free queue room), so small budgets stay bounded
+    "step_budget": [1 << 14, 1 << 16, 1 << 18],
 }
 
 

@@ -206,6 +206,20 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
             uint32_t tmax = 0;
             for (int k = 0; k < 3; ++k) tmax = std::max<uint32_t>(tmax, 4 * (uint32_t)list[i]->pieces_of(k).size());
             rank_tables.emplace_back(ranks[i], ranks[i] + (size_t)list[i]->size() * tmax);
+            // Each row must be a permutation of 0..T-1: two equal ranks would
+            // make "above the last rank" skip a transition.
+            std::vector<uint8_t> seen(tmax);
+            for (uint32_t k = 0; k < list[i]->size(); ++k) {
+                std::fill(seen.begin(), seen.end(), 0);
+                const uint16_t *row = rank_tables.back().data() + (size_t)k * tmax;
+                for (uint32_t t = 0; t < tmax; ++t) {
+                    if (row[t] >= tmax || seen[row[t]])
+                        return fail(EMP_HIP_BAD_ARGUMENT, "rank table " + std::to_string(i) + " row " +
+                                                              std::to_string(k) + " is not a permutation of 0.." +
+                                                              std::to_string(tmax - 1));
+                    seen[row[t]] = 1;
+                }
+            }
         }
     }
     std::vector<emp::EnumRoot> roots;
@@ -262,6 +276,7 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
         stats->items_processed = st.items_processed;
         stats->splits = st.splits;
         stats->candidate_sets = st.candidate_sets;
+        stats->group_ms = st.group_ms;
         stats->max_words = st.max_words;
         stats->lds_tables = st.lds_tables;
         stats->blocks = st.blocks;

@@ -85,6 +85,7 @@ struct EnumStats {
     uint64_t dfs_launches = 0, expand_launches = 0, items_processed = 0, splits = 0;
     uint32_t max_words = 0, lds_tables = 0, blocks = 0, threads = 0;
     uint64_t candidate_sets = 0;   // DFS iterations (one candidate set each)
+    double group_ms = 0;           // grouping items by puzzle before the passes
 };
 
 // A search that starts from a given situation of puzzle `table`: the

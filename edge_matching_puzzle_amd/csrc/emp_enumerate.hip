@@ -65,7 +65,8 @@ struct DfsArgs {
     uint32_t out_cap;
     unsigned int *out_count;
     unsigned int *flags;           // [0] output overflow, [1] splits, [2] items, [3] drain,
-                                   // [4] identical pieces out of order (never cleared per pass)
+                                   // [4] second halves granted (dfs splits), [5] identical
+                                   // pieces out of order (never cleared per pass)
     // per puzzle
     unsigned long long *nodes;
     unsigned long long *solutions;
@@ -467,7 +468,6 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 
     bool active = false;
     bool queue_empty = false;
-    bool may_split = false;   // at most one failed reservation per item
     bool fresh = false;       // c not computed yet for the item's level
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0, depth = 0;
     int last = -1;
@@ -532,7 +532,6 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         steps = 0;
                         nodes = 0;
                         depth = k;
-                        may_split = true;
                         fresh = true;
                         if (RANKED) {
                             rank_base = a.ranks + a.rank_off[puzzle];
@@ -559,7 +558,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                             uint64_t &rv = avail[(r >> 4) * kWave + lane];
                             const uint64_t rm = 0xFull << ((r & 15) * 4);
                             const bool lower = ((avail[(q >> 4) * kWave + lane] >> ((q & 15) * 4)) & 0xF) != 0;
-                            if (lower && !(rv & rm)) atomicOr(&a.flags[4], 1u);
+                            if (lower && !(rv & rm)) atomicOr(&a.flags[5], 1u);
                             rv &= lower ? ~rm : ~0ull;
                         }
                         atomicMin(&a.first_tick[puzzle], (unsigned long long)wall_clock64());
@@ -579,16 +578,6 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             // out of time: the rest of this item is not searched
             atomicOr(&a.truncated[puzzle], 1u);
             atomicOr(a.timed_out, 1u);
-            finish();
-            continue;
-        }
-        // An item that could not split (the output queue was full) runs on
-        // in this pass; it still honours its puzzle's node limit, checked
-        // once per round against the puzzle's count plus its own.
-        if (!may_split && a.node_limit[puzzle] &&
-            __hip_atomic_load(&a.nodes[puzzle], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + nodes >=
-                a.node_limit[puzzle]) {
-            atomicOr(&a.truncated[puzzle], 1u);
             finish();
             continue;
         }
@@ -679,37 +668,35 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             k = stay ? k : tk;
             last = stay ? t : tl;
             cur_ab = stay ? cur_ab : desc_abase(d);
-            if (steps >= budget && may_split) {
-                // Split in two: (a) the candidates above the current choice at
-                // the floor index -- usually the bulk of what remains -- and
-                // (b) the search in progress at level k above `last`, with the
-                // floor one index deeper.  At the floor itself the item is
-                // simply requeued.  Each item splits at most once per pass, so
-                // a pass emits at most twice its input.
-                const uint32_t n = k > kmin ? 2u : 1u;
-                uint32_t base = 0xFFFFFFFFu;
-                if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-                    base = atomicAdd(a.out_count, n);
-                if (base != 0xFFFFFFFFu && (uint64_t)base + n <= a.out_cap) {
-                    uint16_t *da = a.out_stack + (size_t)base * a.stride;
-                    for (uint32_t q = 0; q < kmin; ++q) da[q] = S(q);
-                    a.out_hdr[base] = make_hdr(puzzle, kmin, kmin, k > kmin ? (int)S(kmin) : last);
-                    if (n == 2) {
-                        uint16_t *db = a.out_stack + (size_t)(base + 1) * a.stride;
-                        for (uint32_t q = 0; q < k; ++q) db[q] = S(q);
-                        a.out_hdr[base + 1] = make_hdr(puzzle, kmin + 1, k, last);
-                    }
-                    atomicAdd(&a.flags[1], 1u);
+            if (steps >= budget) {
+                // Requeue what remains: (b) the search in progress at level k
+                // above `last`, and, when k is above the floor and room is
+                // granted, split off (a) the candidates above the current
+                // choice at the floor index -- usually the bulk of what
+                // remains -- with (b)'s floor one index deeper.  Every input
+                // item requeues at most once per pass, so the output always
+                // has room for one item per input item; only the second
+                // halves are counted against the rest (out_cap - n_in).
+                uint32_t n = 1;
+                if (k > kmin && atomicAdd(&a.flags[4], 1u) < a.out_cap - n_in) n = 2;
+                const uint32_t base = atomicAdd(a.out_count, n);
+                if ((uint64_t)base + n > a.out_cap) {
+                    // cannot happen while n_in <= out_cap: reported as an error
+                    atomicOr(&a.flags[0], 1u);
                     finish();
                     break;
                 }
-                // No room: mark what was reserved inside the queue dead and
-                // finish this item here without further split attempts.
-                if (base != 0xFFFFFFFFu)
-                    for (uint64_t o = base; o < (uint64_t)base + n && o < a.out_cap; ++o)
-                        a.out_hdr[o] = make_hdr(0, kDeadItem, kDeadItem, -1);
-                atomicOr(&a.flags[0], 1u);
-                may_split = false;
+                uint16_t *db = a.out_stack + (size_t)base * a.stride;
+                for (uint32_t q = 0; q < k; ++q) db[q] = S(q);
+                a.out_hdr[base] = make_hdr(puzzle, n == 2 ? kmin + 1 : kmin, k, last);
+                if (n == 2) {
+                    uint16_t *da = a.out_stack + (size_t)(base + 1) * a.stride;
+                    for (uint32_t q = 0; q < kmin; ++q) da[q] = S(q);
+                    a.out_hdr[base + 1] = make_hdr(puzzle, kmin, kmin, (int)S(kmin));
+                }
+                atomicAdd(&a.flags[1], 1u);
+                finish();
+                break;
             }
         }
     }
@@ -1011,21 +998,27 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     }
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
-    // A pass emits at most two items per input item; a full queue only makes
-    // lanes finish their items without splitting.
-    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n * 4, 1u << 16});
-    const uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
-                                         : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
-    uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
-    uint16_t *d_stack[2] = {ws.get<uint16_t>("stack0", (size_t)cap * stride),
-                            ws.get<uint16_t>("stack1", (size_t)cap * stride)};
+    // Per-search buffers.
     unsigned long long *d_nodes = ws.get<unsigned long long>("nodes", n);
     unsigned long long *d_sols = ws.get<unsigned long long>("sols", n);
     unsigned long long *d_first = ws.get<unsigned long long>("first", n);
     unsigned long long *d_last = ws.get<unsigned long long>("last", n);
     unsigned long long *d_limit = ws.get<unsigned long long>("limit", n);
     unsigned int *d_trunc = ws.get<unsigned int>("trunc", n);
-    unsigned int *d_counters = ws.get<unsigned int>("counters", 8);   // [0] in_head [1] out_count [2..4] flags
+    // [0] in_head [1] out_count [2..7] flags
+    unsigned int *d_counters = ws.get<unsigned int>("counters", 8);
+    unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
+    unsigned int *d_depth = ws.get<unsigned int>("max_depth", n);
+    unsigned int *d_timed_out = ws.get<unsigned int>("timed_out", 1);
+    unsigned long long *d_now = ws.get<unsigned long long>("clock_now", 1);
+    // Counters saved before each expansion so that an expansion whose
+    // children do not fit can be undone and its items searched instead.
+    unsigned long long *d_nodes_save = ws.get<unsigned long long>("nodes_save", n);
+    unsigned long long *d_sols_save = ws.get<unsigned long long>("sols_save", n);
+    unsigned int *d_counters_save = ws.get<unsigned int>("counters_save", 8);
+    unsigned int *d_n_in = cfg.use_graph ? ws.get<unsigned int>("n_in", 1) : nullptr;
+    const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
+    uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
     // Batches of several tables read them through the caches: group the
     // items by puzzle before each pass (roots of one table gain nothing).
     // Not under a node limit: a puzzle's count grows only as its items
@@ -1033,12 +1026,19 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // overshot many times over (48 node-limited C2 instances: 8x the nodes).
     const bool group_items = !lds_table && n_tab > 1 && cfg.node_limit == 0;
     unsigned int *d_group = group_items ? ws.get<unsigned int>("group_count", n) : nullptr;
+
+    // Work queues.  A pass emits at most two items per input item and always
+    // has room for one; a full queue only stops items splitting in two.
+    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n * 4, 1u << 16});
+    const uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
+                                         : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
+    // prefix-split target within a quarter of the queue
+    const uint64_t expand_to = std::min<uint64_t>(target, std::max<uint64_t>(cap / 4, 1));
+    uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
+    uint16_t *d_stack[2] = {ws.get<uint16_t>("stack0", (size_t)cap * stride),
+                            ws.get<uint16_t>("stack1", (size_t)cap * stride)};
     uint64_t *d_group_hdr = group_items ? ws.get<uint64_t>("group_hdr", cap) : nullptr;
     uint16_t *d_group_stack = group_items ? ws.get<uint16_t>("group_stack", (size_t)cap * stride) : nullptr;
-    unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
-    unsigned int *d_depth = ws.get<unsigned int>("max_depth", n);
-    const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
-    uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
 
     // Root items: index 0 holds its first candidate (one node each), or the
     // given situations (their placements are not counted).
@@ -1091,7 +1091,6 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.ranks = d_ranks;
     a.rank_off = d_rank_off;
     a.rank_stride = d_rank_stride;
-    unsigned int *d_timed_out = ws.get<unsigned int>("timed_out", 1);
     a.timed_out = d_timed_out;
     {
         const unsigned int zero = 0;
@@ -1099,7 +1098,6 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     }
     if (cfg.time_limit_ms) {
         // the device clock counts 100 MHz ticks
-        unsigned long long *d_now = ws.get<unsigned long long>("clock_now", 1);
         hipLaunchKernelGGL(read_clock, dim3(1), dim3(1), 0, stream, d_now);
         unsigned long long now = 0;
         EMP_CHECK(hipMemcpyAsync(&now, d_now, 8, hipMemcpyDeviceToHost, stream));
@@ -1111,19 +1109,17 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         EMP_CHECK(hipMemcpyAsync(d_csets, &zero, 8, hipMemcpyHostToDevice, stream));
     }
 
-    hipEvent_t ev0, ev1;
+    hipEvent_t ev0, ev1, eg0, eg1;
     EMP_CHECK(hipEventCreate(&ev0));
     EMP_CHECK(hipEventCreate(&ev1));
+    EMP_CHECK(hipEventCreate(&eg0));
+    EMP_CHECK(hipEventCreate(&eg1));
     struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    } guard{ev0, ev1};
-
-    // Counters saved before each expansion so that an expansion whose
-    // children do not fit can be undone and its items searched instead.
-    unsigned long long *d_nodes_save = ws.get<unsigned long long>("nodes_save", n);
-    unsigned long long *d_sols_save = ws.get<unsigned long long>("sols_save", n);
-    unsigned int *d_counters_save = ws.get<unsigned int>("counters_save", 8);
+        hipEvent_t e[4];
+        ~EventGuard() {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } guard{{ev0, ev1, eg0, eg1}};
 
     const char *env = std::getenv("EMP_HIP_VERBOSE");
     const bool verbose = env && *env && *env != '0';
@@ -1144,9 +1140,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             if (pinned) (void)hipHostFree(pinned);
         }
     } graphs;
-    unsigned int *d_n_in = nullptr;
     if (cfg.use_graph) {
-        d_n_in = ws.get<unsigned int>("n_in", 1);
         EMP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&graphs.pinned), 8 * sizeof(unsigned int)));
         for (int dir = 0; dir < 2; ++dir) {
             DfsArgs ga = a;
@@ -1158,7 +1152,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             ga.n_in = 0;
             ga.n_in_dev = d_n_in;
             EMP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
             dispatch_dfs(maxw, ga, lds_table, blocks, threads, lds_bytes, stream);
             (void)hipMemcpyAsync(graphs.pinned, d_counters, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
             EMP_CHECK(hipStreamEndCapture(stream, &graphs.g[dir]));
@@ -1179,9 +1173,11 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         a.out_cap = cap;
         // Expand while the queue is short of the target and expansions still
         // multiply it (resumed searches pass through an expansion unchanged).
-        const bool expand = n_items < target && expansions < max_expansions && expansion_grows;
-        if (!expand && group_items && n_items > 1) {
+        const bool expand = n_items < expand_to && expansions < max_expansions && expansion_grows;
+        const bool grouped = !expand && group_items && n_items > 1;
+        if (grouped) {
             const uint32_t ni = (uint32_t)n_items, g = (ni + 255) / 256;
+            EMP_CHECK(hipEventRecord(eg0, stream));
             hipLaunchKernelGGL(fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, d_group, n, 0u);
             hipLaunchKernelGGL(items_per_puzzle, dim3(g), dim3(256), 0, stream, d_hdr[cur], ni, d_group);
             hipLaunchKernelGGL(exclusive_scan_u32, dim3(1), dim3(1024), 0, stream, d_group, n);
@@ -1191,9 +1187,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             EMP_CHECK(hipMemcpyAsync(d_hdr[cur], d_group_hdr, (size_t)ni * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_stack[cur], d_group_stack, (size_t)ni * stride * 2, hipMemcpyDeviceToDevice,
                                      stream));
+            EMP_CHECK(hipEventRecord(eg1, stream));
         }
         if (!cfg.use_graph || expand)
-            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
         if (expand) {
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
@@ -1219,6 +1216,11 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
         float ms = 0;
         EMP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        if (grouped) {
+            float gms = 0;
+            EMP_CHECK(hipEventElapsedTime(&gms, eg0, eg1));
+            stats.group_ms += gms;
+        }
         if (expand) {
             ++expansions;
             stats.expand_kernel_ms += ms;
@@ -1239,7 +1241,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             ++stats.dfs_launches;
             stats.splits += counters[3];
             stats.items_processed += counters[4];
-            if (counters[6])
+            if (counters[2]) throw DeviceError("work queue overflow in a search pass");
+            if (counters[7])
                 throw std::invalid_argument("a situation places an identical piece while a lower-numbered twin is "
                                             "still available (such a piece is never a candidate)");
         }
@@ -1343,7 +1346,7 @@ void expand_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, 
         a.out_hdr = d_hdr[1 - cur];
         a.out_stack = d_stack[1 - cur];
         a.out_cap = cap;
-        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 6u, 0u);
+        hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
         dispatch_expand(maxw, a, stream);
         EMP_CHECK(hipGetLastError());
         unsigned int counters[8];

@@ -104,15 +104,16 @@ typedef struct {
                                   pieces of one kind)                          */
     uint32_t steps_per_round;  /* steps between work refills, 0 = default       */
     uint64_t step_budget;      /* steps per work item per pass before it is
-                                  split, 0 = default                           */
+                                  requeued (and split in two while the queue
+                                  has room), 0 = default                      */
     uint64_t target_items;     /* prefix-split until this many items, 0 = auto  */
     uint64_t node_limit;       /* per puzzle: stop taking its work items once
-                                  this many nodes were counted (an item that
-                                  can no longer split stops too), 0 = none    */
+                                  this many nodes were counted, 0 = none      */
     uint64_t max_solutions;    /* capacity of each boards[i] in solutions       */
     uint32_t use_graph;        /* capture each pass in a hipGraph               */
     uint32_t queue_items;      /* work queue capacity in items, 0 = sized from
-                                  the grid (a full queue only stops splitting) */
+                                  the grid (at least the batch size; a full
+                                  queue only stops items splitting in two)    */
     double   time_limit_ms;    /* device time after which every search stops
                                   (results then incomplete), 0 = none         */
 } emp_hip_enum_options;
@@ -141,6 +142,9 @@ typedef struct {
     uint32_t blocks;           /* grid of the DFS kernel                       */
     uint32_t threads;          /* block size of the DFS kernel                 */
     uint64_t candidate_sets;   /* DFS iterations, one candidate set each       */
+    double   group_ms;         /* device time grouping the work items by puzzle
+                                  before the passes (batches without a node
+                                  limit), outside dfs_kernel_ms               */
 } emp_hip_enum_stats;
 
 void emp_hip_enum_default_options(emp_hip_enum_options *options);

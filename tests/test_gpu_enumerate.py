@@ -81,8 +81,8 @@ def test_forced_splits_and_tiny_budget(gpu_context):
 
 
 def test_full_queue_stops_splitting_and_stays_exact(gpu_context):
-    """Regression: with a tiny work queue most split reservations fail; the
-    lanes must finish their items unsplit and the result stay exact."""
+    """Regression: with a tiny work queue most second halves find no room;
+    the items requeue whole and the result stays exact."""
     w, h = 6, 6
     c, _ = emp.generate(1, w, h, 3, 6)
     print("case: tiny queue 6x6", flush=True)
@@ -239,9 +239,10 @@ def test_graph_replays_match_eager(gpu_context):
 
 
 def test_node_limit_holds_when_no_item_can_split(gpu_context):
-    """A work queue that only holds the root items: no item can split, and each
-    still stops at its puzzle's node limit.  The device deadline bounds the call
-    should that check fail, so a regression fails here instead of running on."""
+    """A work queue that only holds the root items: no item can split in two,
+    each requeues whole after its step budget and stops at its puzzle's node
+    limit.  The device deadline bounds the call should that fail, so a
+    regression fails here instead of running on."""
     from edge_matching_puzzle_amd.workloads import c2_puzzles
     puzzles = [p for p in c2_puzzles(256) if p[0] >= 9][:8]   # trees far beyond the limit
     assert len(puzzles) == 8
@@ -252,3 +253,33 @@ def test_node_limit_holds_when_no_item_can_split(gpu_context):
     assert st["total_ms"] < 15_000
     assert sum(not r["complete"] for r in res) >= 1
     assert all(r["n_nodes"] >= limit for r in res if not r["complete"])
+
+
+def test_queue_of_one_item_per_search_completes_exactly(gpu_context):
+    """Regression (serial tails): with room for one item per search and no
+    node limit, an item over its step budget still requeues what remains
+    each pass instead of searching its whole subtree on one lane; the
+    enumeration stays exact and the passes stay short."""
+    puzzles = _c0_puzzles(4, 300)
+    print("case: one queue slot per search, budget 2^10", flush=True)
+    res, st = gpu_context.enumerate(puzzles, step_budget=1 << 10, queue_items=len(puzzles), target_items=1)
+    assert st["dfs_launches"] > 4
+    for (w, h, c), r in zip(puzzles, res):
+        _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_small_step_budget_under_node_limit_is_bounded(gpu_context):
+    """The 2^14 step budget on node-limited C2 instances (once 40x slower
+    than 2^16 because items whose split found no room ran on serially): every
+    instance stops near its limit."""
+    from edge_matching_puzzle_amd.workloads import c2_puzzles
+    puzzles = c2_puzzles(256)
+    limit = 2_000_000
+    print("case: step budget 2^14 under a node limit", flush=True)
+    res, st = gpu_context.enumerate(puzzles, max_solutions=4, node_limit=limit, step_budget=1 << 14,
+                                    time_limit_ms=60_000.0)
+    assert st["total_ms"] < 20_000
+    assert all(r["complete"] or r["n_nodes"] >= limit for r in res)
+    for (w, h, c), r in list(zip(puzzles, res))[:24]:
+        if r["complete"]:
+            _check_same(r, o.enumerate_(w, h, c))
