@@ -91,6 +91,7 @@ SIGNATURES = {
     "emp_hip_situation_string": (_I32, [_U32, _U32, _P, ctypes.c_char_p, _U64]),
     "emp_hip_enum_default_options": (None, [_P]),
     "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+    "emp_hip_enumerate_noalloc": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
     "emp_hip_backtrack_trace": (_I32, [_P, _P, _U32, _U64, _P, _P, _P, _U64, _P]),
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
@@ -357,13 +358,16 @@ class Context:
 
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
-                  node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0, use_graph=False):
+                  node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0, use_graph=False,
+                  noalloc=False):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
         order of the candidates at each strategy index.  Returns (results,
         stats): one dict per puzzle with n_nodes, n_solutions, complete,
-        solve_ms, max_depth and boards ([n_stored, P] uint16 row-major)."""
+        solve_ms, max_depth and boards ([n_stored, P] uint16 row-major).
+        noalloc: through emp_hip_enumerate_noalloc (device memory from the
+        library's fixed scratch buffer, no allocation)."""
         n = len(puzzles)
         keep = []
         arr = (Puzzle * n)()
@@ -382,7 +386,12 @@ class Context:
         opt.use_graph = 1 if use_graph else 0
         res = (EnumResult * n)()
         st = EnumStats()
-        if ranks is None:
+        if noalloc:
+            if ranks is not None:
+                raise ValueError("the allocation-free enumeration takes no rank tables")
+            _check(lib().emp_hip_enumerate_noalloc(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
+                                                   ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+        elif ranks is None:
             _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
                                            ctypes.byref(res), board_ptrs, ctypes.byref(st)))
         else:

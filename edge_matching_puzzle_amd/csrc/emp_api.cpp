@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -24,9 +26,37 @@ struct emp_hip_context {
     emp::DeviceInfo dev;
     hipStream_t stream = nullptr;
     emp::Workspace workspace;
+    void *scratch = nullptr;   // device address of this device's copy of `scratch`
 };
 
+// Fixed device buffer of the allocation-free enumeration
+// (emp_hip_enumerate_noalloc).  Each device holds its own copy from module
+// load until the process ends.  In host code the name is only a stand-in for
+// the symbol: its device address is taken once per device with
+// hipGetSymbolAddress (scratch_address below), and only that address is ever
+// passed to kernels or copies.
+static __device__ __align__(256) uint8_t scratch[EMP_HIP_SCRATCH_BYTES];
+
 namespace {
+
+std::mutex g_scratch_mutex;
+std::map<int, void *> g_scratch_address;   // device -> address of its `scratch`
+
+// Address of `scratch` on the current device (which must be `device`),
+// looked up on the first call for that device.
+void *scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_address.find(device);
+    if (it != g_scratch_address.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_scratch_address[device] = ptr;
+    return ptr;
+}
 
 thread_local std::string g_last_error;
 
@@ -86,6 +116,11 @@ int emp_hip_create(int device, emp_hip_context **context) {
         ctx->dev.clock_khz = (uint32_t)prop.clockRate;
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
             return fail(EMP_HIP_DEVICE_ERROR, "hipStreamCreate failed");
+        ctx->scratch = scratch_address(device);
+        if (!ctx->scratch) {
+            (void)hipStreamDestroy(ctx->stream);
+            return fail(EMP_HIP_DEVICE_ERROR, "hipGetSymbolAddress of the scratch buffer failed");
+        }
         *context = ctx.release();
         return EMP_HIP_OK;
     });
@@ -180,7 +215,7 @@ namespace {
 int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
                     const emp_hip_enum_options *options, const uint16_t *const *ranks, const uint32_t *probe_moves,
                     uint32_t probe_len, uint32_t probe_rows, emp_hip_enum_result *results, uint16_t *const *boards,
-                    emp_hip_enum_stats *stats) {
+                    emp_hip_enum_stats *stats, emp::Workspace *arena = nullptr) {
     if (!context || !puzzles || !results || !n_puzzles) return fail(EMP_HIP_BAD_ARGUMENT, "null argument or empty batch");
     emp_hip_enum_options opt;
     emp_hip_enum_default_options(&opt);
@@ -250,7 +285,7 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
     cfg.solution_capacity = opt.max_solutions;
     std::vector<emp::EnumOutcome> out;
     emp::EnumStats st;
-    emp::enumerate_on_device(context->dev, context->stream, context->workspace, list, cfg, out, st,
+    emp::enumerate_on_device(context->dev, context->stream, arena ? *arena : context->workspace, list, cfg, out, st,
                              probe_moves ? &roots : nullptr, ranks ? &rank_tables : nullptr);
     for (uint32_t i = 0; i < (uint32_t)out.size(); ++i) {
         emp_hip_enum_result &r = results[i];
@@ -292,6 +327,22 @@ int emp_hip_enumerate(emp_hip_context *context, const emp_hip_puzzle *puzzles, u
                       uint16_t *const *boards, emp_hip_enum_stats *stats) {
     return guarded([&] {
         return run_enumeration(context, puzzles, n_puzzles, options, nullptr, nullptr, 0, 0, results, boards, stats);
+    });
+}
+
+int emp_hip_enumerate_noalloc(emp_hip_context *context, const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                              const emp_hip_enum_options *options, emp_hip_enum_result *results,
+                              uint16_t *const *boards, emp_hip_enum_stats *stats) {
+    return guarded([&] {
+        if (!context) return fail(EMP_HIP_BAD_ARGUMENT, "null context");
+        if (!context->scratch) return fail(EMP_HIP_DEVICE_ERROR, "no scratch buffer address for this device");
+        if (options && options->use_graph)
+            return fail(EMP_HIP_BAD_ARGUMENT, "use_graph allocates graph objects: not available without allocation");
+        // Blocks are carved from this device's scratch buffer for this call
+        // only; nothing is allocated or freed.
+        emp::Workspace arena(context->scratch, EMP_HIP_SCRATCH_BYTES);
+        return run_enumeration(context, puzzles, n_puzzles, options, nullptr, nullptr, 0, 0, results, boards, stats,
+                               &arena);
     });
 }
 

@@ -25,10 +25,13 @@ struct Unsupported : std::runtime_error {
 };
 
 // Grow-only cache of named device allocations, owned by a context so that
-// repeated searches allocate nothing once warm.
+// repeated searches allocate nothing once warm.  In arena form the blocks are
+// carved in order from one caller-provided device buffer (256-byte aligned),
+// nothing is allocated or freed, and a request beyond its end throws.
 class Workspace {
   public:
     Workspace() = default;
+    Workspace(void *arena, size_t bytes) : arena_(static_cast<char *>(arena)), arena_bytes_(bytes) {}
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
     ~Workspace();
@@ -37,6 +40,10 @@ class Workspace {
         return static_cast<T *>(raw(name, (count ? count : 1) * sizeof(T)));
     }
     void release();
+    bool is_arena() const { return arena_ != nullptr; }
+    // Bytes still free in the arena (SIZE_MAX when blocks are allocated).
+    size_t remaining() const { return arena_ ? arena_bytes_ - arena_used_ : SIZE_MAX; }
+    static constexpr size_t kAlign = 256;
 
   private:
     void *raw(const std::string &name, size_t bytes);
@@ -45,6 +52,8 @@ class Workspace {
         size_t bytes = 0;
     };
     std::map<std::string, Block> blocks_;
+    char *arena_ = nullptr;
+    size_t arena_bytes_ = 0, arena_used_ = 0;
 };
 
 struct DeviceInfo {

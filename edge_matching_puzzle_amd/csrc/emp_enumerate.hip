@@ -846,12 +846,27 @@ uint32_t round_words(uint32_t w) {
 Workspace::~Workspace() { release(); }
 
 void Workspace::release() {
-    for (auto &kv : blocks_)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    if (!arena_)
+        for (auto &kv : blocks_)
+            if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     blocks_.clear();
+    arena_used_ = 0;
 }
 
 void *Workspace::raw(const std::string &name, size_t bytes) {
+    if (arena_) {
+        Block &b = blocks_[name];
+        if (b.ptr && b.bytes >= bytes) return b.ptr;
+        const size_t need = (bytes + kAlign - 1) / kAlign * kAlign;
+        if (need > arena_bytes_ - arena_used_)
+            throw OutOfMemory("the search needs more than the " + std::to_string(arena_bytes_) +
+                              " bytes of its fixed scratch buffer (block '" + name + "' of " +
+                              std::to_string(bytes) + " bytes)");
+        b.ptr = arena_ + arena_used_;
+        b.bytes = need;
+        arena_used_ += need;
+        return b.ptr;
+    }
     Block &b = blocks_[name];
     if (b.ptr && b.bytes >= bytes) return b.ptr;
     if (b.ptr) (void)hipFree(b.ptr);
@@ -998,7 +1013,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     }
 
     const uint64_t target = cfg.target_items ? cfg.target_items : lanes * 4;
-    // Per-search buffers.
+    // Per-search buffers first, so that in arena form the work queues take
+    // whatever room is left.
     unsigned long long *d_nodes = ws.get<unsigned long long>("nodes", n);
     unsigned long long *d_sols = ws.get<unsigned long long>("sols", n);
     unsigned long long *d_first = ws.get<unsigned long long>("first", n);
@@ -1030,8 +1046,18 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // Work queues.  A pass emits at most two items per input item and always
     // has room for one; a full queue only stops items splitting in two.
     const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n * 4, 1u << 16});
-    const uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
-                                         : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
+    uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
+                                   : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
+    if (ws.is_arena()) {
+        // queue rows of header + stack, two queues (three when grouping)
+        const size_t row = 8 + (size_t)stride * 2, queues = group_items ? 3 : 2;
+        const size_t room = ws.remaining() > 8 * Workspace::kAlign ? ws.remaining() - 8 * Workspace::kAlign : 0;
+        const uint64_t fit = room / (row * queues);
+        if (fit < n)
+            throw OutOfMemory("the fixed scratch buffer leaves room for " + std::to_string(fit) +
+                              " work items per queue, fewer than the " + std::to_string(n) + " searches");
+        cap = (uint32_t)std::min<uint64_t>(cap, fit);
+    }
     // prefix-split target within a quarter of the queue
     const uint64_t expand_to = std::min<uint64_t>(target, std::max<uint64_t>(cap / 4, 1));
     uint64_t *d_hdr[2] = {ws.get<uint64_t>("hdr0", cap), ws.get<uint64_t>("hdr1", cap)};
@@ -1141,6 +1167,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
     } graphs;
     if (cfg.use_graph) {
+        if (ws.is_arena()) throw std::invalid_argument("graph replay allocates: not available with a fixed scratch buffer");
         EMP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&graphs.pinned), 8 * sizeof(unsigned int)));
         for (int dir = 0; dir < 2; ++dir) {
             DfsArgs ga = a;

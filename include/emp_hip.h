@@ -159,6 +159,29 @@ int emp_hip_enumerate(emp_hip_context *context,
                       emp_hip_enum_stats *stats);
 
 /*
+ * emp_hip_enumerate without any device allocation: the call's device memory
+ * (tables, work queues, counters, the solution store) is carved from a fixed
+ * buffer of EMP_HIP_SCRATCH_BYTES that each device holds from library load
+ * until the process ends, and no hipMalloc / hipFree or other allocating
+ * routine is called.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that
+ * forbid it).
+ *   - Results equal emp_hip_enumerate's; the work queues hold what fits in
+ *     the buffer (fewer items in flight, never a different result).
+ *   - A call needing more (many searches, a large max_solutions) fails with
+ *     EMP_HIP_OUT_OF_MEMORY before any search runs; use_graph is refused.
+ *   - Every call on one device shares that device's buffer and overwrites it:
+ *     calls that overlap on one device (two contexts, two threads) race.
+ */
+#define EMP_HIP_SCRATCH_BYTES (128u << 20)
+int emp_hip_enumerate_noalloc(emp_hip_context *context,
+                              const emp_hip_puzzle *puzzles, uint32_t n_puzzles,
+                              const emp_hip_enum_options *options,
+                              emp_hip_enum_result *results,
+                              uint16_t *const *boards,
+                              emp_hip_enum_stats *stats);
+
+/*
  * The reference's CUDA backtracker search (kernel_and_info,
  * src/CUDA_backtracker.cu:68-178): forward checking to the first solution,
  * candidates of a position in orientation-major order (lowest orientation,
