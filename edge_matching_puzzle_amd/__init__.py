@@ -74,6 +74,14 @@ class GluttonStats(ctypes.Structure):
 
 GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)
 
+NO_MOVE = 0xFFFFFFFF
+
+
+class Item(ctypes.Structure):
+    """emp_hip_item: a unit of remaining search (floor, level, last move)."""
+    _fields_ = [("floor", ctypes.c_uint32), ("level", ctypes.c_uint32), ("last", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
 _lib = None
 
 # Every symbol include/emp_hip.h declares, with (restype, argtypes).
@@ -92,6 +100,11 @@ SIGNATURES = {
     "emp_hip_enum_default_options": (None, [_P]),
     "emp_hip_enumerate": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
     "emp_hip_enumerate_noalloc": (_I32, [_P, _P, _U32, _P, _P, _P, _P]),
+    "emp_hip_enumerate_items": (_I32, [_P, _P, _P, _P, _P, _U32, _P, _P, _P, _P, _P, _U64, _P, _P]),
+    "emp_hip_resume_item": (_I32, [_P, _U32, _P, _P, _P, _P]),
+    "emp_hip_parse_situation": (_I32, [_U32, _U32, ctypes.c_char_p, _P]),
+    "emp_hip_results_bin_write": (_I32, [_U32, _U32, _U32, _U32, _P, _P, _U64, _U64, _P, _U64, _P]),
+    "emp_hip_results_bin_read": (_I32, [_P, _U64, _P, _P, _P, _P, _P, _P, _U64, _P, _P, _P]),
     "emp_hip_backtrack_trace": (_I32, [_P, _P, _U32, _U64, _P, _P, _P, _U64, _P]),
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
@@ -190,6 +203,67 @@ def spiral_prefixes(width, height, colours, length, count, seed, fixed=()):
                                          _ptr(fx) if len(fixed) else None, len(fixed), _ptr(out),
                                          ctypes.byref(n)))
     return out[:n.value].copy()
+
+
+def parse_situation(width, height, text):
+    """Row-major board (uint16, 0xFFFF empty) of a situation string
+    (include/situation_string_formatter.h:84-134)."""
+    board = np.zeros(width * height, dtype=np.uint16)
+    _check(lib().emp_hip_parse_situation(width, height, text.encode(), _ptr(board)))
+    return board
+
+
+def resume_item(width, height, colours, board, order="spiral"):
+    """The work item the reference resumes from a situation
+    (include/emp_strategy.h:418-467): (items [n, 3] uint32 floor/level/last,
+    moves [n, P] uint32) with n = 0 (the whole enumeration) or 1."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    b = np.ascontiguousarray(board, dtype=np.uint16)
+    it = Item()
+    moves = np.zeros(width * height, dtype=np.uint32)
+    n = ctypes.c_uint32()
+    _check(lib().emp_hip_resume_item(ctypes.byref(pz), _ORDERS[order], _ptr(b), ctypes.byref(it), _ptr(moves),
+                                     ctypes.byref(n)))
+    if n.value == 0:
+        return np.zeros((0, 3), dtype=np.uint32), np.zeros((0, width * height), dtype=np.uint32)
+    return np.array([[it.floor, it.level, it.last]], dtype=np.uint32), moves[None, :].copy()
+
+
+def results_bin_write(width, height, boards, node_index, total_nodes, order="spiral", solution_dump=True):
+    """results.bin v1 image (include/emp_situation_binary_dumper.h:72-126)."""
+    b = np.ascontiguousarray(np.asarray(boards, dtype=np.uint16).reshape(-1, width * height))
+    ni = np.ascontiguousarray(np.asarray(node_index, dtype=np.uint64).reshape(-1))
+    size = ctypes.c_uint64()
+    _check(lib().emp_hip_results_bin_write(width, height, _ORDERS[order], 1 if solution_dump else 0, _ptr(b),
+                                           _ptr(ni), b.shape[0], total_nodes, None, 0, ctypes.byref(size)))
+    out = np.zeros(size.value, dtype=np.uint8)
+    _check(lib().emp_hip_results_bin_write(width, height, _ORDERS[order], 1 if solution_dump else 0, _ptr(b),
+                                           _ptr(ni), b.shape[0], total_nodes, _ptr(out), out.size,
+                                           ctypes.byref(size)))
+    return out.tobytes()
+
+
+def results_bin_read(data):
+    """Decodes a results.bin v1 image: dict with width, height,
+    solution_dump, order [(x, y)], boards [n, P], node_index [n],
+    total_nodes, complete."""
+    buf = np.frombuffer(data, dtype=np.uint8).copy()
+    w, h, sd, n, total, comp = (ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64(),
+                                ctypes.c_uint64(), ctypes.c_uint32())
+    _check(lib().emp_hip_results_bin_read(_ptr(buf), buf.size, ctypes.byref(w), ctypes.byref(h), ctypes.byref(sd),
+                                          None, None, None, 0, ctypes.byref(n), ctypes.byref(total),
+                                          ctypes.byref(comp)))
+    P = w.value * h.value
+    xy = np.zeros(2 * P, dtype=np.uint32)
+    boards = np.zeros((n.value, P), dtype=np.uint16)
+    ni = np.zeros(n.value, dtype=np.uint64)
+    _check(lib().emp_hip_results_bin_read(_ptr(buf), buf.size, ctypes.byref(w), ctypes.byref(h), ctypes.byref(sd),
+                                          _ptr(xy), _ptr(boards), _ptr(ni), n.value, ctypes.byref(n),
+                                          ctypes.byref(total), ctypes.byref(comp)))
+    return {"width": w.value, "height": h.value, "solution_dump": bool(sd.value),
+            "order": [(int(xy[2 * i]), int(xy[2 * i + 1])) for i in range(P)], "boards": boards,
+            "node_index": ni, "total_nodes": total.value, "complete": bool(comp.value)}
 
 
 def order(width, height, kind="spiral"):
@@ -335,6 +409,57 @@ class Context:
         opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
         opt.queue_items = queue_items
         return opt
+
+    def enumerate_items(self, width, height, colours, items=None, moves=None, order="spiral", max_solutions=1024,
+                        ordered=False, time_limit_ms=0.0, rest_capacity=0, **launch):
+        """Enumeration of the union of work items (items [n, 3] uint32
+        floor/level/last, moves [n, P]; None: the whole enumeration).
+        ordered: solutions in the sequential search's order with their node
+        indices.  rest_capacity > 0 (with time_limit_ms or node_limit): the
+        work left when a limit cut the run comes back as rest_items /
+        rest_moves.
+        Returns a dict with n_nodes, n_solutions, complete, max_depth, boards,
+        node_index, rest_items, rest_moves and stats."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        P = width * height
+        # emp_hip_item is four u32: floor, level, last, reserved
+        if items is None or len(items) == 0:
+            n, it_arr, mv = 0, None, None
+        else:
+            items = np.asarray(items, dtype=np.uint32).reshape(-1, 3)
+            mv = np.ascontiguousarray(np.asarray(moves, dtype=np.uint32).reshape(-1, P))
+            if mv.shape[0] != items.shape[0]:
+                raise ValueError("one row of moves per item")
+            n = items.shape[0]
+            it_arr = np.zeros((n, 4), dtype=np.uint32)
+            it_arr[:, :3] = items
+        opt = self._options(order, max_solutions, launch.get("waves_per_cu", 0), launch.get("waves_per_block", 0),
+                            launch.get("steps_per_round", 0), launch.get("step_budget", 0),
+                            launch.get("target_items", 0), launch.get("node_limit", 0), launch.get("queue_items", 0))
+        opt.time_limit_ms = time_limit_ms
+        res = EnumResult()
+        boards = np.zeros((max(max_solutions, 1), P), dtype=np.uint16)
+        ni = np.zeros(max(max_solutions, 1), dtype=np.uint64)
+        st = EnumStats()
+        rest = np.zeros((rest_capacity, 4), dtype=np.uint32) if rest_capacity else None
+        rest_mv = np.zeros((rest_capacity, P), dtype=np.uint32) if rest_capacity else None
+        n_rest = ctypes.c_uint64()
+        _check(lib().emp_hip_enumerate_items(self._ctx, ctypes.byref(pz), ctypes.byref(opt), _ptr(it_arr),
+                                             _ptr(mv), n, ctypes.byref(res),
+                                             _ptr(boards) if max_solutions else None,
+                                             _ptr(ni) if ordered and max_solutions else None, _ptr(rest),
+                                             _ptr(rest_mv), rest_capacity,
+                                             ctypes.byref(n_rest) if rest_capacity else None, ctypes.byref(st)))
+        out = {"n_nodes": res.n_nodes, "n_solutions": res.n_solutions, "complete": bool(res.complete),
+               "max_depth": res.max_depth, "solve_ms": res.solve_ms, "boards": boards[:res.n_stored].copy(),
+               "node_index": ni[:res.n_stored].copy() if ordered else None,
+               "stats": {f: getattr(st, f) for f, _ in EnumStats._fields_}}
+        if rest_capacity:
+            m = n_rest.value
+            out["rest_items"] = rest[:m, :3].copy()
+            out["rest_moves"] = rest_mv[:m].copy()
+        return out
 
     def probe(self, width, height, colours, rows, node_limit, order="spiral", time_limit_ms=0.0, **launch):
         """One search per row of moves (strategy indices 0..L-1) with

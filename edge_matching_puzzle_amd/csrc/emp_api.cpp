@@ -19,6 +19,7 @@
 #include "emp_glutton.h"
 #include "emp_mlp.h"
 #include "emp_model.h"
+#include "emp_results.h"
 
 #include <unordered_set>
 
@@ -63,6 +64,52 @@ thread_local std::string g_last_error;
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
+}
+
+emp::EnumConfig config_from(const emp_hip_enum_options &opt) {
+    emp::EnumConfig cfg;
+    cfg.order = (int)opt.order;
+    cfg.waves_per_cu = opt.waves_per_cu;
+    cfg.waves_per_block = opt.waves_per_block;
+    cfg.steps_per_round = opt.steps_per_round;
+    cfg.step_budget = opt.step_budget;
+    cfg.target_items = opt.target_items;
+    cfg.node_limit = opt.node_limit;
+    cfg.queue_items = opt.queue_items;
+    cfg.time_limit_ms = opt.time_limit_ms;
+    cfg.use_graph = opt.use_graph != 0;
+    // Device-side store of up to max_solutions boards per search.
+    cfg.solution_capacity = opt.max_solutions;
+    return cfg;
+}
+
+// Internal form of a public work item (moves checked to be candidates).
+emp::EnumItem to_item(const emp::EnumTable &t, const emp::Puzzle &pz, const emp_hip_item &in, const uint32_t *moves) {
+    if (in.level >= pz.size() || in.floor < 1 || in.floor > in.level)
+        throw std::invalid_argument("a work item needs 1 <= floor <= level < P");
+    emp::EnumItem it;
+    it.floor = in.floor;
+    it.level = in.level;
+    emp::moves_to_stack(t, pz, moves, in.level, it.stack);
+    it.last = in.last == EMP_HIP_NO_MOVE ? -1 : (int)emp::move_to_transition(t, pz, in.level, in.last);
+    return it;
+}
+
+void copy_stats(const emp::EnumStats &st, emp_hip_enum_stats *stats) {
+    if (!stats) return;
+    stats->total_ms = st.total_ms;
+    stats->dfs_kernel_ms = st.dfs_kernel_ms;
+    stats->expand_kernel_ms = st.expand_kernel_ms;
+    stats->dfs_launches = st.dfs_launches;
+    stats->expand_launches = st.expand_launches;
+    stats->items_processed = st.items_processed;
+    stats->splits = st.splits;
+    stats->candidate_sets = st.candidate_sets;
+    stats->group_ms = st.group_ms;
+    stats->max_words = st.max_words;
+    stats->lds_tables = st.lds_tables;
+    stats->blocks = st.blocks;
+    stats->threads = st.threads;
 }
 
 template <typename F>
@@ -269,20 +316,8 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
             }
         }
     }
-    emp::EnumConfig cfg;
-    cfg.order = (int)opt.order;
-    cfg.waves_per_cu = opt.waves_per_cu;
-    cfg.waves_per_block = opt.waves_per_block;
-    cfg.steps_per_round = opt.steps_per_round;
-    cfg.step_budget = opt.step_budget;
-    cfg.target_items = opt.target_items;
-    cfg.node_limit = opt.node_limit;
-    cfg.queue_items = opt.queue_items;
     if (!(opt.time_limit_ms >= 0)) return fail(EMP_HIP_BAD_ARGUMENT, "time_limit_ms must be >= 0");
-    cfg.time_limit_ms = opt.time_limit_ms;
-    cfg.use_graph = opt.use_graph != 0;
-    // Device-side store of up to max_solutions boards per search.
-    cfg.solution_capacity = opt.max_solutions;
+    const emp::EnumConfig cfg = config_from(opt);
     std::vector<emp::EnumOutcome> out;
     emp::EnumStats st;
     emp::enumerate_on_device(context->dev, context->stream, arena ? *arena : context->workspace, list, cfg, out, st,
@@ -302,21 +337,7 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
         r.n_stored = (boards && boards[i]) ? keep : 0;
         if (boards && boards[i] && keep) std::memcpy(boards[i], out[i].boards.data(), keep * P * sizeof(uint16_t));
     }
-    if (stats) {
-        stats->total_ms = st.total_ms;
-        stats->dfs_kernel_ms = st.dfs_kernel_ms;
-        stats->expand_kernel_ms = st.expand_kernel_ms;
-        stats->dfs_launches = st.dfs_launches;
-        stats->expand_launches = st.expand_launches;
-        stats->items_processed = st.items_processed;
-        stats->splits = st.splits;
-        stats->candidate_sets = st.candidate_sets;
-        stats->group_ms = st.group_ms;
-        stats->max_words = st.max_words;
-        stats->lds_tables = st.lds_tables;
-        stats->blocks = st.blocks;
-        stats->threads = st.threads;
-    }
+    copy_stats(st, stats);
     return EMP_HIP_OK;
 }
 
@@ -343,6 +364,188 @@ int emp_hip_enumerate_noalloc(emp_hip_context *context, const emp_hip_puzzle *pu
         emp::Workspace arena(context->scratch, EMP_HIP_SCRATCH_BYTES);
         return run_enumeration(context, puzzles, n_puzzles, options, nullptr, nullptr, 0, 0, results, boards, stats,
                                &arena);
+    });
+}
+
+int emp_hip_enumerate_items(emp_hip_context *context, const emp_hip_puzzle *puzzle,
+                            const emp_hip_enum_options *options,
+                            const emp_hip_item *items, const uint32_t *item_moves, uint32_t n_items,
+                            emp_hip_enum_result *result, uint16_t *boards, uint64_t *node_index,
+                            emp_hip_item *rest, uint32_t *rest_moves, uint64_t rest_capacity,
+                            uint64_t *n_rest, emp_hip_enum_stats *stats) {
+    return guarded([&] {
+        if (!context || !puzzle || !result) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (n_items && (!items || !item_moves)) return fail(EMP_HIP_BAD_ARGUMENT, "null work items");
+        if (rest && (!rest_moves || !n_rest)) return fail(EMP_HIP_BAD_ARGUMENT, "null rest buffers");
+        emp_hip_enum_options opt;
+        emp_hip_enum_default_options(&opt);
+        if (options) opt = *options;
+        if (opt.order != EMP_HIP_ORDER_SPIRAL && opt.order != EMP_HIP_ORDER_BASIC)
+            return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        if (!(opt.time_limit_ms >= 0)) return fail(EMP_HIP_BAD_ARGUMENT, "time_limit_ms must be >= 0");
+        if (rest && !(opt.time_limit_ms > 0) && !opt.node_limit)
+            return fail(EMP_HIP_BAD_ARGUMENT, "the rest of a search needs a time or node limit");
+        if (node_index && opt.use_graph) return fail(EMP_HIP_BAD_ARGUMENT, "ordered results are not recorded from graph replays");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        const uint32_t P = pz->size();
+        const emp::EnumTable t = emp::build_enum_table(*pz, (int)opt.order);
+        emp::EnumWork work;
+        work.record_order = node_index != nullptr;
+        work.keep_rest = rest != nullptr;
+        for (uint32_t i = 0; i < n_items; ++i) {
+            try {
+                work.items.push_back(to_item(t, *pz, items[i], item_moves + (size_t)i * P));
+            } catch (const std::invalid_argument &e) {
+                return fail(EMP_HIP_BAD_ARGUMENT, "item " + std::to_string(i) + ": " + e.what());
+            }
+        }
+        std::vector<const emp::Puzzle *> list{pz.get()};
+        std::vector<emp::EnumOutcome> out;
+        emp::EnumStats st;
+        emp::enumerate_on_device(context->dev, context->stream, context->workspace, list, config_from(opt), out, st,
+                                 nullptr, nullptr, &work);
+        const emp::EnumOutcome &o = out[0];
+        const uint64_t keep = std::min<uint64_t>(o.boards.size() / P, opt.max_solutions);
+        result->n_nodes = o.nodes;
+        result->n_solutions = o.solutions;
+        result->complete = o.complete ? 1 : 0;
+        result->status = EMP_HIP_OK;
+        result->solve_ms = o.solve_ms;
+        result->max_depth = o.max_depth;
+        result->reserved = 0;
+        result->n_stored = boards ? keep : 0;
+        if (boards && keep) std::memcpy(boards, o.boards.data(), keep * P * sizeof(uint16_t));
+        if (node_index && boards && keep) std::memcpy(node_index, o.node_index.data(), keep * sizeof(uint64_t));
+        copy_stats(st, stats);
+        if (rest) {
+            *n_rest = work.rest.size();
+            const uint64_t m = std::min<uint64_t>(work.rest.size(), rest_capacity);
+            for (uint64_t i = 0; i < m; ++i) {
+                const emp::EnumItem &it = work.rest[i];
+                rest[i].floor = it.floor;
+                rest[i].level = it.level;
+                rest[i].last = it.last < 0 ? EMP_HIP_NO_MOVE : emp::transition_to_move(t, it.level, (uint32_t)it.last);
+                rest[i].reserved = 0;
+                uint32_t *row = rest_moves + (size_t)i * P;
+                for (uint32_t j = 0; j < P; ++j)
+                    row[j] = j < it.level ? emp::transition_to_move(t, j, it.stack[j]) : EMP_HIP_NO_MOVE;
+            }
+            if (work.rest.size() > rest_capacity)
+                return fail(EMP_HIP_OUT_OF_MEMORY, std::to_string(work.rest.size()) + " work items left, room for " +
+                                                       std::to_string(rest_capacity));
+        }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_resume_item(const emp_hip_puzzle *puzzle, uint32_t order, const uint16_t *board, emp_hip_item *item,
+                        uint32_t *moves, uint32_t *n_items) {
+    return guarded([&] {
+        if (!puzzle || !board || !item || !moves || !n_items) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC) return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        const uint32_t P = pz->size();
+        const emp::EnumTable t = emp::build_enum_table(*pz, (int)order);
+        // leading strategy positions holding a piece (set_initial_state stops
+        // at the first empty one)
+        std::vector<uint32_t> mv;
+        for (uint32_t i = 0; i < P; ++i) {
+            const uint32_t cell = t.order[i].second * puzzle->width + t.order[i].first;
+            if (board[cell] == 0xFFFF) break;
+            mv.push_back(cell | ((uint32_t)(board[cell] >> 2) << 8) | ((uint32_t)(board[cell] & 3) << 16));
+        }
+        if (mv.empty()) return fail(EMP_HIP_BAD_ARGUMENT, "the situation has no piece at the first strategy position");
+        std::vector<uint16_t> stack;
+        try {
+            emp::moves_to_stack(t, *pz, mv.data(), (uint32_t)mv.size(), stack);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, std::string("the situation is not one the search visits: ") + e.what());
+        }
+        if (mv.size() == 1) {
+            *n_items = 0;
+            return EMP_HIP_OK;
+        }
+        const uint32_t k = (uint32_t)mv.size();
+        item->floor = 1;
+        item->level = k - 1;
+        item->last = EMP_HIP_NO_MOVE;
+        item->reserved = 0;
+        for (uint32_t j = 0; j < P; ++j) moves[j] = j < k - 1 ? mv[j] : EMP_HIP_NO_MOVE;
+        *n_items = 1;
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_parse_situation(uint32_t width, uint32_t height, const char *text, uint16_t *board) {
+    return guarded([&] {
+        if (!text || !board || !width || !height) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        std::vector<uint16_t> b;
+        try {
+            b = emp::parse_situation(width, height, text);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        std::copy(b.begin(), b.end(), board);
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_results_bin_write(uint32_t width, uint32_t height, uint32_t order, uint32_t solution_dump,
+                              const uint16_t *boards, const uint64_t *node_index, uint64_t n_records,
+                              uint64_t total_nodes, uint8_t *out, uint64_t capacity, uint64_t *size) {
+    return guarded([&] {
+        if (!size || (n_records && !boards)) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC) return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        const auto xy = emp::make_order(width, height, (int)order);
+        if (!emp::is_permutation(xy, width, height))
+            return fail(EMP_HIP_UNSUPPORTED, "the order does not visit every cell once");
+        std::vector<uint8_t> bytes;
+        try {
+            bytes = emp::write_results_bin(width, height, xy, solution_dump != 0, boards, node_index, n_records,
+                                           total_nodes);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        *size = bytes.size();
+        if (out) std::memcpy(out, bytes.data(), std::min<uint64_t>(capacity, bytes.size()));
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_results_bin_read(const uint8_t *data, uint64_t size, uint32_t *width, uint32_t *height,
+                             uint32_t *solution_dump, uint32_t *xy, uint16_t *boards, uint64_t *node_index,
+                             uint64_t capacity, uint64_t *n_records, uint64_t *total_nodes, uint32_t *complete) {
+    return guarded([&] {
+        if (!data || !width || !height || !n_records) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        emp::ResultsBin r;
+        try {
+            r = emp::read_results_bin(data, size);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        const uint64_t P = (uint64_t)r.width * r.height, n = r.node_index.size(), m = std::min<uint64_t>(n, capacity);
+        *width = r.width;
+        *height = r.height;
+        *n_records = n;
+        if (solution_dump) *solution_dump = r.solution_dump ? 1 : 0;
+        if (total_nodes) *total_nodes = r.total_nodes;
+        if (complete) *complete = r.complete ? 1 : 0;
+        if (xy)
+            for (uint64_t i = 0; i < P; ++i) xy[2 * i] = r.order[i].first, xy[2 * i + 1] = r.order[i].second;
+        if (boards && m) std::memcpy(boards, r.boards.data(), m * P * sizeof(uint16_t));
+        if (node_index && m) std::memcpy(node_index, r.node_index.data(), m * sizeof(uint64_t));
+        return EMP_HIP_OK;
     });
 }
 

@@ -294,6 +294,27 @@ void stack_to_board(const EnumTable &t, const uint16_t *stack, uint16_t *board) 
     }
 }
 
+uint16_t move_to_transition(const EnumTable &t, const Puzzle &pz, uint32_t index, uint32_t mv) {
+    const EnumTableHeader h = t.header();
+    if (index >= h.P) throw std::invalid_argument("strategy index out of range");
+    const uint32_t pos = mv & 0xFF, piece = (mv >> 8) & 0xFF, o = (mv >> 16) & 3;
+    if (pos != t.order[index].second * h.W + t.order[index].first)
+        throw std::invalid_argument("move is not at strategy index " + std::to_string(index));
+    const int K = (int)(t.blob[h.off_pos + index] & 3);
+    for (uint32_t ki = 0; ki < t.piece_of[K].size(); ++ki)
+        if (t.piece_of[K][ki] == piece) return (uint16_t)(4 * ki + o);
+    (void)pz;
+    throw std::invalid_argument("move at strategy index " + std::to_string(index) + ": piece of another kind");
+}
+
+uint32_t transition_to_move(const EnumTable &t, uint32_t index, uint32_t tr) {
+    const EnumTableHeader h = t.header();
+    const int K = (int)(t.blob[h.off_pos + index] & 3);
+    const uint32_t piece = t.piece_of[K].at(tr >> 2);
+    const uint32_t pos = t.order[index].second * h.W + t.order[index].first;
+    return pos | (piece << 8) | ((tr & 3) << 16);
+}
+
 void moves_to_stack(const EnumTable &t, const Puzzle &pz, const uint32_t *moves, uint32_t len,
                     std::vector<uint16_t> &stack) {
     const EnumTableHeader h = t.header();

@@ -100,6 +100,14 @@ void stack_to_board(const EnumTable &table, const uint16_t *stack, uint16_t *boa
 void moves_to_stack(const EnumTable &table, const Puzzle &puzzle, const uint32_t *moves, uint32_t len,
                     std::vector<uint16_t> &stack);
 
+// Transition of the move (pos | piece << 8 | o << 16) at strategy index
+// `index`: the move must be at that index's position with a piece of its
+// kind (std::invalid_argument otherwise); it need not be a candidate.
+uint16_t move_to_transition(const EnumTable &table, const Puzzle &puzzle, uint32_t index, uint32_t move);
+
+// Move (pos | piece << 8 | o << 16) of transition `tr` at strategy index `index`.
+uint32_t transition_to_move(const EnumTable &table, uint32_t index, uint32_t tr);
+
 // Random spiral prefixes: up to `count` distinct rows, each the fixed moves
 // followed by a placement of strategy indices 0..length-1 in spiral order,
 // drawn by seeded random dives over matching pieces (pieces used by the fixed

@@ -87,6 +87,31 @@ struct EnumOutcome {
     bool complete = true;
     double solve_ms = 0;
     std::vector<uint16_t> boards;   // stored solutions, P cells each
+    // With ordered results: the stored solutions in the order of the
+    // sequential DFS, each with its node index there (the placements up to
+    // and including it; exact when the enumeration is complete).
+    std::vector<uint64_t> node_index;
+};
+
+// A work item of a search: the transitions placed at strategy indices
+// 0..level-1 (stack), resumed at index `level` above transition `last` (-1:
+// every candidate), ending when index `floor` has no candidate left -- the
+// state of the reference's DFS loop (include/emp_strategy.h:332-415) with
+// the loop stopping when it backtracks below `floor`.
+struct EnumItem {
+    uint32_t floor = 1, level = 1;
+    int last = -1;
+    std::vector<uint16_t> stack;
+};
+
+// One puzzle's enumeration given as work items, with ordered results and
+// resumable cuts.
+struct EnumWork {
+    std::vector<EnumItem> items;      // empty: the root item (index 0's first candidate)
+    bool record_order = false;        // node index of each stored solution in DFS order
+    bool keep_rest = false;           // at the deadline or the node limit, return the unfinished work in `rest`
+    uint64_t max_executions = 1u << 22;   // item executions an ordered run may record
+    std::vector<EnumItem> rest;       // out: what remains when a limit cut the run
 };
 
 struct EnumStats {
@@ -108,12 +133,14 @@ struct EnumRoot {
 // or, with `roots`, each given situation (outcomes per root; their own
 // placements are not counted).  With `ranks` (one table per puzzle, P rows of
 // 4 x (pieces of the largest kind) u16), candidates are visited in rank order
-// instead of index order; the set of situations visited is the same.
+// instead of index order; the set of situations visited is the same.  With
+// `work` (one puzzle, no roots or ranks), the search is the union of its
+// items (one outcome), optionally with ordered results and a resumable cut.
 void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
                          std::vector<EnumOutcome> &out, EnumStats &stats,
                          const std::vector<EnumRoot> *roots = nullptr,
-                         const std::vector<std::vector<uint16_t>> *ranks = nullptr);
+                         const std::vector<std::vector<uint16_t>> *ranks = nullptr, EnumWork *work = nullptr);
 
 // Breadth expansion (glutton_wide's play_from applied level by level,
 // include/CUDA_glutton_situations.h:713-757): every situation the DFS visits

@@ -88,6 +88,20 @@ struct DfsArgs {
     uint64_t deadline;             // wall_clock64 tick after which searches stop (0 = none)
     unsigned int *timed_out;       // set when the deadline cut the search
     const unsigned int *n_in_dev;  // when set, the input item count is read here (graph replays)
+    uint32_t keep_rest;            // past the deadline or the node limit, requeue the
+                                   // unfinished work instead of dropping it (timed_out
+                                   // then marks the cut)
+    // Ordered results.  exec_nodes[exec_base + i]: the nodes counted while
+    // input item i of this pass was searched; per stored solution (slot as in
+    // sol_stack), the execution that found it and the nodes that execution
+    // had counted when it placed the solution (the solution included).
+    // Expansions then leave the placements they make to the executions that
+    // follow them in the DFS (header field hdr_pending), so every node
+    // belongs to exactly one execution.
+    unsigned long long *exec_nodes;
+    uint32_t exec_base;
+    uint32_t *sol_exec;
+    unsigned long long *sol_local;
 };
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
@@ -147,6 +161,12 @@ __device__ __forceinline__ uint32_t hdr_floor(uint64_t h) { return (uint32_t)(h 
 __device__ __forceinline__ uint32_t hdr_level(uint64_t h) { return (uint32_t)(h >> 31) & 0x7FF; }
 __device__ __forceinline__ int hdr_last(uint64_t h) { return (int)((uint32_t)(h >> 42) & 0xFFF) - 1; }
 constexpr uint32_t kDeadItem = 0x7FF;   // floor of a slot reserved but never written
+// Bits 54-63 (ordered results only): placements not counted yet that come
+// right before the item in the DFS -- its own placement (stack[level - 1])
+// and, for the first child of an expansion, the parent's pending ones; the
+// item's execution counts them.
+__device__ __forceinline__ uint32_t hdr_pending(uint64_t h) { return (uint32_t)(h >> 54); }
+constexpr uint64_t kPendingMask = 0x3FFull << 54;
 
 // Read-only view of one table blob.  Per-kind fields are kept as three
 // scalars and picked with selects, so a runtime kind never indexes a private
@@ -335,8 +355,10 @@ __global__ void expand_kernel(DfsArgs a) {
     if (kmin == kDeadItem) return;
     const uint32_t puzzle = hdr_puzzle(hdr);
     const int last = hdr_last(hdr);
-    if (hdr_level(hdr) != kmin) {
-        // A resumed search is passed through unchanged.
+    TableView tv;
+    tv.load(a.tables + a.table_off[puzzle]);
+    const bool record = a.exec_nodes != nullptr;
+    auto pass_through = [&]() {
         if (__hip_atomic_load(&a.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
         const uint32_t o = atomicAdd(a.out_count, 1u);
         if (o >= a.out_cap) {
@@ -346,10 +368,14 @@ __global__ void expand_kernel(DfsArgs a) {
         const uint32_t lv = hdr_level(hdr);
         for (uint32_t j = 0; j < lv; ++j) a.out_stack[(size_t)o * a.stride + j] = a.in_stack[(size_t)idx * a.stride + j];
         a.out_hdr[o] = hdr;
+    };
+    if (hdr_level(hdr) != kmin || (record && kmin + 1 >= tv.P)) {
+        // A resumed search is passed through unchanged; so is, for ordered
+        // results, an item whose children would be solutions (their
+        // executions record them).
+        pass_through();
         return;
     }
-    TableView tv;
-    tv.load(a.tables + a.table_off[puzzle]);
     const uint16_t *stack = a.in_stack + (size_t)idx * a.stride;
     uint64_t avail[3][W];
 #pragma unroll
@@ -382,8 +408,13 @@ __global__ void expand_kernel(DfsArgs a) {
     uint32_t n = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) n += (uint32_t)__popcll(c[w]);
-    if (!n) return;
-    atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
+    if (!n) {
+        // nothing below: an item still holding uncounted placements is
+        // passed on, so that an execution counts them in their place
+        if (record && hdr_pending(hdr)) pass_through();
+        return;
+    }
+    if (!record) atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
     atomicMax(&a.max_depth[puzzle], kmin + 1);
     if (kmin == tv.P - 1) {
         uint64_t s = atomicAdd(&a.solutions[puzzle], (unsigned long long)n);
@@ -409,6 +440,9 @@ __global__ void expand_kernel(DfsArgs a) {
         return;
     }
     uint32_t o = base;
+    // ordered results: each child holds its own placement, and the first
+    // (lowest transition, first in the DFS) the parent's pending ones too
+    uint64_t pending = record ? (uint64_t)(hdr_pending(hdr) + 1) << 54 : 0ull;
     for (int w = 0; w < W; ++w) {
         uint64_t v = c[w];
         while (v) {
@@ -417,7 +451,8 @@ __global__ void expand_kernel(DfsArgs a) {
             uint16_t *dst = a.out_stack + (size_t)o * a.stride;
             for (uint32_t j = 0; j < kmin; ++j) dst[j] = stack[j];
             dst[kmin] = (uint16_t)t;
-            a.out_hdr[o] = make_hdr(puzzle, kmin + 1, kmin + 1, -1);
+            a.out_hdr[o] = make_hdr(puzzle, kmin + 1, kmin + 1, -1) | pending;
+            pending = record ? 1ull << 54 : 0ull;
             ++o;
         }
     }
@@ -470,6 +505,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     bool queue_empty = false;
     bool fresh = false;       // c not computed yet for the item's level
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0, depth = 0;
+    uint32_t item = 0;   // input index of the item (ordered results)
     int last = -1;
     uint32_t cur_ab = 0;   // available-region offset of level k's kind
     const uint16_t *rank_base = nullptr;   // ranked: this puzzle's rank rows
@@ -485,6 +521,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     auto A = [&](uint32_t ab, uint32_t w) -> uint64_t & { return avail[(ab + w) * kWave + lane]; };
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
+        if (a.exec_nodes) a.exec_nodes[a.exec_base + item] = nodes;
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
         atomicAdd(a.candidate_sets, (unsigned long long)steps);
         atomicMax(&a.max_depth[puzzle], depth);
@@ -515,13 +552,29 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                     kmin = hdr_floor(hdr);
                     puzzle = hdr_puzzle(hdr);
                     bool take = kmin != kDeadItem;
-                    if (take && a.node_limit[puzzle] && a.nodes[puzzle] >= a.node_limit[puzzle]) {
+                    const bool over_limit =
+                        take && a.node_limit[puzzle] && a.nodes[puzzle] >= a.node_limit[puzzle];
+                    if (over_limit && !a.keep_rest) {
                         atomicOr(&a.truncated[puzzle], 1u);
                         take = false;
                     }
-                    if (take && a.deadline && wall_clock64() > a.deadline) {
+                    if (take && (over_limit || (a.deadline && wall_clock64() > a.deadline))) {
                         atomicOr(&a.truncated[puzzle], 1u);
                         atomicOr(a.timed_out, 1u);
+                        if (a.keep_rest) {
+                            // the item is kept as it is for a later call (an
+                            // uncounted placements are counted now)
+                            if (hdr_pending(hdr)) atomicAdd(&a.nodes[puzzle], (unsigned long long)hdr_pending(hdr));
+                            const uint32_t o = atomicAdd(a.out_count, 1u);
+                            if (o < a.out_cap) {
+                                const uint32_t lv = hdr_level(hdr);
+                                for (uint32_t j = 0; j < lv; ++j)
+                                    a.out_stack[(size_t)o * a.stride + j] = a.in_stack[(size_t)idx * a.stride + j];
+                                a.out_hdr[o] = hdr & ~kPendingMask;
+                            } else {
+                                atomicOr(&a.flags[0], 1u);
+                            }
+                        }
                         take = false;
                     }
                     if (take) {
@@ -530,7 +583,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         last = hdr_last(hdr);
                         k = hdr_level(hdr);
                         steps = 0;
-                        nodes = 0;
+                        nodes = hdr_pending(hdr);
+                        item = idx;
                         depth = k;
                         fresh = true;
                         if (RANKED) {
@@ -575,9 +629,21 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
         if (!active) continue;
 
         if (a.deadline && wall_clock64() > a.deadline) {
-            // out of time: the rest of this item is not searched
+            // out of time: the rest of this item is not searched here; with
+            // keep_rest it is requeued as one item (level k above `last`,
+            // which is what the candidates held in c stand for)
             atomicOr(&a.truncated[puzzle], 1u);
             atomicOr(a.timed_out, 1u);
+            if (a.keep_rest) {
+                const uint32_t o = atomicAdd(a.out_count, 1u);
+                if (o < a.out_cap) {
+                    uint16_t *db = a.out_stack + (size_t)o * a.stride;
+                    for (uint32_t q = 0; q < k; ++q) db[q] = S(q);
+                    a.out_hdr[o] = make_hdr(puzzle, kmin, k, last);
+                } else {
+                    atomicOr(&a.flags[0], 1u);
+                }
+            }
             finish();
             continue;
         }
@@ -631,8 +697,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             if (sol) {
                 const uint64_t sidx = atomicAdd(&a.solutions[puzzle], 1ull);
                 if (sidx < a.sol_per_puzzle) {
-                    uint16_t *dst = a.sol_stack + ((size_t)puzzle * a.sol_per_puzzle + sidx) * a.stride;
+                    const size_t slot_s = (size_t)puzzle * a.sol_per_puzzle + sidx;
+                    uint16_t *dst = a.sol_stack + slot_s * a.stride;
                     for (uint32_t j = 0; j < P; ++j) dst[j] = S(j);
+                    if (a.sol_exec) {
+                        a.sol_exec[slot_s] = a.exec_base + item;
+                        a.sol_local[slot_s] = nodes + 1;
+                    }
                 }
             }
             nodes += plc;
@@ -885,8 +956,16 @@ void *Workspace::raw(const std::string &name, size_t bytes) {
 void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
                          const std::vector<const Puzzle *> &puzzles, const EnumConfig &cfg,
                          std::vector<EnumOutcome> &out, EnumStats &stats, const std::vector<EnumRoot> *roots,
-                         const std::vector<std::vector<uint16_t>> *ranks) {
+                         const std::vector<std::vector<uint16_t>> *ranks, EnumWork *work) {
     const uint32_t n_tab = (uint32_t)puzzles.size();
+    if (work && (n_tab != 1 || roots || ranks))
+        throw std::invalid_argument("work items describe one puzzle's plain enumeration");
+    const bool record = work && work->record_order;
+    const bool keep_rest = work && work->keep_rest;
+    if (record && cfg.use_graph) throw std::invalid_argument("ordered results are not recorded from graph replays");
+    if (keep_rest && !cfg.time_limit_ms && !cfg.node_limit)
+        throw std::invalid_argument("keeping the rest of a search needs a time or node limit");
+    const bool has_items = work && !work->items.empty();
     // One search per root; by default each puzzle's root item (index 0 holds
     // its first candidate).
     const uint32_t n = roots ? (uint32_t)roots->size() : n_tab;
@@ -921,6 +1000,19 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     std::vector<uint64_t> blob(total_words);
     for (uint32_t i = 0; i < n_tab; ++i)
         std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
+    const uint32_t n_init = has_items ? (uint32_t)work->items.size() : n;
+    if (has_items) {
+        const uint32_t P = tables[0].header().P;
+        const uint32_t T = tables[0].header().row * 64;
+        if (n_init > (1u << 25)) throw std::invalid_argument("at most 2^25 work items");
+        for (const EnumItem &it : work->items) {
+            if (it.floor < 1 || it.floor > it.level || it.level >= P || it.stack.size() != it.level)
+                throw std::invalid_argument("a work item needs 1 <= floor <= level < P and level placed transitions");
+            if (it.last < -1 || it.last >= (int)T) throw std::invalid_argument("a work item's last transition is out of range");
+            for (uint16_t t : it.stack)
+                if (t >= T) throw std::invalid_argument("a work item's transition is out of range");
+        }
+    }
     std::vector<uint32_t> root_table(n), root_off(n);
     for (uint32_t r = 0; r < n; ++r) {
         root_table[r] = roots ? (*roots)[r].table : r;
@@ -1035,6 +1127,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned int *d_n_in = cfg.use_graph ? ws.get<unsigned int>("n_in", 1) : nullptr;
     const uint32_t sol_per_puzzle = (uint32_t)std::min<uint64_t>(cfg.solution_capacity, (1ull << 26) / n);
     uint16_t *d_sol_stack = ws.get<uint16_t>("sol_stack", (size_t)n * std::max(sol_per_puzzle, 1u) * stride);
+    const uint64_t exec_cap = record ? std::max<uint64_t>(work->max_executions, 1) : 0;
+    unsigned long long *d_exec_nodes = record ? ws.get<unsigned long long>("exec_nodes", exec_cap) : nullptr;
+    uint32_t *d_sol_exec =
+        record ? ws.get<uint32_t>("sol_exec", (size_t)n * std::max(sol_per_puzzle, 1u)) : nullptr;
+    unsigned long long *d_sol_local =
+        record ? ws.get<unsigned long long>("sol_local", (size_t)n * std::max(sol_per_puzzle, 1u)) : nullptr;
     // Batches of several tables read them through the caches: group the
     // items by puzzle before each pass (roots of one table gain nothing).
     // Not under a node limit: a puzzle's count grows only as its items
@@ -1045,17 +1143,17 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
 
     // Work queues.  A pass emits at most two items per input item and always
     // has room for one; a full queue only stops items splitting in two.
-    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n * 4, 1u << 16});
-    uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n)
+    const uint64_t cap64 = std::max<uint64_t>({target * 4, lanes * 16, (uint64_t)n_init * 4, 1u << 16});
+    uint32_t cap = cfg.queue_items ? std::max<uint32_t>(cfg.queue_items, n_init)
                                    : (uint32_t)std::min<uint64_t>(cap64, 1u << 25);
     if (ws.is_arena()) {
         // queue rows of header + stack, two queues (three when grouping)
         const size_t row = 8 + (size_t)stride * 2, queues = group_items ? 3 : 2;
         const size_t room = ws.remaining() > 8 * Workspace::kAlign ? ws.remaining() - 8 * Workspace::kAlign : 0;
         const uint64_t fit = room / (row * queues);
-        if (fit < n)
+        if (fit < n_init)
             throw OutOfMemory("the fixed scratch buffer leaves room for " + std::to_string(fit) +
-                              " work items per queue, fewer than the " + std::to_string(n) + " searches");
+                              " work items per queue, fewer than the " + std::to_string(n_init) + " to start with");
         cap = (uint32_t)std::min<uint64_t>(cap, fit);
     }
     // prefix-split target within a quarter of the queue
@@ -1068,20 +1166,31 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
 
     // Root items: index 0 holds its first candidate (one node each), or the
     // given situations (their placements are not counted).
-    std::vector<uint64_t> h_hdr(n);
-    std::vector<uint16_t> h_stack((size_t)n * stride, 0);
-    std::vector<unsigned long long> h_nodes(n, roots ? 0 : 1), h_zero(n, 0), h_first(n, ~0ull),
+    std::vector<uint64_t> h_hdr(n_init);
+    std::vector<uint16_t> h_stack((size_t)n_init * stride, 0);
+    std::vector<unsigned long long> h_nodes(n, (roots || has_items) ? 0 : 1), h_zero(n, 0), h_first(n, ~0ull),
         h_limit(n, cfg.node_limit);
     std::vector<unsigned int> h_depth(n, 1);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t len = roots ? (uint32_t)(*roots)[i].stack.size() : 1;
-        h_hdr[i] = (uint64_t)i | ((uint64_t)len << 20) | ((uint64_t)len << 31);   // floor = level = len, last = -1
-        if (roots) std::copy((*roots)[i].stack.begin(), (*roots)[i].stack.end(), h_stack.begin() + (size_t)i * stride);
-        else h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
-        h_depth[i] = len;
+    if (has_items) {
+        // Given items of search 0; their placements are not counted.
+        h_depth[0] = 0;
+        for (uint32_t i = 0; i < n_init; ++i) {
+            const EnumItem &it = work->items[i];
+            h_hdr[i] = ((uint64_t)it.floor << 20) | ((uint64_t)it.level << 31) | ((uint64_t)(uint32_t)(it.last + 1) << 42);
+            std::copy(it.stack.begin(), it.stack.end(), h_stack.begin() + (size_t)i * stride);
+            h_depth[0] = std::max(h_depth[0], it.level);
+        }
+    } else {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t len = roots ? (uint32_t)(*roots)[i].stack.size() : 1;
+            h_hdr[i] = (uint64_t)i | ((uint64_t)len << 20) | ((uint64_t)len << 31);   // floor = level = len, last = -1
+            if (roots) std::copy((*roots)[i].stack.begin(), (*roots)[i].stack.end(), h_stack.begin() + (size_t)i * stride);
+            else h_stack[(size_t)i * stride] = (uint16_t)tables[i].header().root;
+            h_depth[i] = len;
+        }
     }
     EMP_CHECK(hipMemcpyAsync(d_depth, h_depth.data(), n * 4, hipMemcpyHostToDevice, stream));
-    EMP_CHECK(hipMemcpyAsync(d_hdr[0], h_hdr.data(), n * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_hdr[0], h_hdr.data(), n_init * 8, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_stack[0], h_stack.data(), h_stack.size() * 2, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_nodes, h_nodes.data(), n * 8, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_sols, h_zero.data(), n * 8, hipMemcpyHostToDevice, stream));
@@ -1118,6 +1227,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.rank_off = d_rank_off;
     a.rank_stride = d_rank_stride;
     a.timed_out = d_timed_out;
+    a.keep_rest = keep_rest ? 1u : 0u;
+    a.exec_nodes = d_exec_nodes;
+    a.sol_exec = d_sol_exec;
+    a.sol_local = d_sol_local;
     {
         const unsigned int zero = 0;
         EMP_CHECK(hipMemcpyAsync(d_timed_out, &zero, 4, hipMemcpyHostToDevice, stream));
@@ -1188,7 +1301,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     }
     int cur = 0;
     bool expansion_grows = true;
-    uint64_t n_items = n;
+    uint64_t n_items = n_init;
+    // ordered results: each DFS pass's input items (their keys) and the
+    // executions recorded so far
+    std::vector<uint64_t> exec_hdr;
+    std::vector<uint16_t> exec_stack;
+    uint64_t exec_base = 0;
     uint32_t expansions = 0;
     const uint32_t max_expansions = cfg.max_expansions ? cfg.max_expansions : 1024;
     while (n_items) {
@@ -1222,6 +1340,23 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_counters_save, d_counters, 8 * 4, hipMemcpyDeviceToDevice, stream));
+        }
+        if (record && !expand) {
+            // this pass's input items are executions exec_base..+n_items:
+            // keep their keys (header + stack) and clear their counts
+            if (exec_base + n_items > exec_cap)
+                throw Unsupported("an ordered enumeration recorded more than " + std::to_string(exec_cap) +
+                                  " work items; raise the step budget");
+            const uint32_t words = (uint32_t)(2 * n_items);
+            hipLaunchKernelGGL(fill_u32, dim3((words + 255) / 256), dim3(256), 0, stream,
+                               reinterpret_cast<unsigned int *>(d_exec_nodes + exec_base), words, 0u);
+            const size_t h0 = exec_hdr.size(), s0 = exec_stack.size();
+            exec_hdr.resize(h0 + n_items);
+            exec_stack.resize(s0 + (size_t)n_items * stride);
+            EMP_CHECK(hipMemcpyAsync(exec_hdr.data() + h0, d_hdr[cur], n_items * 8, hipMemcpyDeviceToHost, stream));
+            EMP_CHECK(hipMemcpyAsync(exec_stack.data() + s0, d_stack[cur], (size_t)n_items * stride * 2,
+                                     hipMemcpyDeviceToHost, stream));
+            a.exec_base = (uint32_t)exec_base;
         }
         unsigned int counters[8];
         if (cfg.use_graph && !expand) {
@@ -1272,6 +1407,36 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             if (counters[7])
                 throw std::invalid_argument("a situation places an identical piece while a lower-numbered twin is "
                                             "still available (such a piece is never a candidate)");
+            if (record) exec_base += n_items;
+            if (keep_rest) {
+                unsigned int cut = 0;
+                EMP_CHECK(hipMemcpyAsync(&cut, d_timed_out, 4, hipMemcpyDeviceToHost, stream));
+                EMP_CHECK(hipStreamSynchronize(stream));
+                if (cut) {
+                    // The output queue holds everything not searched: the
+                    // items requeued at the deadline and the split halves.
+                    const uint32_t m = std::min<uint32_t>(counters[1], cap);
+                    std::vector<uint64_t> rh(m);
+                    std::vector<uint16_t> rs((size_t)m * stride);
+                    EMP_CHECK(hipMemcpyAsync(rh.data(), d_hdr[1 - cur], (size_t)m * 8, hipMemcpyDeviceToHost, stream));
+                    EMP_CHECK(hipMemcpyAsync(rs.data(), d_stack[1 - cur], (size_t)m * stride * 2,
+                                             hipMemcpyDeviceToHost, stream));
+                    EMP_CHECK(hipStreamSynchronize(stream));
+                    work->rest.clear();
+                    for (uint32_t i = 0; i < m; ++i) {
+                        const uint64_t h = rh[i];
+                        const uint32_t fl = (uint32_t)(h >> 20) & 0x7FF;
+                        if (fl == kDeadItem) continue;
+                        EnumItem it;
+                        it.floor = fl;
+                        it.level = (uint32_t)(h >> 31) & 0x7FF;
+                        it.last = (int)((uint32_t)(h >> 42) & 0xFFF) - 1;
+                        it.stack.assign(rs.begin() + (size_t)i * stride, rs.begin() + (size_t)i * stride + it.level);
+                        work->rest.push_back(std::move(it));
+                    }
+                    break;
+                }
+            }
         }
         n_items = std::min<uint64_t>(counters[1], cap);
         cur = 1 - cur;
@@ -1308,14 +1473,78 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         o.max_depth = r_depth[i];
         o.solve_ms = (r_first[i] != ~0ull && r_last[i] >= r_first[i]) ? (double)(r_last[i] - r_first[i]) * tick_ms : 0.0;
     }
+    // Ordered results: each execution's part of the tree is an interval of
+    // the sequential DFS starting right after its key -- its stack, then its
+    // `last` transition (-1 before every transition), compared transition by
+    // transition; when one key extends the other, the longer one lies inside
+    // the subtree the shorter one has finished, so it comes first.  The
+    // executions sorted that way give each one the nodes before it.
+    std::vector<uint64_t> exec_offset;
+    std::vector<uint32_t> r_sol_exec;
+    std::vector<unsigned long long> r_sol_local;
+    if (record) {
+        std::vector<unsigned long long> en(exec_base);
+        r_sol_exec.resize((size_t)n * sol_per_puzzle);
+        r_sol_local.resize((size_t)n * sol_per_puzzle);
+        if (exec_base) EMP_CHECK(hipMemcpyAsync(en.data(), d_exec_nodes, exec_base * 8, hipMemcpyDeviceToHost, stream));
+        if (!r_sol_exec.empty()) {
+            EMP_CHECK(hipMemcpyAsync(r_sol_exec.data(), d_sol_exec, r_sol_exec.size() * 4, hipMemcpyDeviceToHost, stream));
+            EMP_CHECK(hipMemcpyAsync(r_sol_local.data(), d_sol_local, r_sol_local.size() * 8, hipMemcpyDeviceToHost,
+                                     stream));
+        }
+        EMP_CHECK(hipStreamSynchronize(stream));
+        auto level_of = [](uint64_t h) { return (uint32_t)(h >> 31) & 0x7FF; };
+        auto last_of = [](uint64_t h) { return (int)((uint32_t)(h >> 42) & 0xFFF) - 1; };
+        auto key_less = [&](uint64_t x, uint64_t y) {
+            const uint64_t hx = exec_hdr[x], hy = exec_hdr[y];
+            const uint32_t px = (uint32_t)hx & 0xFFFFF, py = (uint32_t)hy & 0xFFFFF;
+            if (px != py) return px < py;
+            const uint32_t lx = level_of(hx), ly = level_of(hy);
+            const uint16_t *sx = &exec_stack[x * stride], *sy = &exec_stack[y * stride];
+            for (uint32_t j = 0, m = std::min(lx, ly); j <= m; ++j) {
+                const int vx = j < lx ? (int)sx[j] : last_of(hx);
+                const int vy = j < ly ? (int)sy[j] : last_of(hy);
+                if (vx != vy) return vx < vy;
+            }
+            return lx > ly;
+        };
+        std::vector<uint64_t> by_start(exec_base);
+        for (uint64_t e = 0; e < exec_base; ++e) by_start[e] = e;
+        std::stable_sort(by_start.begin(), by_start.end(), key_less);
+        exec_offset.assign(exec_base, 0);
+        uint64_t run = 0;
+        uint32_t search = ~0u;
+        for (uint64_t e : by_start) {
+            const uint32_t ps = (uint32_t)exec_hdr[e] & 0xFFFFF;
+            if (ps != search) run = 0, search = ps;
+            exec_offset[e] = run;
+            run += en[e];
+        }
+    }
     for (uint32_t i = 0; i < n; ++i) {
         EnumOutcome &o = out[i];
         const EnumTable &tb = tables[root_table[i]];
         const uint32_t P = tb.header().P;
         const uint64_t kept = std::min<uint64_t>(o.solutions, sol_per_puzzle);
         o.boards.resize(kept * P);
+        std::vector<uint64_t> sol_order(kept);
+        for (uint64_t s = 0; s < kept; ++s) sol_order[s] = s;
+        if (record) {
+            // node index = the root placement (default root), the nodes of
+            // the executions before this one, and the nodes this execution
+            // had counted when it placed the solution
+            const uint64_t base = (!roots && !has_items) ? 1 : 0;
+            std::vector<uint64_t> idx(kept);
+            for (uint64_t s = 0; s < kept; ++s) {
+                const size_t slot = (size_t)i * sol_per_puzzle + s;
+                idx[s] = base + exec_offset.at(r_sol_exec[slot]) + r_sol_local[slot];
+            }
+            std::stable_sort(sol_order.begin(), sol_order.end(), [&](uint64_t x, uint64_t y) { return idx[x] < idx[y]; });
+            o.node_index.resize(kept);
+            for (uint64_t s = 0; s < kept; ++s) o.node_index[s] = idx[sol_order[s]];
+        }
         for (uint64_t s = 0; s < kept; ++s)
-            stack_to_board(tb, &r_stack[((size_t)i * sol_per_puzzle + s) * stride], &o.boards[s * P]);
+            stack_to_board(tb, &r_stack[((size_t)i * sol_per_puzzle + sol_order[s]) * stride], &o.boards[s * P]);
     }
     stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
 }

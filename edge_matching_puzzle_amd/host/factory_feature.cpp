@@ -3,13 +3,23 @@
 #include "feature_CUDA_backtracker.h"
 #include "feature_CUDA_glutton_max.h"
 #include "feature_CUDA_glutton_wide.h"
+#include "feature_new_strategy.h"
 
 namespace edge_matching_puzzle {
 
 std::unique_ptr<feature_if> factory_feature::create_feature(const std::string &p_name,
                                                             const puzzle_description &p_puzzle,
+                                                            const std::string &p_initial_situation,
+                                                            const std::string &p_dump_file_name,
                                                             const std::string &p_strategy,
                                                             const std::string &p_hint) {
+    if (p_name == "new_strategy")
+        return std::unique_ptr<feature_if>(
+            new feature_new_strategy(p_puzzle, p_strategy, p_dump_file_name, p_initial_situation));
+    // the GPU features start from an empty board and write no dump
+    // (include/factory_feature.h:142-153 passes them neither)
+    if (!p_initial_situation.empty())
+        throw logic_exception("--initial_situation is only used by new_strategy", __LINE__, __FILE__);
     if (p_name == "CUDA_backtracker")
         return std::unique_ptr<feature_if>(new feature_CUDA_backtracker(p_puzzle, p_strategy));
     if (p_name == "CUDA_enumerate")

@@ -1,7 +1,8 @@
 // factory_feature.h -- creates a feature by name, as the reference's
 // factory_feature::create_feature (include/factory_feature.h:49-167) does for
-// CUDA_backtracker, CUDA_glutton_max and CUDA_glutton_wide.  Other names are
-// refused with the reference's message.
+// CUDA_backtracker, CUDA_glutton_max, CUDA_glutton_wide and new_strategy,
+// with the same command-line inputs (initial situation, dump file, strategy,
+// hint).  Other names are refused with the reference's message.
 #ifndef EMP_FACTORY_FEATURE_H
 #define EMP_FACTORY_FEATURE_H
 
@@ -16,6 +17,8 @@ namespace edge_matching_puzzle {
 class factory_feature {
   public:
     static std::unique_ptr<feature_if> create_feature(const std::string &p_name, const puzzle_description &p_puzzle,
+                                                      const std::string &p_initial_situation,
+                                                      const std::string &p_dump_file_name,
                                                       const std::string &p_strategy, const std::string &p_hint);
 };
 

@@ -181,6 +181,86 @@ int emp_hip_enumerate_noalloc(emp_hip_context *context,
                               uint16_t *const *boards,
                               emp_hip_enum_stats *stats);
 
+/* ------------------------------------------------------------------------ *
+ * Work items: one puzzle's enumeration resumed from given situations, with
+ * the solutions in the order of the sequential search and a resumable cut.
+ * ------------------------------------------------------------------------ */
+#define EMP_HIP_NO_MOVE 0xFFFFFFFFu
+
+/* The state of the reference's DFS loop (include/emp_strategy.h:332-415):
+ * moves placed at strategy indices 0..level-1, the search continuing at index
+ * `level` above the move `last` (EMP_HIP_NO_MOVE: from its first candidate),
+ * backtracking, and ending when index `floor` has no candidate left.  The
+ * moves of an item are a row of P packed moves (pos | piece << 8 | o << 16),
+ * the first `level` of them used. */
+typedef struct {
+    uint32_t floor;     /* 1 <= floor <= level (index 0 is never revisited)  */
+    uint32_t level;     /* level < P                                         */
+    uint32_t last;      /* a move at index `level`, or EMP_HIP_NO_MOVE       */
+    uint32_t reserved;
+} emp_hip_item;
+
+/*
+ * Enumerates the union of the items (n_items = 0: the whole enumeration from
+ * its root, as emp_hip_enumerate); the items' own moves are not counted.
+ *   boards      [max_solutions][P] stored solutions
+ *   node_index  (may be NULL) when given, the stored solutions come in the
+ *               order of the sequential search, each with its node index
+ *               there (the placements up to and including it, counted from
+ *               the items; exact when the result is complete) -- the record
+ *               of results.bin (include/emp_strategy.h:367-375)
+ *   rest        (may be NULL, needs options->time_limit_ms > 0 or
+ *               options->node_limit > 0) receives the work left when the time
+ *               or node limit cut the run (*n_rest items, moves in rest_moves
+ *               [rest_capacity][P]; the node limit is checked as items are
+ *               taken, so the run stops somewhat past it); enumerating those
+ *               items later completes the enumeration: node counts add up
+ *               and the solution sets are disjoint.  EMP_HIP_OUT_OF_MEMORY
+ *               when more than rest_capacity items are left (*n_rest holds
+ *               the count, the first rest_capacity are written).
+ * Replaces: the reference's CPU search resumed from a situation
+ * (include/emp_strategy.h:418-467) and its save of the search state
+ * (include/emp_advanced_feature_base.h:270-322), on the GPU.
+ */
+int emp_hip_enumerate_items(emp_hip_context *context, const emp_hip_puzzle *puzzle,
+                            const emp_hip_enum_options *options,
+                            const emp_hip_item *items, const uint32_t *item_moves, uint32_t n_items,
+                            emp_hip_enum_result *result, uint16_t *boards, uint64_t *node_index,
+                            emp_hip_item *rest, uint32_t *rest_moves, uint64_t rest_capacity,
+                            uint64_t *n_rest, emp_hip_enum_stats *stats);
+
+/* The item the reference resumes from a situation (emp_strategy::
+ * set_initial_state, include/emp_strategy.h:418-467): with k leading strategy
+ * positions placed, the search restarts at index k - 1 from its first
+ * candidate with the moves of indices 0..k-2 kept (floor 1).  board is
+ * row-major; each of the k moves must be a candidate after the earlier ones
+ * (else EMP_HIP_BAD_ARGUMENT, as is k = 0).  *n_items = 1 with item / moves
+ * [P] written, or 0 when k = 1 (the whole enumeration). */
+int emp_hip_resume_item(const emp_hip_puzzle *puzzle, uint32_t order, const uint16_t *board,
+                        emp_hip_item *item, uint32_t *moves, uint32_t *n_items);
+
+/* Board of a situation string (include/situation_string_formatter.h:84-134):
+ * fixed-width cells, the id right-aligned then N/E/S/W, '-' cells empty. */
+int emp_hip_parse_situation(uint32_t width, uint32_t height, const char *text, uint16_t *board);
+
+/* results.bin v1 (include/emp_situation_binary_dumper.h:72-126): header
+ * (version, width, height, solution_dump, the (x, y) of each position of the
+ * order), one record per board (its fields in strategy order, then its node
+ * index), the total node count.  boards [n_records][P] row-major.  Writes at
+ * most capacity bytes to out (may be NULL); *size is the file size. */
+int emp_hip_results_bin_write(uint32_t width, uint32_t height, uint32_t order, uint32_t solution_dump,
+                              const uint16_t *boards, const uint64_t *node_index, uint64_t n_records,
+                              uint64_t total_nodes, uint8_t *out, uint64_t capacity, uint64_t *size);
+
+/* Reads a results.bin v1 image (include/emp_situation_binary_reader.h).
+ * xy [2P], boards [capacity][P] and node_index [capacity] may be NULL; P is
+ * width * height of the header.  *complete = 0 when the records do not fill
+ * the file (truncated, *total_nodes then 0). */
+int emp_hip_results_bin_read(const uint8_t *data, uint64_t size, uint32_t *width, uint32_t *height,
+                             uint32_t *solution_dump, uint32_t *xy, uint16_t *boards, uint64_t *node_index,
+                             uint64_t capacity, uint64_t *n_records, uint64_t *total_nodes,
+                             uint32_t *complete);
+
 /*
  * The reference's CUDA backtracker search (kernel_and_info,
  * src/CUDA_backtracker.cu:68-178): forward checking to the first solution,

@@ -361,6 +361,31 @@ struct Enumerator {
         return true;
     }
 
+    // emp_strategy::set_initial_state (include/emp_strategy.h:418-467): for
+    // each of the k leading positions of the situation, the transitions of
+    // the index are taken in order (each selection masking it and its
+    // identical twins) until the situation's own; the search then restarts at
+    // index k - 1 (specific_run recomputes that index's transitions).
+    // Returns the start index, or -1 when a piece is not reached that way.
+    int resume_from(const std::vector<int32_t> &situation) {
+        uint32_t k = 0;
+        while (k < P && situation[k] >= 0) ++k;
+        if (k == 0) return -1;
+        for (uint32_t i = 0; i < k; ++i) {
+            compute_available(i);
+            const uint32_t searched = (uint32_t)situation[i];
+            uint32_t next = 0;
+            do {
+                next = next_transition(i);
+                if (!next) return -1;
+                --next;
+                select(i, next);
+            } while (next != searched);
+            placed[i] = (int32_t)searched;
+        }
+        return (int)k - 1;
+    }
+
     // emp_strategy::specific_run (include/emp_strategy.h:332-415).  With
     // start = 0 this is the reference's loop: index 0 takes its first
     // candidate and the search ends when index 1 has no transition left.
@@ -368,13 +393,15 @@ struct Enumerator {
     // of that prefix and ends when index L has no transition left.  With
     // stop_depth = D > 0 placements at index D-1 are recorded as prefixes and
     // not descended.
+    // floor_index > 0 overrides where the search ends (resumed runs end, as
+    // the reference's loop, when index 1 has no transition left).
     void run(Result &r, uint64_t node_limit, uint64_t keep, bool want_trace,
              uint32_t start = 0, uint32_t stop_depth = 0,
-             std::vector<uint16_t> *prefixes = nullptr) {
+             std::vector<uint16_t> *prefixes = nullptr, uint32_t floor_index = 0) {
         r.depth_nodes.assign(P, 0);
         r.depth_dead.assign(P, 0);
         uint32_t index = start, max = 0;
-        const uint32_t floor = start ? start : 1;
+        const uint32_t floor = floor_index ? floor_index : (start ? start : 1);
         compute_available(index);
         bool go_on = true;
         while (go_on) {
@@ -976,6 +1003,41 @@ int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *co
     Enumerator::Result r;
     e.run(r, 0, boards ? max_solutions : 0, false, prefix_len);
     for (size_t s = 0; s < r.boards.size(); ++s) std::copy(r.boards[s].begin(), r.boards[s].end(), boards + s * m.P);
+    if (n_solutions) *n_solutions = r.solutions;
+    if (n_nodes) *n_nodes = r.nodes;
+    return ORACLE_OK;
+}
+
+int oracle_enumerate_resume(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                            const uint16_t *situation, uint64_t max_solutions, uint16_t *boards,
+                            uint64_t *solution_nodes, uint64_t *n_solutions, uint64_t *n_nodes) {
+    Model m;
+    if (order != ORACLE_ORDER_SPIRAL && order != ORACLE_ORDER_BASIC) return ORACLE_BAD_ARGUMENT;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!is_permutation(m, order) || !situation) return ORACLE_BAD_ARGUMENT;
+    Enumerator e(m, order);
+    // situation cells (row-major) -> transitions by strategy index
+    std::vector<int32_t> tr(m.P, -1);
+    for (uint32_t i = 0; i < m.P; ++i) {
+        const uint16_t v = situation[e.order[i].second * m.W + e.order[i].first];
+        if (v == 0xFFFF) continue;
+        const uint32_t piece = v >> 2;
+        if (piece >= m.P) return ORACLE_BAD_ARGUMENT;
+        const auto &list = m.by_kind[e.kind[i]];
+        int32_t ki = -1;
+        for (size_t j = 0; j < list.size(); ++j)
+            if (list[j] == piece) ki = (int32_t)j;
+        if (ki < 0) return ORACLE_BAD_ARGUMENT;
+        tr[i] = 4 * ki + (v & 3);
+    }
+    const int start = e.resume_from(tr);
+    if (start < 0) return ORACLE_BAD_ARGUMENT;
+    Enumerator::Result r;
+    e.run(r, 0, boards ? max_solutions : 0, false, (uint32_t)start, 0, nullptr, 1);
+    for (size_t s = 0; s < r.boards.size(); ++s) {
+        std::copy(r.boards[s].begin(), r.boards[s].end(), boards + s * m.P);
+        if (solution_nodes) solution_nodes[s] = r.solution_nodes[s];
+    }
     if (n_solutions) *n_solutions = r.solutions;
     if (n_nodes) *n_nodes = r.nodes;
     return ORACLE_OK;

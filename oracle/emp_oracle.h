@@ -78,6 +78,18 @@ int oracle_enumerate_subtree(uint32_t width, uint32_t height, const uint32_t *co
                              uint16_t *boards, uint64_t *n_solutions, uint64_t *n_nodes);
 
 /*
+ * The reference's search resumed from a situation (emp_strategy::
+ * set_initial_state then specific_run, include/emp_strategy.h:332-467):
+ * situation is a row-major board whose leading strategy positions hold the
+ * pieces to replay; the search restarts at the last of them and ends when
+ * index 1 has no transition left.  Node counts start at 0 (placements of the
+ * resumed run only); solution_nodes gives each stored solution's count.
+ */
+int oracle_enumerate_resume(uint32_t width, uint32_t height, const uint32_t *colours, int order,
+                            const uint16_t *situation, uint64_t max_solutions, uint16_t *boards,
+                            uint64_t *solution_nodes, uint64_t *n_solutions, uint64_t *n_nodes);
+
+/*
  * Same DFS below one prefix, reporting its placements and the most pieces
  * placed in one situation (prefix_len when nothing fits below it).
  */

@@ -34,6 +34,8 @@ def lib():
         _lib.oracle_enumerate_prefixes.restype = i32
         _lib.oracle_enumerate_subtree.argtypes = [u32, u32, p, i32, u32, p, u64, p, p, p]
         _lib.oracle_enumerate_subtree.restype = i32
+        _lib.oracle_enumerate_resume.argtypes = [u32, u32, p, i32, p, u64, p, p, p, p]
+        _lib.oracle_enumerate_resume.restype = i32
         _lib.oracle_subtree_depth.argtypes = [u32, u32, p, i32, u32, p, p, p]
         _lib.oracle_subtree_depth.restype = i32
         _lib.oracle_backtracker.argtypes = [u32, u32, p, i32, p, p, p, u64, p]
@@ -108,6 +110,22 @@ def subtree(w, h, colours, prefix, order=SPIRAL, max_solutions=1 << 12):
         raise ValueError("oracle_enumerate_subtree status %d" % st)
     n = int(sols[0])
     return {"n_nodes": int(nodes[0]), "n_solutions": n, "boards": boards[:min(n, max_solutions)].copy()}
+
+
+def resume(w, h, colours, situation, order=SPIRAL, max_solutions=1 << 12):
+    """The reference's search resumed from a situation (row-major board)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    sit = np.ascontiguousarray(situation, dtype=np.uint16)
+    boards = np.zeros((max_solutions, w * h), dtype=np.uint16)
+    sol_nodes = np.zeros(max_solutions, dtype=np.uint64)
+    ns, nn = np.zeros(1, np.uint64), np.zeros(1, np.uint64)
+    st = lib().oracle_enumerate_resume(w, h, _ptr(colours), order, _ptr(sit), max_solutions, _ptr(boards),
+                                       _ptr(sol_nodes), _ptr(ns), _ptr(nn))
+    if st != OK:
+        raise ValueError("oracle_enumerate_resume status %d" % st)
+    k = min(int(ns[0]), max_solutions)
+    return {"n_nodes": int(nn[0]), "n_solutions": int(ns[0]), "boards": boards[:k].copy(),
+            "solution_nodes": sol_nodes[:k].copy()}
 
 
 def subtree_depth(w, h, colours, prefix, order=SPIRAL):

@@ -19,7 +19,22 @@ def run(*args, timeout=120):
 
 def test_unknown_feature_is_an_error():
     rc, out = run(os.path.join(PUZZLES, "3_3.txt"), "ressources", "no_such_feature")
-    assert rc == 255 and out.startswith('ERROR : Unsupported feature "no_such_feature"')
+    assert rc == 255 and out.splitlines()[0] == "9 pieces loaded"
+    assert out.splitlines()[-1].startswith('ERROR : Unsupported feature "no_such_feature"')
+
+
+@pytest.mark.parametrize("feature,flag", [("CUDA_glutton_max", "--dump_file=x.bin"),
+                                          ("CUDA_backtracker", "--initial_situation=1N"),
+                                          ("CUDA_glutton_wide", "--enable_gui=true"),
+                                          ("new_strategy", "--strategy=strategy.txt"),
+                                          ("new_strategy", "--no_such_flag=1")])
+def test_flags_a_feature_does_not_use_are_errors(feature, flag):
+    """Every flag of the reference's command line (src/main.cpp:51-70) is
+    either used or refused before any device work: the GPU features take no
+    dump file or initial situation, there is no GUI, only the basic and
+    spiral orders exist."""
+    rc, out = run(os.path.join(PUZZLES, "3_3.txt"), "ressources", feature, flag)
+    assert rc == 255 and out.splitlines()[-1].startswith("ERROR : "), out
 
 
 def test_refused_puzzle_is_an_error(tmp_path):
