@@ -75,6 +75,7 @@ class GluttonStats(ctypes.Structure):
 GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)
 
 NO_MOVE = 0xFFFFFFFF
+WIDE_VALID, WIDE_EMPTIED, WIDE_BAD_MOVE = range(3)
 
 
 class Item(ctypes.Structure):
@@ -109,6 +110,7 @@ SIGNATURES = {
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
+    "emp_hip_glutton_wide": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _U64, _P, _P, _P, _P, _P]),
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
     "emp_hip_stream_copy": (_I32, [_P, _U64, _P]),
@@ -373,6 +375,38 @@ class Context:
                                          ctypes.byref(opt), ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
         results = [{"max_depth": r.max_depth, "status": r.status, "pushes": r.pushes} for r in res]
         return results, boards, {f: getattr(st, f) for f, _ in GluttonStats._fields_}
+
+    def glutton_wide(self, width, height, colours, parents=None, cells=None, capacity=1 << 16):
+        """glutton_wide's breadth step: each parent (rows of packed moves from
+        the level-0 capability; None: the level-0 situation alone) expanded
+        at its cell (default: (0, 0) for every parent).  Returns a dict with
+        parent_masks [n, P, nw] u64, parent_status, child_parent,
+        child_move, child_masks [m, P, nw] and child_status."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        P = width * height
+        nw = (4 * P + 63) // 64
+        if parents is None or np.asarray(parents).size == 0:
+            n = 1 if parents is None else len(parents)
+            rows = np.zeros((n, 0), dtype=np.uint32)
+        else:
+            rows = np.ascontiguousarray(np.asarray(parents, dtype=np.uint32))
+            n = rows.shape[0]
+        L = rows.shape[1]
+        cl = np.ascontiguousarray(np.zeros(n, dtype=np.uint32) if cells is None else np.asarray(cells, dtype=np.uint32))
+        pm = np.zeros((n, P, nw), dtype=np.uint64)
+        ps = np.zeros(n, dtype=np.uint32)
+        cp = np.zeros(capacity, dtype=np.uint32)
+        cmv = np.zeros(capacity, dtype=np.uint32)
+        cmask = np.zeros((capacity, P, nw), dtype=np.uint64)
+        cs = np.zeros(capacity, dtype=np.uint32)
+        nc = ctypes.c_uint64()
+        _check(lib().emp_hip_glutton_wide(self._ctx, ctypes.byref(pz), _ptr(rows) if L else None, L, n, _ptr(cl),
+                                          _ptr(pm), _ptr(ps), capacity, _ptr(cp), _ptr(cmv), _ptr(cmask), _ptr(cs),
+                                          ctypes.byref(nc)))
+        m = min(nc.value, capacity)
+        return {"parent_masks": pm, "parent_status": ps, "n_children": nc.value, "child_parent": cp[:m].copy(),
+                "child_move": cmv[:m].copy(), "child_masks": cmask[:m].copy(), "child_status": cs[:m].copy()}
 
     def backtrack_trace(self, width, height, colours, order="basic", max_selections=0, trace_capacity=1 << 16):
         """The reference's CUDA backtracker: first solution and the selection

@@ -17,6 +17,7 @@
 #include "emp_enum_plan.h"
 #include "emp_enumerate.h"
 #include "emp_glutton.h"
+#include "emp_glutton_wide.h"
 #include "emp_mlp.h"
 #include "emp_model.h"
 #include "emp_results.h"
@@ -651,6 +652,41 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, 
             stats->waves = st.waves;
             stats->lds_bytes = st.lds_bytes;
         }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_glutton_wide(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *parents,
+                         uint32_t parent_len, uint32_t n_parents, const uint32_t *cells, uint64_t *parent_masks,
+                         uint32_t *parent_status, uint64_t capacity, uint32_t *child_parent, uint32_t *child_move,
+                         uint64_t *child_masks, uint32_t *child_status, uint64_t *n_children) {
+    return guarded([&] {
+        if (!context || !puzzle || !cells || !n_children || !n_parents || (parent_len && !parents))
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument or no parent");
+        if (n_parents > (1u << 20)) return fail(EMP_HIP_BAD_ARGUMENT, "at most 2^20 parents");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        const uint32_t P = pz->size();
+        for (uint32_t p = 0; p < n_parents; ++p)
+            if (cells[p] >= P) return fail(EMP_HIP_BAD_ARGUMENT, "cell of parent " + std::to_string(p) + " out of range");
+        std::vector<uint32_t> rows(parents, parents + (size_t)n_parents * parent_len);
+        std::vector<uint32_t> cl(cells, cells + n_parents);
+        emp::WideOutcome out;
+        emp::glutton_wide_on_device(context->stream, context->workspace, *pz, rows, parent_len, n_parents, cl, out);
+        const size_t per = (size_t)P * ((4 * P + 63) / 64);
+        if (parent_masks) std::memcpy(parent_masks, out.parent_masks.data(), out.parent_masks.size() * 8);
+        if (parent_status) std::memcpy(parent_status, out.parent_status.data(), n_parents * 4);
+        const uint64_t n = out.child_parent.size(), m = std::min<uint64_t>(n, capacity);
+        *n_children = n;
+        if (child_parent && m) std::memcpy(child_parent, out.child_parent.data(), m * 4);
+        if (child_move && m) std::memcpy(child_move, out.child_move.data(), m * 4);
+        if (child_masks && m) std::memcpy(child_masks, out.child_masks.data(), m * per * 8);
+        if (child_status && m) std::memcpy(child_status, out.child_status.data(), m * 4);
         return EMP_HIP_OK;
     });
 }

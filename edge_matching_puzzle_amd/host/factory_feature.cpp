@@ -26,7 +26,7 @@ std::unique_ptr<feature_if> factory_feature::create_feature(const std::string &p
         return std::unique_ptr<feature_if>(new feature_CUDA_enumerate(p_puzzle, p_strategy));
     if (p_name == "CUDA_glutton_max") return std::unique_ptr<feature_if>(new feature_CUDA_glutton_max(p_puzzle));
     if (p_name == "CUDA_glutton_wide") {
-        unsigned int depth = 1;
+        unsigned int depth = 0;
         if (!p_hint.empty()) depth = (unsigned int)std::stoul(p_hint);
         return std::unique_ptr<feature_if>(new feature_CUDA_glutton_wide(p_puzzle, depth));
     }

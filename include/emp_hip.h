@@ -337,6 +337,35 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle,
                         emp_hip_glutton_result *results, uint16_t *boards,
                         emp_hip_glutton_stats *stats);
 
+/*
+ * glutton_wide's breadth step (src/CUDA_glutton_wide.cpp:29-67, with
+ * CUDA_glutton_situations::play_from, include/CUDA_glutton_situations.h
+ * :713-891).  Each parent is a row of parent_len moves (pos | piece << 8 |
+ * o << 16) played from the level-0 capability (include/CUDA_common_glutton.h
+ * :148-211): a move removes its piece from every position's mask and ANDs
+ * the colour constraint of each side into the free neighbour that side
+ * faces.  Each valid parent is then expanded at cells[p], one child per bit
+ * of that position's mask in bit order (bit 4 * piece + o).
+ *   masks        piece-major, nw = ceil(4 P / 64) u64 words per position,
+ *                position-indexed [P][nw], 0 at placed positions
+ *   parent_masks [n_parents][P][nw] (may be NULL); parent_status per parent
+ *   child_*      up to `capacity` children: parent index, move, masks
+ *                [capacity][P][nw] (may be NULL), status; *n_children is the
+ *                full count
+ *   status       EMP_HIP_WIDE_VALID, EMP_HIP_WIDE_EMPTIED (a free position
+ *                lost every option -- the reference stops the program
+ *                there), EMP_HIP_WIDE_BAD_MOVE (a parent move on a placed
+ *                position or outside its mask)
+ * Replaces: launch_CUDA_glutton_wide (src/CUDA_glutton_wide.cpp:70-75).
+ */
+#define EMP_HIP_WIDE_VALID    0
+#define EMP_HIP_WIDE_EMPTIED  1
+#define EMP_HIP_WIDE_BAD_MOVE 2
+int emp_hip_glutton_wide(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *parents,
+                         uint32_t parent_len, uint32_t n_parents, const uint32_t *cells, uint64_t *parent_masks,
+                         uint32_t *parent_status, uint64_t capacity, uint32_t *child_parent, uint32_t *child_move,
+                         uint64_t *child_masks, uint32_t *child_status, uint64_t *n_children);
+
 /* Seeded prefixes for glutton searches: each row is the n_fixed given moves
  * followed by one placement of each of the first `length` positions of the
  * spiral order, drawn by random dives that only place matching pieces

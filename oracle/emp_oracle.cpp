@@ -1118,6 +1118,26 @@ int oracle_backtracker(uint32_t width, uint32_t height, const uint32_t *colours,
     return ORACLE_OK;
 }
 
+int oracle_glutton_masks(uint32_t width, uint32_t height, const uint32_t *colours, uint32_t n_moves,
+                         const uint32_t *moves, uint64_t *masks, uint32_t *status) {
+    // glutton_wide's play_from (include/CUDA_glutton_situations.h:713-891)
+    // makes the masks apply_best_candidate makes (include/CUDA_glutton_max.h
+    // :1303-1476): each move is replayed as a chosen candidate.
+    Model m;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (m.P > 256 || !masks || !status) return ORACLE_BAD_ARGUMENT;
+    Glutton g(m);
+    *status = 0;
+    for (uint32_t i = 0; i < n_moves && !*status; ++i) {
+        const int r = g.seed(moves[3 * i], moves[3 * i + 1], moves[3 * i + 2]);
+        *status = r == 1 ? 2u : (r == 2 ? 1u : 0u);
+    }
+    std::fill(masks, masks + (size_t)m.P * g.nw, 0ull);
+    for (uint32_t q = 0; q < m.P; ++q)
+        if (g.is_free(q)) std::copy(g.cur(g.slot_of_pos[q]), g.cur(g.slot_of_pos[q]) + g.nw, masks + (size_t)q * g.nw);
+    return ORACLE_OK;
+}
+
 int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
                        uint32_t n_prefix, const uint32_t *prefix_moves,
                        uint64_t push_limit, uint16_t *board,

@@ -109,6 +109,15 @@ int oracle_results_bin(uint32_t width, uint32_t height, const uint32_t *colours,
 int oracle_order(uint32_t width, uint32_t height, int order, uint32_t *xy);
 
 /*
+ * Position masks (piece-major, [P][ceil(4P/64)] u64, 0 at placed positions)
+ * after the moves (pos, piece, o triples) are played from the level-0
+ * capability as glutton_wide's play_from does; status 0 valid, 1 a free
+ * position lost every option, 2 a move is not playable.
+ */
+int oracle_glutton_masks(uint32_t width, uint32_t height, const uint32_t *colours, uint32_t n_moves,
+                         const uint32_t *moves, uint64_t *masks, uint32_t *status);
+
+/*
  * glutton_max restatement (include/CUDA_glutton_max.h:742-1637,
  * include/CUDA_glutton_max_stack.h:38-630).
  *

@@ -40,6 +40,8 @@ def lib():
         _lib.oracle_subtree_depth.restype = i32
         _lib.oracle_backtracker.argtypes = [u32, u32, p, i32, p, p, p, u64, p]
         _lib.oracle_backtracker.restype = i32
+        _lib.oracle_glutton_masks.argtypes = [u32, u32, p, u32, p, p, p]
+        _lib.oracle_glutton_masks.restype = i32
         _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
         _lib.oracle_glutton_max.restype = i32
     return _lib
@@ -154,6 +156,20 @@ def order(w, h, kind=SPIRAL):
     xy = np.zeros(2 * w * h, dtype=np.uint32)
     lib().oracle_order(w, h, kind, _ptr(xy))
     return [(int(xy[2 * i]), int(xy[2 * i + 1])) for i in range(w * h)]
+
+
+def glutton_masks(w, h, colours, moves=()):
+    """Position masks [P, nw] u64 after the (pos, piece, o) moves and the
+    status (0 valid, 1 a position emptied, 2 unplayable move)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    mv = np.ascontiguousarray(np.array(list(moves), dtype=np.uint32).reshape(-1, 3))
+    nw = (4 * w * h + 63) // 64
+    masks = np.zeros((w * h, nw), dtype=np.uint64)
+    status = np.zeros(1, dtype=np.uint32)
+    st = lib().oracle_glutton_masks(w, h, _ptr(colours), mv.shape[0], _ptr(mv), _ptr(masks), _ptr(status))
+    if st != OK:
+        raise ValueError("oracle_glutton_masks status %d" % st)
+    return masks, int(status[0])
 
 
 def glutton_max(w, h, colours, prefix=(), push_limit=0):
