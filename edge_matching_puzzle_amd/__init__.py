@@ -110,6 +110,7 @@ SIGNATURES = {
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
+    "emp_hip_backtrack_variables": (_I32, [_P, _U32, _P]),
     "emp_hip_glutton_wide": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _U64, _P, _P, _P, _P, _P]),
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
@@ -213,6 +214,16 @@ def parse_situation(width, height, text):
     board = np.zeros(width * height, dtype=np.uint16)
     _check(lib().emp_hip_parse_situation(width, height, text.encode(), _ptr(board)))
     return board
+
+
+def backtrack_variables(width, height, colours, order="basic"):
+    """Variables of the backtracker search (create_variables; the reference
+    prints "Nb variables: <n>")."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    n = ctypes.c_uint64()
+    _check(lib().emp_hip_backtrack_variables(ctypes.byref(pz), _ORDERS[order], ctypes.byref(n)))
+    return n.value
 
 
 def resume_item(width, height, colours, board, order="spiral"):

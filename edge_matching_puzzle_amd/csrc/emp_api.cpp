@@ -656,6 +656,23 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, 
     });
 }
 
+int emp_hip_backtrack_variables(const emp_hip_puzzle *puzzle, uint32_t order, uint64_t *n_variables) {
+    return guarded([&] {
+        if (!puzzle || !n_variables) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (order != EMP_HIP_ORDER_SPIRAL && order != EMP_HIP_ORDER_BASIC) return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        uint64_t n = 0;
+        for (uint64_t w : emp::backtrack_capability(*pz, (int)order)) n += (uint64_t)__builtin_popcountll(w);
+        *n_variables = n;
+        return EMP_HIP_OK;
+    });
+}
+
 int emp_hip_glutton_wide(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *parents,
                          uint32_t parent_len, uint32_t n_parents, const uint32_t *cells, uint64_t *parent_masks,
                          uint32_t *parent_status, uint64_t capacity, uint32_t *child_parent, uint32_t *child_move,

@@ -22,6 +22,12 @@ struct BacktrackOutcome {
     double elapsed_ms = 0;
 };
 
+// Level-0 capability (P positions in strategy order x 16 u64, bit 256 * o +
+// piece): the variables of create_variables (include/emp_variable_generator.h
+// :336-531) set by prepare_initial (include/system_equation_for_CUDA.h
+// :159-173).  Its popcount is the reference's variable count.
+std::vector<uint64_t> backtrack_capability(const Puzzle &puzzle, int order);
+
 void backtrack_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &puzzle, int order,
                          BacktrackOutcome &out);
 

@@ -128,21 +128,15 @@ __global__ void __launch_bounds__(64) backtrack_kernel(BtArgs a) {
 
 }  // namespace
 
-void backtrack_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &pz, int order,
-                         BacktrackOutcome &out) {
-    (void)dev;
+std::vector<uint64_t> backtrack_capability(const Puzzle &pz, int order) {
     const uint32_t P = pz.size();
     if (P > 256) throw Unsupported("the backtracker's capabilities hold at most 256 pieces");
     const auto ord = make_order(pz.width(), pz.height(), order);
     if (!is_permutation(ord, pz.width(), pz.height()))
         throw std::domain_error("the spiral order is not defined for this board (the walk revisits cells)");
-    std::vector<uint32_t> index_of(P);
-    for (uint32_t i = 0; i < P; ++i) index_of[ord[i].second * pz.width() + ord[i].first] = i;
     // Level-0 capability: every kept (piece, o) of the position's kind that
     // matches the frame (create_variables + prepare_initial).
-    std::vector<uint64_t> init((size_t)P * kWords, 0), cm((size_t)(pz.max_colour() + 1) * 4 * kWords, 0);
-    std::vector<uint32_t> pcol(P);
-    std::vector<int32_t> nb((size_t)P * 4, -1);
+    std::vector<uint64_t> init((size_t)P * kWords, 0);
     for (uint32_t i = 0; i < P; ++i) {
         const uint32_t x = ord[i].first, y = ord[i].second;
         const bool off[4] = {y == 0, x + 1 == pz.width(), y + 1 == pz.height(), x == 0};
@@ -157,6 +151,26 @@ void backtrack_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                 init[(size_t)i * kWords + (b >> 6)] |= 1ull << (b & 63);
             }
         }
+    }
+    return init;
+}
+
+void backtrack_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &pz, int order,
+                         BacktrackOutcome &out) {
+    (void)dev;
+    const uint32_t P = pz.size();
+    if (P > 256) throw Unsupported("the backtracker's capabilities hold at most 256 pieces");
+    const auto ord = make_order(pz.width(), pz.height(), order);
+    if (!is_permutation(ord, pz.width(), pz.height()))
+        throw std::domain_error("the spiral order is not defined for this board (the walk revisits cells)");
+    std::vector<uint32_t> index_of(P);
+    for (uint32_t i = 0; i < P; ++i) index_of[ord[i].second * pz.width() + ord[i].first] = i;
+    const std::vector<uint64_t> init = backtrack_capability(pz, order);
+    std::vector<uint64_t> cm((size_t)(pz.max_colour() + 1) * 4 * kWords, 0);
+    std::vector<uint32_t> pcol(P);
+    std::vector<int32_t> nb((size_t)P * 4, -1);
+    for (uint32_t i = 0; i < P; ++i) {
+        const uint32_t x = ord[i].first, y = ord[i].second;
         if (y > 0) nb[i * 4 + 0] = (int32_t)index_of[(y - 1) * pz.width() + x];
         if (x + 1 < pz.width()) nb[i * 4 + 1] = (int32_t)index_of[y * pz.width() + x + 1];
         if (y + 1 < pz.height()) nb[i * 4 + 2] = (int32_t)index_of[(y + 1) * pz.width() + x];

@@ -21,6 +21,11 @@ void feature_CUDA_backtracker::run() {
     std::vector<uint32_t> trace(3 * capacity);
     uint32_t found = 0;
     uint64_t length = 0;
+    uint64_t variables = 0;
+    EMP_HIP_CALL(emp_hip_backtrack_variables(&puzzle, m_order, &variables));
+    // as prepare_transitions reports them (include/system_equation_for_CUDA.h:81-82)
+    std::cout << "Max variables nb: " << n * 256 * 4 << std::endl;
+    std::cout << "Nb variables: " << variables << std::endl;
     std::cout << "Launch kernels" << std::endl;
     EMP_HIP_CALL(emp_hip_backtrack_trace(session.get(), &puzzle, m_order, 0, board.data(), &found, trace.data(),
                                          capacity, &length));

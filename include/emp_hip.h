@@ -276,6 +276,13 @@ int emp_hip_backtrack_trace(emp_hip_context *context, const emp_hip_puzzle *puzz
                             uint64_t max_selections, uint16_t *board, uint32_t *found,
                             uint32_t *trace, uint64_t trace_capacity, uint64_t *trace_length);
 
+/* Number of variables of the backtracker search: (position, piece,
+ * orientation) triples of emp_variable_generator::create_variables
+ * (include/emp_variable_generator.h:336-531), i.e. the set bits of the
+ * level-0 capability the trace kernel starts from; the reference prints it as
+ * "Nb variables: <n>" (include/system_equation_for_CUDA.h:82).  No device work. */
+int emp_hip_backtrack_variables(const emp_hip_puzzle *puzzle, uint32_t order, uint64_t *n_variables);
+
 /*
  * Breadth expansion: every situation the enumeration visits with `depth`
  * pieces placed (indices 0..depth-1 of the order), level by level on the GPU.

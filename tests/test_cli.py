@@ -74,6 +74,11 @@ def test_backtracker_feature_stdout(expected_stdout, name):
     lines = out.splitlines()
     for s in expected_stdout["backtracker_" + name]["strings"]:
         assert s in lines
+    # prepare_transitions' report (include/system_equation_for_CUDA.h:81-82);
+    # the counts of tests/system_equation_<name>/test.info:3
+    nb = {"3_3": 36, "4_4": 144, "5_5": 484}[name]
+    P = {"3_3": 9, "4_4": 16, "5_5": 25}[name]
+    assert "Nb variables: %d" % nb in lines and "Max variables nb: %d" % (P * 1024) in lines
 
 
 @pytest.mark.gpu

@@ -148,3 +148,28 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode != 0
     assert "one GPU" in r.stderr + r.stdout
     assert r.stdout.strip() == ""
+
+
+@pytest.mark.parametrize("name,count", [("3_3", 36), ("4_4", 144), ("5_5", 484)])
+def test_backtracker_variable_counts(golden_dir, name, count):
+    """tests/system_equation_<name>/test.info:3 -- "Nb variables : 36 / 144 /
+    484" -- counted from the backtracker's level-0 capability."""
+    w, h, c = emp.parse_puzzle(os.path.join(golden_dir, "puzzles", name + ".txt"))
+    assert emp.backtrack_variables(w, h, c) == count
+    assert emp.backtrack_variables(w, h, c, order="spiral") == count
+
+
+def test_variable_count_of_synthetic_boards():
+    """Counted independently: each corner cell takes each corner piece in one
+    orientation, each border cell each border piece in one, each centre cell
+    each centre piece in its kept orientations (4, 2 when N = S and E = W,
+    1 when all four sides match; include/emp_piece_db.h:1266-1274)."""
+    for w, h, B, I, seed in ((5, 5, 3, 4, 1), (8, 8, 4, 8, 7), (7, 5, 3, 5, 2), (6, 6, 1, 2, 3)):
+        c, _ = emp.generate(seed, w, h, B, I)
+        zeros = (c == 0).sum(axis=1)
+        nb, nc = int((zeros == 1).sum()), int((zeros == 0).sum())
+        kept = 0
+        for p in np.nonzero(zeros == 0)[0]:
+            n, e, s_, wst = (int(v) for v in c[p])
+            kept += 1 if n == e == s_ == wst else (2 if (n == s_ and e == wst) else 4)
+        assert emp.backtrack_variables(w, h, c) == 16 + nb * nb + nc * kept
