@@ -1,23 +1,33 @@
 """Benchmark of the HIP search path on one MI355X.
 
-Workload (one "step"): full exhaustive enumeration of one 8x8 synthetic
-puzzle (64 pieces, 4 border colours, 8 interior colours, generator seed 7),
-corner kind-index 0 fixed at (0,0), spiral order -- the plain backtracker
-configuration the metric is quoted on.  The puzzle tables are built and
-uploaded once before timing; each step runs the whole enumeration on the GPU.
+Default line (the metric's plain leg, with the batched leg beside it):
+  one step = full exhaustive enumeration of one 8x8 synthetic puzzle (64
+  pieces, 4 border colours, 8 interior colours, generator seed 7), corner
+  kind-index 0 fixed at (0,0), spiral order -- the plain backtracker
+  configuration the metric is quoted on.  Each step runs the whole
+  enumeration on the GPU (tables built and uploaded by the call).
+  batched: one step = 4096 synthetic puzzles of side 5..10, 3..5 border and
+  4..12 interior colours in one enumeration (DFS passes replayed from
+  hipGraphs), each stopped at 2e7 nodes (the cells with more than 10^9
+  expected solutions cannot be enumerated by anyone; the count of complete and
+  cut instances is reported).
 
 Prints one JSON line:
   value            search nodes per second (placements, the reference's count)
   roofline         dominant kernel (dfs_kernel) against the LDS roofline: the
-                   algorithmic LDS bytes per node (8 M (n + 3), M = ceil(4P/64)
-                   u64 words per mask, n = placed neighbours of the next
-                   position, averaged over the depth histogram) times nodes,
-                   divided by the kernel time measured with HIP events on the
-                   library's stream; peak = 256 CUs x 256 B/clk x 2.4 GHz.
+                   algorithmic LDS bytes of the enumeration, 8 M (n + 3) per
+                   placement (M = ceil(4P/64) u64 words per mask, n = placed
+                   neighbours of the next position), counted exactly by an
+                   instrumented run of the same enumeration outside the timed
+                   region, per node times the timed nodes, divided by the
+                   kernel time measured with HIP events on the library's
+                   stream; peak = 256 CUs x 256 B/clk x 2.4 GHz.
                    peak_measured / valu_peak_measured_gops: the same peaks
                    measured on the device by stream kernels (emp_hip_peaks).
   cpu_baseline     the CPU oracle (single thread) on the first nodes of the
                    same DFS.
+  batched          the same fields for the batched leg (cpu_baseline: the
+                   oracle in a pool of host processes, one instance each).
 """
 import argparse
 import json
@@ -34,36 +44,13 @@ sys.path.insert(0, ROOT)
 METRIC = "search nodes/s on 1 MI355X, batched+learned vs plain; max depth per GPU-minute"
 LDS_PEAK_GBS = 256 * 256 * 2.4          # GB/s: 256 CUs x 256 B/clk (CDNA4 LDS reads) x 2.4 GHz
 VALU_PEAK_GOPS = 256 * 4 * 16 * 2.4     # 32-bit integer lane-ops/ns: 4 SIMD x 16 lanes per CU
+C2_NODE_LIMIT = 20_000_000              # per instance, batched leg
 
 
 def puzzle_c1(seed=7):
     import edge_matching_puzzle_amd as emp
     colours, _ = emp.generate(seed, 8, 8, 4, 8)
     return 8, 8, colours
-
-
-def per_node_work(w, h, depth_nodes):
-    """Algorithmic LDS bytes and integer ops per node, averaged over depth."""
-    import edge_matching_puzzle_amd as emp
-    order = emp.order(w, h)
-    index = {xy: i for i, xy in enumerate(order)}
-    P = w * h
-    M = math.ceil(4 * P / 64)
-    placed_nb = []
-    for i, (x, y) in enumerate(order):
-        n = 0
-        for dx, dy in ((0, -1), (1, 0), (0, 1), (-1, 0)):
-            nx, ny = x + dx, y + dy
-            if 0 <= nx < w and 0 <= ny < h and index[(nx, ny)] < i:
-                n += 1
-        placed_nb.append(n)
-    total = sum(depth_nodes)
-    bytes_node = ops_node = 0.0
-    for k, cnt in enumerate(depth_nodes):
-        n_next = placed_nb[k + 1] if k + 1 < P else 0
-        bytes_node += cnt * 8 * M * (n_next + 3)
-        ops_node += cnt * (2 * M * (n_next + 1) + 2 * M)
-    return bytes_node / total, ops_node / total, M
 
 
 def cpu_baseline(w, h, colours, nodes):
@@ -91,7 +78,8 @@ def _cpu_pool_c2(puzzles, limit, seconds=15.0):
         pool.terminate()
     dt = time.perf_counter() - t
     return {"value": nodes / dt, "unit": "nodes/s", "cores": cores, "kind": "port",
-            "sample": "first %d C2 instances, node limit %d each, one instance per process (%.1f s)" % (done, limit, dt)}
+            "sample": "first %d of the batch's instances, node limit %d each, one instance per process (%.1f s)"
+                      % (done, limit, dt)}
 
 
 def _cpu_one(args):
@@ -102,40 +90,70 @@ def _cpu_one(args):
     return o.enumerate_(w, h, np.array(c, dtype=np.uint32), node_limit=limit, max_solutions=1)["n_nodes"]
 
 
-def bench_c2(args, ctx):
-    """C2: 4096 uneven instances in one launch, per-instance node limit."""
+def _roofline(nodes, kernel_ms, work, launches, peaks=None):
+    """LDS roofline of dfs_kernel: work = (nodes, bytes, ops) of an
+    instrumented run of the same enumeration."""
+    w_nodes, w_bytes, w_ops = work
+    bytes_node, ops_node = w_bytes / w_nodes, w_ops / w_nodes
+    achieved = nodes * bytes_node / (kernel_ms * 1e-3) / 1e9 if kernel_ms else 0.0
+    valu = nodes * ops_node / (kernel_ms * 1e-3) / 1e9 if kernel_ms else 0.0
+    r = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / LDS_PEAK_GBS, "traffic": None, "bytes_per_node": bytes_node,
+         "kernel_ms": kernel_ms, "kernel_launches": launches, "valu_int_ops_per_node": ops_node,
+         "valu_achieved_gops": valu, "valu_peak_gops": VALU_PEAK_GOPS, "valu_frac": valu / VALU_PEAK_GOPS}
+    if peaks:
+        r.update({"peak_measured": peaks["lds_gbs"], "frac_of_measured": achieved / peaks["lds_gbs"],
+                  "valu_peak_measured_gops": peaks["valu_gops"], "valu_frac_of_measured": valu / peaks["valu_gops"]})
+    return r
+
+
+def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LIMIT):
+    """The batched leg: `instances` uneven puzzles in one enumeration."""
     from edge_matching_puzzle_amd.workloads import c2_puzzles
-    puzzles = c2_puzzles(args.instances)
-    limit = args.node_limit
-    for i in range(args.warmup):
-        ctx.enumerate(puzzles[:256], max_solutions=4, node_limit=limit)
-        print("warmup step %d done" % i, file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    nodes = 0
+    puzzles = c2_puzzles(instances)
+    for i in range(warmup):
+        ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True)
+        print("batched warmup step %d done" % i, file=sys.stderr, flush=True)
+    nodes = launches = 0
     kernel_ms = 0.0
-    for _ in range(args.steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
         res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True)
-        nodes += sum(r["n_nodes"] for r in res)
+        step_nodes = sum(r["n_nodes"] for r in res)
+        nodes += step_nodes
         kernel_ms += st["dfs_kernel_ms"]
-        print("step done: %d nodes, %.1f ms kernel" % (sum(r["n_nodes"] for r in res), st["dfs_kernel_ms"]),
+        launches += st["dfs_launches"]
+        print("batched step done: %d nodes, %.1f ms kernel" % (step_nodes, st["dfs_kernel_ms"]),
               file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     done = [r for r in res if r["complete"]]
     times = sorted(r["solve_ms"] for r in done)
     pct = lambda q: times[min(len(times) - 1, int(q * len(times)))] if times else None
-    cpu = getattr(args, "cpu_c2", None)
+    # algorithmic work of the same batch (instrumented run, not timed)
+    wres, wst = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True, count_work=True)
+    work = (sum(r["n_nodes"] for r in wres), wst["work_bytes"], wst["work_ops"])
     return {
-        "metric": METRIC, "value": nodes / elapsed, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "C2 batched enumeration: %d puzzles, side 5..10, border colours 3..5, "
-                               "interior colours 4..12, one batch, DFS passes replayed from hipGraphs, "
-                               "node limit %d per instance" % (len(puzzles), limit),
-                   "completed": len(done), "truncated": len(res) - len(done),
-                   "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99),
-                   "nodes_per_step": sum(r["n_nodes"] for r in res)},
-        "roofline": None, "cpu_baseline": cpu,
+        "value": nodes / elapsed, "unit": "nodes/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "config": {"workload": "batched enumeration: %d synthetic puzzles, side 5..10, border colours 3..5, "
+                               "interior colours 4..12, one enumeration, DFS passes replayed from hipGraphs, "
+                               "each instance stopped at %d nodes" % (len(puzzles), limit),
+                   "completed": len(done), "cut_at_node_limit": len(res) - len(done),
+                   "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99), "nodes_per_step": nodes // steps,
+                   "dfs_launches_per_step": launches // steps, "kernel_ms_per_step": kernel_ms / steps,
+                   "group_ms_per_step": st["group_ms"]},
+        "roofline": _roofline(nodes, kernel_ms, work, launches),
+        "cpu_baseline": cpu,
     }
+
+
+def bench_c2(args, ctx):
+    """The batched workload alone, as a full line."""
+    b = bench_batched(ctx, args.steps, args.warmup, getattr(args, "cpu_c2", None), args.instances, args.node_limit)
+    return {"metric": METRIC, "value": b["value"], "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": b["config"],
+            "roofline": b["roofline"], "cpu_baseline": b["cpu_baseline"]}
 
 
 def bench_c3(args, ctx):
@@ -156,7 +174,7 @@ def bench_c3(args, ctx):
             r, b, s = ctx.glutton_max(w, h, c, prefixes=rows[lo:lo + chunk], push_budget=args.push_budget)
             res += r
             boards.append(b)
-            for key in ("pushes", "kernel_ms", "evaluations", "score_evaluations", "rounds"):
+            for key in ("pushes", "kernel_ms", "evaluations", "score_evaluations", "rounds", "free_slot_sum"):
                 st[key] = st.get(key, 0) + s[key]
             st["waves"], st["lds_bytes"] = s["waves"], s["lds_bytes"]
             if chunk < len(rows):
@@ -185,9 +203,28 @@ def bench_c3(args, ctx):
                    "scoring_share": st["score_evaluations"] / max(st["evaluations"], 1),
                    "removal_rounds_per_push": st["rounds"] / max(st["pushes"], 1),
                    "waves": st["waves"], "lds_bytes_per_wave": st["lds_bytes"]},
-        "roofline": None,
+        "roofline": _glutton_roofline(w * h, st),
         "cpu_baseline": None if args.no_cpu else _cpu_c3(w, h, c, rows[0]),
     }
+
+
+def _glutton_roofline(P, st):
+    """glutton_kernel against the LDS roofline by the reference's work per
+    candidate evaluation (include/CUDA_glutton_max.h:455-714): one mask AND
+    per other free position and up to four colour-mask ANDs, each mask
+    ceil(4P/64) u64 words -- 8 nw (S_free + 3) bytes, S_free summed over the
+    evaluations by the kernel.  The kernel reaches the same choices with far
+    fewer reads (slot contexts, affected slots only), so this is the
+    reference-equivalent rate."""
+    nw = math.ceil(4 * P / 64)
+    bytes_ = 8 * nw * (st["free_slot_sum"] + 3 * st["evaluations"])
+    achieved = bytes_ / (st["kernel_ms"] * 1e-3) / 1e9 if st["kernel_ms"] else 0.0
+    return {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+            "bytes_per_evaluation": bytes_ / max(st["evaluations"], 1), "kernel_ms": st["kernel_ms"],
+            "definition": "reference mask reads per candidate evaluation, 8 nw (S_free + 3) bytes",
+            "note": "a fraction above 1 is the reference-equivalent rate: the kernel evaluates a candidate "
+                    "from per-slot colour counts instead of one pass over every free position"}
 
 
 def _invalid_glutton_boards(w, h, c, rows, res, boards):
@@ -213,19 +250,6 @@ def _cpu_c3(w, h, c, row):
             "sample": "one prefix, one push (CPU oracle, %.1f s)" % dt}
 
 
-def hbm_traffic():
-    """HBM bytes per dfs_kernel launch of the C1 step, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the reading
-    of a plain copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "dfs_kernel_c1_hbm.json")
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        return d["bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
-        return None
-
-
 def bench_c4(args, ctx):
     """C4: learned candidate ordering on the 16x16 shape (5 border / 17
     interior colours): train on depth-probe traces of 64 seeds, then compare
@@ -249,6 +273,19 @@ def bench_c4(args, ctx):
     }
 
 
+def hbm_traffic():
+    """HBM bytes per dfs_kernel launch of the plain step, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the reading
+    of a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "dfs_kernel_c1_hbm.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,14 +296,15 @@ def main():
     ap.add_argument("--c4-time-ms", type=float, default=2000.0, help="C4 device time per search")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, copy 1 GiB with a plain kernel (HBM counter calibration)")
-    ap.add_argument("--instances", type=int, default=4096, help="C2 instances")
-    ap.add_argument("--node-limit", type=int, default=200_000_000, help="C2 nodes per instance")
+    ap.add_argument("--instances", type=int, default=4096, help="batched instances")
+    ap.add_argument("--node-limit", type=int, default=C2_NODE_LIMIT, help="batched nodes per instance")
+    ap.add_argument("--batched-steps", type=int, default=2, help="batched leg steps in the default line (0: none)")
     ap.add_argument("--prefixes", type=int, default=4096, help="C3 prefixes")
     ap.add_argument("--push-budget", type=int, default=16, help="C3 pushes per prefix")
     ap.add_argument("--chunk", type=int, default=0, help="C3 prefixes per launch (0: all in one launch)")
     ap.add_argument("--autotune", action="store_true", help="tune the C1 launch before timing")
     ap.add_argument("--cpu-nodes", type=int, default=600_000_000,
-                    help="nodes of the CPU baseline sample (about 15 s on one core)")
+                    help="nodes of the CPU baseline sample (about 10 s on one core)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -275,11 +313,11 @@ def main():
         sys.exit(2)
 
     import edge_matching_puzzle_amd as emp
-    if args.workload == "c2" and not args.no_cpu:
+    if args.workload in ("c1", "c2") and not args.no_cpu and (args.workload == "c2" or args.batched_steps):
         # the host pool forks, so it runs before this process touches the GPU
         from edge_matching_puzzle_amd.workloads import c2_puzzles
-        args.cpu_c2 = _cpu_pool_c2(c2_puzzles(args.instances), 20_000_000)
-        print("cpu baseline done", file=sys.stderr, flush=True)
+        args.cpu_c2 = _cpu_pool_c2(c2_puzzles(args.instances), args.node_limit)
+        print("batched cpu baseline done", file=sys.stderr, flush=True)
     ctx = emp.Context(0)
     if args.workload != "c1":
         line = {"c2": bench_c2, "c3": bench_c3, "c4": bench_c4}[args.workload](args, ctx)
@@ -317,20 +355,17 @@ def main():
     if not r["complete"]:
         raise SystemExit("enumeration incomplete")
     value = nodes / elapsed
+    # algorithmic work of the same enumeration (instrumented run, not timed)
+    (wr,), wst = ctx.enumerate([(w, h, colours)], max_solutions=64, count_work=True, **tuned)
+    if wr["n_nodes"] != r["n_nodes"]:
+        raise SystemExit("instrumented run visited %d nodes, timed runs %d" % (wr["n_nodes"], r["n_nodes"]))
+    work = (wr["n_nodes"], wst["work_bytes"], wst["work_ops"])
 
     cpu = None
     if not args.no_cpu:
         rc, dt = cpu_baseline(w, h, colours, args.cpu_nodes)
-        depth = [int(v) for v in rc["depth_nodes"]]
         cpu = {"value": rc["n_nodes"] / dt, "unit": "nodes/s", "cores": 1, "kind": "port",
                "sample": "first %d nodes of the same DFS (CPU oracle, one thread, %.1f s)" % (rc["n_nodes"], dt)}
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_bindings as o
-        depth = [int(v) for v in o.enumerate_(w, h, colours, node_limit=20_000_000)["depth_nodes"]]
-    bytes_node, ops_node, M = per_node_work(w, h, depth)
-    achieved = nodes * bytes_node / (kernel_ms * 1e-3) / 1e9 if kernel_ms else 0.0
-    valu = nodes * ops_node / (kernel_ms * 1e-3) / 1e9 if kernel_ms else 0.0
     line = {
         "metric": METRIC,
         "value": value,
@@ -349,20 +384,20 @@ def main():
                    "nodes_per_step": r["n_nodes"], "solutions": r["n_solutions"],
                    "dfs_launches_per_step": st["dfs_launches"], "grid": [st["blocks"], st["threads"]],
                    "lds_tables": bool(st["lds_tables"]), "tuned": tuned or None},
-        "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / LDS_PEAK_GBS, "traffic": hbm_traffic(),
-                     "bytes_per_node": bytes_node, "words_per_mask": M,
-                     "kernel_ms": kernel_ms, "kernel_launches": launches,
-                     "valu_int_ops_per_node": ops_node, "valu_achieved_gops": valu,
-                     "valu_peak_gops": VALU_PEAK_GOPS, "valu_frac": valu / VALU_PEAK_GOPS},
         "cpu_baseline": cpu,
     }
+    if args.batched_steps:
+        line["batched"] = bench_batched(ctx, args.batched_steps, 1, getattr(args, "cpu_c2", None),
+                                        args.instances, args.node_limit)
     # The same two peaks measured on this device by stream kernels (after the
-    # timed region): the fractions against them sit beside the published ones.
+    # timed regions): the fractions against them sit beside the published ones.
     pk = ctx.peaks()
-    line["roofline"].update({"peak_measured": pk["lds_gbs"], "frac_of_measured": achieved / pk["lds_gbs"],
-                             "valu_peak_measured_gops": pk["valu_gops"],
-                             "valu_frac_of_measured": valu / pk["valu_gops"]})
+    line["roofline"] = _roofline(nodes, kernel_ms, work, launches, pk)
+    line["roofline"]["traffic"] = hbm_traffic()
+    line["roofline"]["words_per_mask"] = math.ceil(4 * w * h / 64)
+    if "batched" in line:
+        b = line["batched"]["roofline"]
+        b.update({"peak_measured": pk["lds_gbs"], "frac_of_measured": b["achieved"] / pk["lds_gbs"]})
     if args.calibrate_copy:
         print("copy kernel: 1 GiB in %.3f ms" % ctx.stream_copy(1 << 30), file=sys.stderr, flush=True)
     ctx.close()

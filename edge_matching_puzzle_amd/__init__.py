@@ -36,7 +36,8 @@ class EnumOptions(ctypes.Structure):
                 ("step_budget", ctypes.c_uint64), ("target_items", ctypes.c_uint64),
                 ("node_limit", ctypes.c_uint64), ("max_solutions", ctypes.c_uint64),
                 ("use_graph", ctypes.c_uint32), ("queue_items", ctypes.c_uint32),
-                ("time_limit_ms", ctypes.c_double)]
+                ("time_limit_ms", ctypes.c_double), ("count_work", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class EnumResult(ctypes.Structure):
@@ -53,7 +54,8 @@ class EnumStats(ctypes.Structure):
                 ("splits", ctypes.c_uint64), ("max_words", ctypes.c_uint32),
                 ("lds_tables", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
                 ("threads", ctypes.c_uint32), ("candidate_sets", ctypes.c_uint64),
-                ("group_ms", ctypes.c_double)]
+                ("group_ms", ctypes.c_double), ("work_bytes", ctypes.c_uint64),
+                ("work_ops", ctypes.c_uint64)]
 
 
 class GluttonOptions(ctypes.Structure):
@@ -69,7 +71,8 @@ class GluttonStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
                 ("pushes", ctypes.c_uint64), ("evaluations", ctypes.c_uint64),
                 ("waves", ctypes.c_uint32), ("lds_bytes", ctypes.c_uint32),
-                ("score_evaluations", ctypes.c_uint64), ("rounds", ctypes.c_uint64)]
+                ("score_evaluations", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("free_slot_sum", ctypes.c_uint64)]
 
 
 GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)
@@ -529,7 +532,7 @@ class Context:
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
                   node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0, use_graph=False,
-                  noalloc=False):
+                  noalloc=False, count_work=False):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
@@ -554,6 +557,7 @@ class Context:
                             target_items, node_limit, queue_items)
         opt.time_limit_ms = time_limit_ms
         opt.use_graph = 1 if use_graph else 0
+        opt.count_work = 1 if count_work else 0
         res = (EnumResult * n)()
         st = EnumStats()
         if noalloc:

@@ -79,6 +79,7 @@ emp::EnumConfig config_from(const emp_hip_enum_options &opt) {
     cfg.queue_items = opt.queue_items;
     cfg.time_limit_ms = opt.time_limit_ms;
     cfg.use_graph = opt.use_graph != 0;
+    cfg.count_work = opt.count_work != 0;
     // Device-side store of up to max_solutions boards per search.
     cfg.solution_capacity = opt.max_solutions;
     return cfg;
@@ -107,6 +108,8 @@ void copy_stats(const emp::EnumStats &st, emp_hip_enum_stats *stats) {
     stats->splits = st.splits;
     stats->candidate_sets = st.candidate_sets;
     stats->group_ms = st.group_ms;
+    stats->work_bytes = st.work_bytes;
+    stats->work_ops = st.work_ops;
     stats->max_words = st.max_words;
     stats->lds_tables = st.lds_tables;
     stats->blocks = st.blocks;
@@ -649,6 +652,7 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, 
             stats->evaluations = st.evaluations;
             stats->score_evaluations = st.score_evaluations;
             stats->rounds = st.rounds;
+            stats->free_slot_sum = st.free_slot_sum;
             stats->waves = st.waves;
             stats->lds_bytes = st.lds_bytes;
         }

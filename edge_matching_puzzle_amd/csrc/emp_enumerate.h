@@ -78,6 +78,7 @@ struct EnumConfig {
     uint32_t queue_items = 0;       // capacity of each work queue, 0 = sized from the grid
     double time_limit_ms = 0;       // stop every search after this device time (0 = none)
     bool use_graph = false;         // replay the DFS passes from two captured hipGraphs
+    bool count_work = false;        // accumulate the algorithmic bytes / ops of every placement
 };
 
 struct EnumOutcome {
@@ -120,6 +121,8 @@ struct EnumStats {
     uint32_t max_words = 0, lds_tables = 0, blocks = 0, threads = 0;
     uint64_t candidate_sets = 0;   // DFS iterations (one candidate set each)
     double group_ms = 0;           // grouping items by puzzle before the passes
+    uint64_t work_bytes = 0;       // count_work: algorithmic LDS bytes of the placements
+    uint64_t work_ops = 0;         // count_work: algorithmic integer ops of the placements
 };
 
 // A search that starts from a given situation of puzzle `table`: the

@@ -102,6 +102,11 @@ struct DfsArgs {
     uint32_t exec_base;
     uint32_t *sol_exec;
     unsigned long long *sol_local;
+    // Work accounting (instrumented runs): [0] algorithmic LDS bytes, [1]
+    // integer ops of the placements, 8 M (n + 3) and 2 M (n + 1) + 2 M per
+    // placement with M = ceil(4 P / 64) u64 words and n the placed neighbours
+    // of the next strategy index (0 after the last one).
+    unsigned long long *work;
 };
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
@@ -415,6 +420,12 @@ __global__ void expand_kernel(DfsArgs a) {
         return;
     }
     if (!record) atomicAdd(&a.nodes[puzzle], (unsigned long long)n);
+    if (a.work && !record) {
+        const uint64_t M = (4ull * tv.P + 63) / 64;
+        const uint64_t nn = kmin + 1 < tv.P ? (tv.nbr(kmin + 1) & 7) : 0;
+        atomicAdd(&a.work[0], (unsigned long long)(n * 8 * M * (nn + 3)));
+        atomicAdd(&a.work[1], (unsigned long long)(n * 2 * M * (nn + 2)));
+    }
     atomicMax(&a.max_depth[puzzle], kmin + 1);
     if (kmin == tv.P - 1) {
         uint64_t s = atomicAdd(&a.solutions[puzzle], (unsigned long long)n);
@@ -511,6 +522,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     const uint16_t *rank_base = nullptr;   // ranked: this puzzle's rank rows
     uint32_t rank_stride = 0;
     uint64_t steps = 0, nodes = 0;
+    uint64_t work_bytes = 0, work_ops = 0;   // instrumented runs only
+    uint32_t words_alg = 0;                  // M = ceil(4 P / 64) of the item's puzzle
     uint64_t c[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) c[w] = 0;
@@ -522,6 +535,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
         if (a.exec_nodes) a.exec_nodes[a.exec_base + item] = nodes;
+        if (a.work) {
+            atomicAdd(&a.work[0], (unsigned long long)work_bytes);
+            atomicAdd(&a.work[1], (unsigned long long)work_ops);
+        }
         atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
         atomicAdd(a.candidate_sets, (unsigned long long)steps);
         atomicMax(&a.max_depth[puzzle], depth);
@@ -585,6 +602,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         steps = 0;
                         nodes = hdr_pending(hdr);
                         item = idx;
+                        work_bytes = work_ops = 0;
+                        words_alg = (4 * P + 63) / 64;
                         depth = k;
                         fresh = true;
                         if (RANKED) {
@@ -713,6 +732,12 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             const int tl = pop ? (int)held : (down ? -1 : (sol ? t : last));
             fresh = false;
             const uint64_t d = tv.nbr(tk);
+            if (a.work) {
+                // a placement at k reads the masks of index k + 1 (none after the last)
+                const uint32_t nn = sol ? 0u : (uint32_t)d & 7;
+                work_bytes += plc ? 8ull * words_alg * (nn + 3) : 0ull;
+                work_ops += plc ? 2ull * words_alg * (nn + 2) : 0ull;
+            }
             uint64_t cc[W];
             candidates<W, false>(tv, tk, d, tl,
                           [&](uint32_t off) {
@@ -1118,6 +1143,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     unsigned long long *d_csets = ws.get<unsigned long long>("candidate_sets", 1);
     unsigned int *d_depth = ws.get<unsigned int>("max_depth", n);
     unsigned int *d_timed_out = ws.get<unsigned int>("timed_out", 1);
+    unsigned long long *d_work = ws.get<unsigned long long>("work", 2);
     unsigned long long *d_now = ws.get<unsigned long long>("clock_now", 1);
     // Counters saved before each expansion so that an expansion whose
     // children do not fit can be undone and its items searched instead.
@@ -1228,6 +1254,11 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.rank_stride = d_rank_stride;
     a.timed_out = d_timed_out;
     a.keep_rest = keep_rest ? 1u : 0u;
+    a.work = cfg.count_work ? d_work : nullptr;
+    if (cfg.count_work) {
+        const unsigned long long zero2[2] = {0, 0};
+        EMP_CHECK(hipMemcpyAsync(d_work, zero2, 16, hipMemcpyHostToDevice, stream));
+    }
     a.exec_nodes = d_exec_nodes;
     a.sol_exec = d_sol_exec;
     a.sol_local = d_sol_local;
@@ -1459,6 +1490,22 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     EMP_CHECK(hipMemcpyAsync(r_depth.data(), d_depth, n * 4, hipMemcpyDeviceToHost, stream));
     EMP_CHECK(hipStreamSynchronize(stream));
     stats.candidate_sets = r_csets;
+    if (cfg.count_work) {
+        unsigned long long w[2] = {0, 0};
+        EMP_CHECK(hipMemcpyAsync(w, d_work, 16, hipMemcpyDeviceToHost, stream));
+        EMP_CHECK(hipStreamSynchronize(stream));
+        stats.work_bytes = w[0];
+        stats.work_ops = w[1];
+        if (!roots && !has_items)
+            // the root placements (index 0, counted on the host)
+            for (uint32_t i = 0; i < n; ++i) {
+                const EnumTableHeader h = tables[i].header();
+                const uint64_t M = (4ull * h.P + 63) / 64;
+                const uint64_t nn = h.P > 1 ? (tables[i].blob[h.off_nbr + kNbrWords] & 7) : 0;
+                stats.work_bytes += 8 * M * (nn + 3);
+                stats.work_ops += 2 * M * (nn + 2);
+            }
+    }
     std::vector<uint16_t> r_stack((size_t)n * sol_per_puzzle * stride);
     if (!r_stack.empty()) {
         EMP_CHECK(hipMemcpyAsync(r_stack.data(), d_sol_stack, r_stack.size() * 2, hipMemcpyDeviceToHost, stream));

@@ -37,6 +37,7 @@ struct GluttonOutcome {
 struct GluttonStats {
     double total_ms = 0, kernel_ms = 0;
     uint64_t pushes = 0, evaluations = 0, score_evaluations = 0, rounds = 0;
+    uint64_t free_slot_sum = 0;   // free slots of the level, summed over the evaluations
     uint32_t waves = 0, lds_bytes = 0;
 };
 

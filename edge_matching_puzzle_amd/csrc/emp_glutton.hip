@@ -62,7 +62,8 @@ struct GlArgs {
     uint64_t *out_pushes;        // [n_items]
     uint32_t *out_status;        // [n_items]
     uint16_t *out_board;         // [n_items][P]
-    unsigned long long *evals;   // [0] candidates evaluated, [1] of them scored, [2] removal rounds
+    unsigned long long *evals;   // [0] candidates evaluated, [1] of them scored, [2] removal rounds,
+                                 // [3] free slots summed over the evaluations
 };
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
@@ -281,12 +282,14 @@ struct Wave {
         const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)w);
         return (uint64_t)hi << 32 | lo;
     }
-    // a.evals: [0] candidate evaluations, [1] of them for scoring, [2] removal rounds
+    // a.evals: [0] candidate evaluations, [1] of them for scoring, [2]
+    // removal rounds, [3] free slots (this level's) summed over evaluations
     __device__ void count(uint64_t evaluated, bool scoring) {
         for (int o = 32; o > 0; o >>= 1) evaluated += __shfl_xor(evaluated, o);
         if (lane == 0 && evaluated) {
             atomicAdd(&a.evals[0], (unsigned long long)evaluated);
             if (scoring) atomicAdd(&a.evals[1], (unsigned long long)evaluated);
+            atomicAdd(&a.evals[3], (unsigned long long)(evaluated * n()));
         }
     }
     __device__ bool is_free(uint32_t pos) const { return slot_of_pos[pos] < n(); }
@@ -1590,14 +1593,14 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     uint64_t *d_pushes = ws.get<uint64_t>("gl_pushes", n_items);
     uint32_t *d_status = ws.get<uint32_t>("gl_status", n_items);
     uint16_t *d_board = ws.get<uint16_t>("gl_board", (size_t)n_items * t.P);
-    unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 3);
+    unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 4);
     GL_CHECK(hipMemcpyAsync(d_cm, t.cm.data(), t.cm.size() * 8, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_pcol, t.piece_col.data(), t.piece_col.size() * 4, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_init, t.init_masks.data(), t.init_masks.size() * 8, hipMemcpyHostToDevice, stream));
     if (!prefixes.empty())
         GL_CHECK(hipMemcpyAsync(d_pre, prefixes.data(), prefixes.size() * 4, hipMemcpyHostToDevice, stream));
     const unsigned int zero2[2] = {0, 0};
-    const unsigned long long zero64[3] = {0, 0, 0};
+    const unsigned long long zero64[4] = {0, 0, 0, 0};
     GL_CHECK(hipMemcpyAsync(d_next, zero2, sizeof(zero2), hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_evals, zero64, sizeof(zero64), hipMemcpyHostToDevice, stream));
 
@@ -1648,7 +1651,7 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     std::vector<uint32_t> depth(n_items), status(n_items);
     std::vector<uint64_t> pushes(n_items);
     std::vector<uint16_t> boards((size_t)n_items * t.P);
-    unsigned long long evals[3] = {0, 0, 0};
+    unsigned long long evals[4] = {0, 0, 0, 0};
     GL_CHECK(hipMemcpyAsync(depth.data(), d_depth, n_items * 4, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(status.data(), d_status, n_items * 4, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(pushes.data(), d_pushes, n_items * 8, hipMemcpyDeviceToHost, stream));
@@ -1668,6 +1671,7 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     stats.evaluations = evals[0];
     stats.score_evaluations = evals[1];
     stats.rounds = evals[2];
+    stats.free_slot_sum = evals[3];
     stats.waves = waves;
     stats.lds_bytes = L.total;
     stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();

@@ -116,6 +116,10 @@ typedef struct {
                                   queue only stops items splitting in two)    */
     double   time_limit_ms;    /* device time after which every search stops
                                   (results then incomplete), 0 = none         */
+    uint32_t count_work;       /* 1: count the algorithmic work of every
+                                  placement into the stats (an instrumented
+                                  run, for the roofline; plain order only)    */
+    uint32_t reserved;
 } emp_hip_enum_options;
 
 typedef struct {
@@ -145,6 +149,10 @@ typedef struct {
     double   group_ms;         /* device time grouping the work items by puzzle
                                   before the passes (batches without a node
                                   limit), outside dfs_kernel_ms               */
+    uint64_t work_bytes;       /* count_work: algorithmic LDS bytes, 8 M (n + 3)
+                                  per placement (M = ceil(4P/64) u64 words, n
+                                  the placed neighbours of the next index)    */
+    uint64_t work_ops;         /* count_work: integer ops, 2 M (n + 1) + 2 M  */
 } emp_hip_enum_stats;
 
 void emp_hip_enum_default_options(emp_hip_enum_options *options);
@@ -334,6 +342,11 @@ typedef struct {
     uint32_t lds_bytes;        /* LDS per wave                               */
     uint64_t score_evaluations; /* evaluations that scored a candidate       */
     uint64_t rounds;           /* rounds of invalid-candidate removal        */
+    uint64_t free_slot_sum;    /* free positions of the level, summed over the
+                                  evaluations: the reference evaluates a
+                                  candidate with one mask AND per other free
+                                  position and up to four colour-mask ANDs
+                                  (include/CUDA_glutton_max.h:455-714)      */
 } emp_hip_glutton_stats;
 
 /* boards: [n_searches][P], the solution or the deepest situation reached

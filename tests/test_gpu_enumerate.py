@@ -283,3 +283,40 @@ def test_small_step_budget_under_node_limit_is_bounded(gpu_context):
     for (w, h, c), r in list(zip(puzzles, res))[:24]:
         if r["complete"]:
             _check_same(r, o.enumerate_(w, h, c))
+
+
+def _oracle_work(w, h, c):
+    """Algorithmic bytes and ops of the whole enumeration from the oracle's
+    per-depth node counts: 8 M (n + 3) and 2 M (n + 1) + 2 M per placement at
+    index k, n = placed neighbours of index k + 1 (0 after the last)."""
+    ref = o.enumerate_(w, h, c, max_solutions=1)
+    order = emp.order(w, h)
+    index = {xy: i for i, xy in enumerate(order)}
+    P = w * h
+    M = (4 * P + 63) // 64
+    nb = []
+    for i, (x, y) in enumerate(order):
+        nb.append(sum(1 for dx, dy in ((0, -1), (1, 0), (0, 1), (-1, 0))
+                      if 0 <= x + dx < w and 0 <= y + dy < h and index[(x + dx, y + dy)] < i))
+    bytes_, ops = 0, 0
+    for k, cnt in enumerate(int(v) for v in ref["depth_nodes"]):
+        n = nb[k + 1] if k + 1 < P else 0
+        bytes_ += cnt * 8 * M * (n + 3)
+        ops += cnt * (2 * M * (n + 1) + 2 * M)
+    return ref["n_nodes"], bytes_, ops
+
+
+def test_work_accounting_matches_the_oracle_histogram(gpu_context):
+    """The instrumented run (roofline's algorithmic bytes) against the same
+    sum over the oracle's exact depth histogram, single and batched, and the
+    results of an instrumented run are unchanged."""
+    cases = [(5, 5, 3, 4, 3), (6, 6, 3, 6, 1), (8, 8, 6, 16, 11), (7, 5, 3, 5, 6)]
+    puzzles = [(w, h, emp.generate(s, w, h, B, I)[0]) for w, h, B, I, s in cases]
+    expect = [_oracle_work(w, h, c) for w, h, c in puzzles]
+    print("case: work accounting", flush=True)
+    for (w, h, c), (nodes, b, ops) in zip(puzzles, expect):
+        (r,), st = gpu_context.enumerate([(w, h, c)], count_work=True, step_budget=1 << 12)
+        assert r["n_nodes"] == nodes and st["work_bytes"] == b and st["work_ops"] == ops
+    res, st = gpu_context.enumerate(puzzles, count_work=True, use_graph=True)
+    assert [r["n_nodes"] for r in res] == [e[0] for e in expect]
+    assert st["work_bytes"] == sum(e[1] for e in expect) and st["work_ops"] == sum(e[2] for e in expect)
