@@ -555,10 +555,15 @@ struct Wave {
     }
 
     // Validity of candidate (slot i, piece p, orientation o) against the live
-    // masks (remove_invalid_transitions).
-    __device__ bool valid(uint32_t i, uint32_t p, uint32_t o) {
+    // masks (remove_invalid_transitions), for the lanes with `active` set.
+    // Called by the whole wave: the walk over the at-risk list reads other
+    // lanes' entries (readlane), which is only defined while every lane
+    // executes it, so that walk runs with all lanes and the per-lane work
+    // around it is predicated.
+    __device__ bool valid(uint32_t i, uint32_t p, uint32_t o, bool active) {
         const uint32_t nc = scratch[1];
-        if (a.max_colour < 64) {
+        bool ok = active, decided = !active;
+        if (active && a.max_colour < 64) {
             // Fast form: only the thin colours of a side need the count; a
             // piece without a single-piece slot passes the single check; with
             // no piece at risk and every free neighbour constrained nothing
@@ -577,31 +582,45 @@ struct Wave {
                 if (con && ((thin[l] >> col) & 1))
                     invalid |= cnt[l * ncol + col] == facing(nibble(nbw(l), p), pc_p, l, col);
             }
-            if (invalid) return false;
-            if (((single_set[p >> 6] >> (p & 63)) & 1) && single[p] > own_single) return false;
-            if (nc == 0 && bounded) return true;
+            if (invalid || (((single_set[p >> 6] >> (p & 63)) & 1) && single[p] > own_single)) {
+                ok = false;
+                decided = true;
+            } else if (nc == 0 && bounded) {
+                decided = true;
+            }
         }
         Cand cd;
-        if (!neighbours(i, p, o, cd)) return false;
-        // Only the critical pieces at risk in this slot (lb = 0) can be left
-        // without option (slot context).
-        uint32_t k = 0;
-        if (cd.bounded)
-            for (; k < nc && k < kWave; ++k) {
-                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)rk_q, (int)k);
-                if (q == p) continue;
-                if (!matches(cd, __builtin_amdgcn_readlane((int)rk_f0, (int)k),
-                             __builtin_amdgcn_readlane((int)rk_f1, (int)k),
-                             __builtin_amdgcn_readlane((int)rk_f2, (int)k),
-                             __builtin_amdgcn_readlane((int)rk_f3, (int)k)))
-                    return false;
-            }
-        for (; k < nc; ++k) {
-            const uint32_t q = risk[k];
-            if (q == p) continue;
-            if ((uint32_t)T[q] == lost(i, cd, q)) return false;
+        if (!decided && !neighbours(i, p, o, cd)) {
+            ok = false;
+            decided = true;
         }
-        return true;
+        // Only the critical pieces at risk in this slot (lb = 0) can be left
+        // without option (slot context); lane k holds the k-th of the first 64.
+        bool walking = !decided && cd.bounded;
+        const uint32_t listed = nc < kWave ? nc : kWave;
+        for (uint32_t k = 0; k < listed; ++k) {
+            if (!__ballot(walking)) break;
+            const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)rk_q, (int)k);
+            const uint32_t f0 = __builtin_amdgcn_readlane((int)rk_f0, (int)k);
+            const uint32_t f1 = __builtin_amdgcn_readlane((int)rk_f1, (int)k);
+            const uint32_t f2 = __builtin_amdgcn_readlane((int)rk_f2, (int)k);
+            const uint32_t f3 = __builtin_amdgcn_readlane((int)rk_f3, (int)k);
+            if (walking && q != p && !matches(cd, f0, f1, f2, f3)) {
+                ok = false;
+                decided = true;
+                walking = false;
+            }
+        }
+        if (!decided)
+            for (uint32_t k = cd.bounded ? listed : 0; k < nc; ++k) {
+                const uint32_t q = risk[k];
+                if (q == p) continue;
+                if ((uint32_t)T[q] == lost(i, cd, q)) {
+                    ok = false;
+                    break;
+                }
+            }
+        return ok;
     }
 
     // Score total of a candidate whose neighbour part is cd (the algebraic
@@ -611,9 +630,12 @@ struct Wave {
     }
 
     // Min / max of the per-slot and per-piece option counts after the move,
-    // (max << 16) + min; false when the move leaves a piece without option.
-    // Needs the slot's scoring lists (score_lists()).
-    __device__ bool score_minmax(uint32_t i, uint32_t p, const Cand &cd, uint32_t &minmax) {
+    // (max << 16) + min, for the lanes with `active` set; false when the move
+    // leaves a piece without option (and for inactive lanes).  Needs the
+    // slot's scoring lists (score_lists()).  Called by the whole wave: the
+    // walks over the bound lists read other lanes' entries (readlane), so they
+    // run with all lanes and the per-lane work in them is predicated.
+    __device__ bool score_minmax(uint32_t i, uint32_t p, const Cand &cd, uint32_t &minmax, bool active) {
         const uint32_t nn = n();
         const uint32_t pw = p >> 4;
         uint32_t mn = cd.mn, mx = cd.mx;
@@ -638,7 +660,7 @@ struct Wave {
                     fold(pc[j] - nib(M[j * a.nw + pw], p));
                 }
         };
-        if (nn > 1) {
+        if (active && nn > 1) {
             const uint32_t l = lo[p], h = hi[p];
             if (!excluded((l >> 11) & 0xFF)) {
                 mn = (l & 0x7FF) < mn ? (l & 0x7FF) : mn;
@@ -660,19 +682,22 @@ struct Wave {
         // (or when a free neighbour is left unconstrained, which the bounds and
         // the listed colours do not cover), every available piece.
         uint32_t pmn = 0xFFFFFFFFu, pmx = 0;
-        bool open = true;
-        for (uint32_t r = 0; r < kWave && cd.bounded; ++r) {
+        bool open = active, walking = active && cd.bounded;
+        for (uint32_t r = 0; r < kWave; ++r) {
+            if (!__ballot(walking)) break;
             const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_lo, (int)r);
+            const uint32_t f0 = __builtin_amdgcn_readlane((int)lo_f0, (int)r);
+            const uint32_t f1 = __builtin_amdgcn_readlane((int)lo_f1, (int)r);
+            const uint32_t f2 = __builtin_amdgcn_readlane((int)lo_f2, (int)r);
+            const uint32_t f3 = __builtin_amdgcn_readlane((int)lo_f3, (int)r);
+            if (!walking) continue;
             if (k == 0xFFFFFFFFu || (k >> 8) >= pmn) {
                 open = false;
-                break;
+                walking = false;
+                continue;
             }
-            const uint32_t q = k & 0xFF;
-            if (q == p) continue;
-            const uint32_t c = (k >> 8) + matches(cd, __builtin_amdgcn_readlane((int)lo_f0, (int)r),
-                                                  __builtin_amdgcn_readlane((int)lo_f1, (int)r),
-                                                  __builtin_amdgcn_readlane((int)lo_f2, (int)r),
-                                                  __builtin_amdgcn_readlane((int)lo_f3, (int)r));
+            if ((k & 0xFF) == p) continue;
+            const uint32_t c = (k >> 8) + matches(cd, f0, f1, f2, f3);
             pmn = c < pmn ? c : pmn;
         }
         if (open)
@@ -681,35 +706,41 @@ struct Wave {
                     const uint32_t c = (uint32_t)T[q] - lost(i, cd, q);
                     pmn = c < pmn ? c : pmn;
                 }
-        if (pmn == 0) return false;
+        const bool ok = active && pmn != 0;
         // The max in steps of 4 listed pieces (independent, so their reads
         // and compares overlap); a piece past the stopping bound cannot raise
         // the max, so taking whole steps changes nothing.
-        open = !cd.bounded;
-        if (cd.bounded)
-            for (uint32_t r = 0; r < kWave; r += 4) {
-                uint32_t last = 0;
+        open = ok && !cd.bounded;
+        walking = ok && cd.bounded;
+        for (uint32_t r = 0; r < kWave; r += 4) {
+            if (!__ballot(walking)) break;
+            uint32_t k[4], lb[4], f0[4], f1[4], f2[4], f3[4];
 #pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key_hi, (int)(r + e));
-                    last = k;
-                    if (k == 0xFFFFFFFFu || (k & 0xFF) == p) continue;
-                    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)(r + e)) +
-                                       matches(cd, __builtin_amdgcn_readlane((int)hi_f0, (int)(r + e)),
-                                               __builtin_amdgcn_readlane((int)hi_f1, (int)(r + e)),
-                                               __builtin_amdgcn_readlane((int)hi_f2, (int)(r + e)),
-                                               __builtin_amdgcn_readlane((int)hi_f3, (int)(r + e)));
-                    pmx = c > pmx ? c : pmx;
-                }
-                if (last == 0xFFFFFFFFu || 0xFFFFu - (last >> 8) <= pmx) break;
-                if (r + 4 == kWave) open = true;
+            for (uint32_t e = 0; e < 4; ++e) {
+                k[e] = (uint32_t)__builtin_amdgcn_readlane((int)key_hi, (int)(r + e));
+                lb[e] = (uint32_t)__builtin_amdgcn_readlane((int)hi_lb, (int)(r + e));
+                f0[e] = __builtin_amdgcn_readlane((int)hi_f0, (int)(r + e));
+                f1[e] = __builtin_amdgcn_readlane((int)hi_f1, (int)(r + e));
+                f2[e] = __builtin_amdgcn_readlane((int)hi_f2, (int)(r + e));
+                f3[e] = __builtin_amdgcn_readlane((int)hi_f3, (int)(r + e));
             }
+            if (!walking) continue;
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                if (k[e] == 0xFFFFFFFFu || (k[e] & 0xFF) == p) continue;
+                const uint32_t c = lb[e] + matches(cd, f0[e], f1[e], f2[e], f3[e]);
+                pmx = c > pmx ? c : pmx;
+            }
+            if (k[3] == 0xFFFFFFFFu || 0xFFFFu - (k[3] >> 8) <= pmx) walking = false;
+            else if (r + 4 == kWave) open = true;
+        }
         if (open)
             for (uint32_t q = 0; q < a.P; ++q)
                 if (q != p && piece_avail(q)) {
                     const uint32_t c = (uint32_t)T[q] - lost(i, cd, q);
                     pmx = c > pmx ? c : pmx;
                 }
+        if (!ok) return false;
         mn = pmn < mn ? pmn : mn;
         mx = pmx > mx ? pmx : mx;
         minmax = (mx << 16) + mn;
@@ -1103,7 +1134,6 @@ struct Wave {
                     if (fill == 0) break;   // the remaining words are empty
                     __syncthreads();
                     uint32_t bit = 0;
-                    bool bad = false;
                     if (lane < fill) {
                         bit = lo[lane];
                         // neighbour words of the candidate's word (the
@@ -1118,8 +1148,9 @@ struct Wave {
                         nbw2 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
                         j = field(snbp, 3);
                         nbw3 = j != 0xFFFFu ? M[j * a.nw + bw] : 0ull;
-                        bad = !valid(i, bit >> 2, bit & 3);
                     }
+                    const bool passes = valid(i, bit >> 2, bit & 3, lane < fill);   // whole wave
+                    const bool bad = lane < fill && !passes;
                     __syncthreads();
                     const uint64_t bads = __ballot(bad);
                     // words up to the first with a removal are final
@@ -1251,7 +1282,8 @@ struct Wave {
                     lists = true;
                 }
                 uint32_t mm = 0;
-                if (!need || !score_minmax(j, p, cd, mm)) continue;
+                const bool scored = score_minmax(j, p, cd, mm, need);
+                if (!scored) continue;
                 const uint64_t key = last_level ? 0 : (((uint64_t)total << 32) | mm);
                 const uint32_t idx = (j << 16) | bit;
                 // higher key wins; on equal keys the earlier candidate, except
