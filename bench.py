@@ -28,6 +28,10 @@ Prints one JSON line:
                    same DFS.
   batched          the same fields for the batched leg (cpu_baseline: the
                    oracle in a pool of host processes, one instance each).
+  learned          the learned-ordering leg at a reduced size: a scorer
+                   trained on 8 seeds, then the mean max depth on 4 held-out
+                   16x16 seeds in equal device time (300 ms) for the plain,
+                   trained and random orders.
 """
 import argparse
 import json
@@ -273,6 +277,34 @@ def bench_c4(args, ctx):
     }
 
 
+def bench_learned(ctx, time_ms):
+    """The learned-ordering leg of the default line, at a reduced size: a
+    scorer trained on depth-probe traces of 8 training seeds (64 situations
+    each, 5,000-placement probes, 20 epochs), then the max depth reached on 4
+    held-out 16x16 seeds in equal device time by the plain order, the trained
+    order (ranked kernel, and pieces renumbered by score) and random orders.
+    Every order searches the same tree (the ranked and renumbered searches
+    are exact reorderings)."""
+    from edge_matching_puzzle_amd import learned
+    t0 = time.perf_counter()
+    res = learned.experiment(ctx, situations_per_seed=64, probe_nodes=5000, epochs=20, time_ms=time_ms,
+                             train_seeds=learned.TRAIN_SEEDS[:8], heldout_seeds=learned.HELDOUT_SEEDS[:4],
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    res.pop("params")
+    per = res.pop("per_seed")
+    orders = ("plain", "learned", "learned_relabel", "random", "random_relabel")
+    return {
+        "workload": "16x16, 5 border / 17 interior colours; 8 training seeds, 4 held-out seeds, "
+                    "%.0f ms device time per search" % time_ms,
+        "mean_max_depth": {k: res[k]["mean_max_depth"] for k in orders},
+        "nodes_per_s": {k: res[k]["nodes_per_s"] for k in orders},
+        "per_seed_max_depth": [{k: r[k]["max_depth"] for k in orders} | {"seed": r["seed"]} for r in per],
+        "trace_rows": res["trace_rows"], "heldout_mse": res["heldout_mse"],
+        "heldout_mse_of_mean": res["heldout_mse_of_mean"], "loss_first": res["loss_first"],
+        "loss_last": res["loss_last"], "seconds": time.perf_counter() - t0,
+    }
+
+
 def hbm_traffic():
     """HBM bytes per dfs_kernel launch of the plain step, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by the reading
@@ -299,6 +331,8 @@ def main():
     ap.add_argument("--instances", type=int, default=4096, help="batched instances")
     ap.add_argument("--node-limit", type=int, default=C2_NODE_LIMIT, help="batched nodes per instance")
     ap.add_argument("--batched-steps", type=int, default=2, help="batched leg steps in the default line (0: none)")
+    ap.add_argument("--learned-ms", type=float, default=300.0,
+                    help="device time per search of the learned leg in the default line (0: none)")
     ap.add_argument("--prefixes", type=int, default=4096, help="C3 prefixes")
     ap.add_argument("--push-budget", type=int, default=16, help="C3 pushes per prefix")
     ap.add_argument("--chunk", type=int, default=0, help="C3 prefixes per launch (0: all in one launch)")
@@ -389,6 +423,8 @@ def main():
     if args.batched_steps:
         line["batched"] = bench_batched(ctx, args.batched_steps, 1, getattr(args, "cpu_c2", None),
                                         args.instances, args.node_limit)
+    if args.learned_ms > 0:
+        line["learned"] = bench_learned(ctx, args.learned_ms)
     # The same two peaks measured on this device by stream kernels (after the
     # timed regions): the fractions against them sit beside the published ones.
     pk = ctx.peaks()

@@ -13,7 +13,9 @@ Features of a candidate (piece p at orientation o, at strategy index k):
   * its four edge colours as placed (north, east, south, west), one-hot over
     colours 1..22 (the frame colour 0 sets no bit): 88 inputs;
   * the colour histogram of the pieces still to place (edges of colour c over
-    all their edges): 22 inputs;
+    all their edges): 22 inputs.  Traces use the situation's own remaining
+    pieces; a static rank table, which serves every situation at index k,
+    uses their expectation at k (expected_histogram);
   * the strategy index, sin / cos of pi 2^i k / P for i = 0..7: 16 inputs.
 Labels: the most pieces placed in one situation of the plain search below
 the candidate within a node budget (emp_hip_probe), less the pieces placed
@@ -78,6 +80,34 @@ def histogram(colours, remaining):
     np.add.at(h, c - 1, 1.0)
     total = max(4 * int(np.count_nonzero(remaining)), 1)
     return h / total
+
+
+def expected_histogram(colours, width, height, k):
+    """Expected colour histogram of the pieces still to place at strategy
+    index k: a cell of kind K takes a piece of kind K, so kind K keeps
+    (cells of kind K at indices >= k) / (cells of kind K) of its pieces.  On
+    the spiral the frame comes first, so past the frame only interior
+    colours remain."""
+    c = np.asarray(colours).reshape(-1, 4)
+    kind, _ = piece_kinds(c)
+    order_xy = emp.order(width, height)
+    left = np.zeros(3)
+    total = np.zeros(3)
+    for i, (x, y) in enumerate(order_xy):
+        K = cell_kind(x, y, width, height)
+        total[K] += 1
+        left[K] += i >= k
+    h = np.zeros(N_COLOURS, dtype=np.float64)
+    edges = 0.0
+    for K in range(3):
+        if total[K] == 0:
+            continue
+        share = left[K] / total[K]
+        ck = c[kind == K].reshape(-1)
+        edges += 4 * share * np.count_nonzero(kind == K)
+        ck = ck[(ck >= 1) & (ck <= N_COLOURS)]
+        np.add.at(h, ck - 1, share)
+    return (h / max(edges, 1.0)).astype(np.float32)
 
 
 def position_embedding(k, P):
@@ -243,22 +273,22 @@ def init_params(seed):
     return p
 
 
-def rank_table(ctx, params, width, height, colours, remaining=None):
+def rank_table(ctx, params, width, height, colours):
     """uint16 [P, T] visit ranks (T = 4 x pieces of the largest kind): at each
     strategy index the transitions of the cell's kind by decreasing score
-    (ties by transition), then the unused entries."""
+    (ties by transition), then the unused entries.  The histogram input at
+    index k is the expected one (expected_histogram)."""
     colours = np.asarray(colours).reshape(-1, 4)
     P = width * height
     _, by_kind = piece_kinds(colours)
     T = 4 * max(len(b) for b in by_kind)
     order_xy = emp.order(width, height)
-    hist = histogram(colours, np.ones(P, bool) if remaining is None else remaining)
     feats, spans = [], []
     for k, (x, y) in enumerate(order_xy):
         K = cell_kind(x, y, width, height)
         cands = [(int(p), o) for p in by_kind[K] for o in range(4)]
         spans.append((K, len(cands)))
-        feats.append(features(colours, P, k, cands, hist))
+        feats.append(features(colours, P, k, cands, expected_histogram(colours, width, height, k)))
     scores = ctx.mlp_score(params, np.concatenate(feats))
     ranks = np.zeros((P, T), dtype=np.uint16)
     at = 0
@@ -271,30 +301,52 @@ def rank_table(ctx, params, width, height, colours, remaining=None):
     return ranks
 
 
+def relabel_by(colours, priority):
+    """The puzzle with its pieces renumbered by decreasing priority (ties by
+    id), except that the first corner piece stays the first corner: index 0
+    only ever takes its first candidate (include/emp_strategy.h:389), so that
+    piece fixes the tree searched, and keeping it leaves the tree -- node
+    count and solutions up to identical pieces -- unchanged; only the visit
+    order changes.  Returns (colours in the new numbering, old id of each new
+    id)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    kind, by_kind = piece_kinds(colours)
+    prio = np.asarray(priority, dtype=np.float64).copy()
+    prio[by_kind[2][0]] = np.inf
+    perm = np.lexsort((np.arange(len(prio)), -prio))
+    return colours[perm].copy(), perm
+
+
+def shuffled(colours, seed):
+    """A random renumbering that keeps the tree (relabel_by)."""
+    colours = np.asarray(colours).reshape(-1, 4)
+    return relabel_by(colours, np.random.default_rng(seed).random(len(colours)))
+
+
 def relabelled(ctx, params, width, height, colours):
     """The puzzle with its pieces renumbered by decreasing score, so that the
     plain order (kind index = id order within a kind) visits the better
     scored pieces first at full speed.  A piece's score is its best
-    orientation's score averaged over the strategy indices of its kind.
-    Returns (colours in the new numbering, old id of each new id)."""
+    orientation's score averaged over the strategy indices of its kind; the
+    first corner piece keeps its place (relabel_by).  Returns (colours in the
+    new numbering, old id of each new id)."""
     colours = np.asarray(colours).reshape(-1, 4)
     P = width * height
     _, by_kind = piece_kinds(colours)
     order_xy = emp.order(width, height)
-    hist = histogram(colours, np.ones(P, bool))
     prio = np.zeros(P, dtype=np.float64)
     for K in range(3):
         ks = [k for k, (x, y) in enumerate(order_xy) if cell_kind(x, y, width, height) == K]
         if not ks or len(by_kind[K]) == 0:
             continue
         cands = [(int(p), o) for p in by_kind[K] for o in range(4)]
-        feats = np.concatenate([features(colours, P, k, cands, hist) for k in ks])
+        feats = np.concatenate([features(colours, P, k, cands, expected_histogram(colours, width, height, k))
+                                for k in ks])
         sc = ctx.mlp_score(params, feats).reshape(len(ks), len(by_kind[K]), 4)
         for i, p in enumerate(by_kind[K]):
             kept = list(orientations_kept(colours, p))
             prio[p] = sc[:, i, kept].max(axis=1).mean()
-    perm = np.lexsort((np.arange(P), -prio))
-    return colours[perm].copy(), perm
+    return relabel_by(colours, prio)
 
 
 def random_rank_table(width, height, colours, seed):
@@ -353,9 +405,9 @@ def experiment(ctx, situations_per_seed=64, probe_nodes=20000, epochs=10, batch=
         rank_s = time.perf_counter() - t0
         out = {"seed": seed, "rank_table_s": rank_s}
         relab, _ = relabelled(ctx, params, w, h, c)
-        shuffled = np.asarray(c).reshape(-1, 4)[np.random.default_rng(seed).permutation(w * h)]
+        shuf, _ = shuffled(c, seed)
         for name, cc, rk in (("plain", c, None), ("learned", c, [ranks]), ("random", c, [random_rank_table(w, h, c, seed)]),
-                             ("learned_relabel", relab, None), ("random_relabel", shuffled, None)):
+                             ("learned_relabel", relab, None), ("random_relabel", shuf, None)):
             (r,), st = ctx.enumerate([(w, h, cc)], max_solutions=0, ranks=rk, time_limit_ms=time_ms)
             out[name] = {"max_depth": r["max_depth"], "nodes": r["n_nodes"], "kernel_ms": st["dfs_kernel_ms"]}
         rows.append(out)
