@@ -115,6 +115,7 @@ SIGNATURES = {
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
     "emp_hip_backtrack_variables": (_I32, [_P, _U32, _P]),
     "emp_hip_glutton_wide": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _U64, _P, _P, _P, _P, _P]),
+    "emp_hip_situation_profile": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
     "emp_hip_stream_copy": (_I32, [_P, _U64, _P]),
@@ -421,6 +422,25 @@ class Context:
         m = min(nc.value, capacity)
         return {"parent_masks": pm, "parent_status": ps, "n_children": nc.value, "child_parent": cp[:m].copy(),
                 "child_move": cmv[:m].copy(), "child_masks": cmask[:m].copy(), "child_status": cs[:m].copy()}
+
+    def situation_profile(self, width, height, colours, board):
+        """Capability profiles of a situation (row-major board, 0xFFFF
+        empty): level, glutton [10, P + 1] (total, max, min, vector_op,
+        rvector_op; each min then max), surface [level + 1, 2P] sorted
+        profiles, row_index [level + 1], n_variables."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        board = np.ascontiguousarray(board, dtype=np.uint16)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        P = width * height
+        level, nvar = ctypes.c_uint32(), ctypes.c_uint32()
+        glutton = np.zeros((10, P + 1), dtype=np.uint32)
+        surface = np.zeros((P + 1, 2 * P), dtype=np.uint32)
+        row_index = np.zeros(P + 1, dtype=np.uint32)
+        _check(lib().emp_hip_situation_profile(self._ctx, ctypes.byref(pz), _ptr(board), ctypes.byref(level),
+                                               _ptr(glutton), _ptr(surface), _ptr(row_index), ctypes.byref(nvar)))
+        L = level.value
+        return {"level": L, "glutton": glutton, "surface": surface[:L + 1].copy(),
+                "row_index": row_index[:L + 1].copy(), "n_variables": nvar.value}
 
     def backtrack_trace(self, width, height, colours, order="basic", max_selections=0, trace_capacity=1 << 16):
         """The reference's CUDA backtracker: first solution and the selection

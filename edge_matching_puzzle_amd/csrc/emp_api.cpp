@@ -20,6 +20,7 @@
 #include "emp_glutton_wide.h"
 #include "emp_mlp.h"
 #include "emp_model.h"
+#include "emp_profile.h"
 #include "emp_results.h"
 
 #include <unordered_set>
@@ -673,6 +674,34 @@ int emp_hip_backtrack_variables(const emp_hip_puzzle *puzzle, uint32_t order, ui
         uint64_t n = 0;
         for (uint64_t w : emp::backtrack_capability(*pz, (int)order)) n += (uint64_t)__builtin_popcountll(w);
         *n_variables = n;
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_situation_profile(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint16_t *situation,
+                              uint32_t *level, uint32_t *glutton, uint32_t *surface, uint32_t *row_index,
+                              uint32_t *n_variables) {
+    return guarded([&] {
+        if (!context || !puzzle || !situation || !level || !glutton || !surface || !row_index)
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        emp::ProfileOutcome out;
+        try {
+            emp::situation_profile_on_device(context->stream, context->workspace, *pz, situation, out);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        *level = out.level;
+        if (n_variables) *n_variables = out.n_variables;
+        std::copy(out.glutton.begin(), out.glutton.end(), glutton);
+        std::copy(out.surface.begin(), out.surface.end(), surface);
+        std::copy(out.row_index.begin(), out.row_index.end(), row_index);
         return EMP_HIP_OK;
     });
 }

@@ -3,7 +3,8 @@
 //                        [--strategy=basic|spiral] [--hint=<depth>]
 //                        [--enable_gui=false] [--dump_file=] [--initial_situation=]
 // new_strategy uses the spiral order and CUDA_backtracker (and
-// CUDA_enumerate) the basic order unless --strategy is given
+// CUDA_enumerate, situation_profile) the basic order unless --strategy is
+// given
 // (src/main.cpp:108-119); the dump file defaults to results.bin (:74).
 // There is no GUI: --enable_gui=true is refused.  Errors print
 // "ERROR : <what> at <file>:<line>" and return -1 (src/main.cpp:137-146).
@@ -36,7 +37,10 @@ int main(int argc, char **argv) {
             } else throw logic_exception("Unknown option \"" + a + "\"", __LINE__, __FILE__);
         }
         const std::string feature = argv[3];
-        if (strategy.empty()) strategy = feature == "CUDA_backtracker" || feature == "CUDA_enumerate" ? "basic" : "spiral";
+        if (strategy.empty())
+            strategy = feature == "CUDA_backtracker" || feature == "CUDA_enumerate" || feature == "situation_profile"
+                           ? "basic"
+                           : "spiral";
         if (dump_file_set && feature != "new_strategy")
             throw logic_exception("--dump_file is only written by new_strategy", __LINE__, __FILE__);
         if (!dump_file_set) dump_file = "results.bin";

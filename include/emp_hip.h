@@ -292,6 +292,34 @@ int emp_hip_backtrack_trace(emp_hip_context *context, const emp_hip_puzzle *puzz
 int emp_hip_backtrack_variables(const emp_hip_puzzle *puzzle, uint32_t order, uint64_t *n_variables);
 
 /*
+ * Capability profiles of a situation (the reference's situation_profile
+ * feature).  The system equation's capability -- per position and per piece
+ * the live (piece | position, orientation) variables -- is followed through
+ * the situation's placements, positions in the basic order; a profile is the
+ * 2P bit counts, sorted (include/situation_capability.h:145-155).
+ * situation: row-major board, ((id - 1) << 2) | o, 0xFFFF empty.
+ *   *level       pieces the situation places
+ *   glutton      [10][P + 1]: compute_glutton (include/feature_situation_profile.h
+ *                :388-483) for total, max, min, vector_op, rvector_op, each as
+ *                a min then a max search -- the total after each placement,
+ *                0 for an invalid one, padded with 0
+ *   surface      [P + 1][2P], rows 0..level: the profile before each placement
+ *                in strategy order, then at the end (treat_situation_capability,
+ *                :290-305)
+ *   row_index    [P + 1]: the strategy index of each row's placement (P for the
+ *                last row); the reference's line plot adds P - row_index
+ *   n_variables  variables of the system equation
+ * EMP_HIP_BAD_ARGUMENT when a placement is not a variable of its position;
+ * EMP_HIP_UNSUPPORTED above 256 pieces.
+ * Replaces: the computation of feature_situation_profile::template_run
+ * (include/feature_situation_profile.h:150-183); the host feature writes its
+ * files.
+ */
+int emp_hip_situation_profile(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint16_t *situation,
+                              uint32_t *level, uint32_t *glutton, uint32_t *surface, uint32_t *row_index,
+                              uint32_t *n_variables);
+
+/*
  * Breadth expansion: every situation the enumeration visits with `depth`
  * pieces placed (indices 0..depth-1 of the order), level by level on the GPU.
  * boards: [capacity][P] row-major (cells beyond the depth empty); *n_situations

@@ -1172,3 +1172,180 @@ int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// situation_profile: feature_situation_profile::template_run and
+// compute_glutton (include/feature_situation_profile.h:150-483).  A
+// capability is the set of live variables: each variable sits in the info of
+// its position and in the info of its piece, so a profile -- the 2P bit
+// counts, sorted (include/situation_capability.h:145-155) -- counts the live
+// variables per position and per piece.
+// ---------------------------------------------------------------------------
+struct Profiler {
+    struct Var {
+        uint32_t pos, x, y, piece, o;
+    };
+    const Model &m;
+    uint32_t P;
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    std::vector<Var> vars;
+    std::vector<int32_t> var_at;   // [P cells][P pieces][4] -> variable index or -1
+
+    Profiler(const Model &model) : m(model), P(model.P) {
+        order = make_order(m.W, m.H, ORACLE_ORDER_BASIC);
+        var_at.assign((size_t)P * P * 4, -1);
+        // create_variables: every kept orientation of a piece of the cell's
+        // kind that shows the frame exactly on the frame sides
+        for (uint32_t i = 0; i < P; ++i) {
+            const uint32_t x = order[i].first, y = order[i].second;
+            const bool off[4] = {y == 0, x + 1 == m.W, y + 1 == m.H, x == 0};
+            for (uint32_t q = 0; q < P; ++q) {
+                const Piece &p = m.pieces[q];
+                for (unsigned o = 0; o < 4; ++o) {
+                    if (o & ((1u << p.sim) - 1u)) continue;
+                    bool ok = true;
+                    for (unsigned s = 0; s < 4 && ok; ++s) ok = (shown(p, s, o) == 0) == off[s];
+                    if (!ok) continue;
+                    var_at[((size_t)(y * m.W + x) * P + q) * 4 + o] = (int32_t)vars.size();
+                    vars.push_back({i, x, y, q, o});
+                }
+            }
+        }
+    }
+
+    // Does the transition of v (prepare_transitions) mask u?  Same position,
+    // same piece, or an adjacent cell whose shared edge shows another colour
+    // (treat_piece_relation_equation, pieces differing).
+    bool masks(const Var &v, const Var &u) const {
+        if (u.pos == v.pos || u.piece == v.piece) return true;
+        int side = -1;   // side of v facing u
+        if (u.x == v.x && u.y + 1 == v.y) side = 0;
+        else if (u.x == v.x + 1 && u.y == v.y) side = 1;
+        else if (u.x == v.x && u.y == v.y + 1) side = 2;
+        else if (u.x + 1 == v.x && u.y == v.y) side = 3;
+        if (side < 0) return false;
+        return shown(m.pieces[v.piece], (unsigned)side, v.o) != shown(m.pieces[u.piece], ((unsigned)side + 2) & 3, u.o);
+    }
+
+    void apply(std::vector<char> &live, uint32_t v) const {
+        for (size_t u = 0; u < vars.size(); ++u)
+            if (live[u] && masks(vars[v], vars[u])) live[u] = 0;
+    }
+
+    std::vector<uint32_t> profile(const std::vector<char> &live) const {
+        std::vector<uint32_t> c(2 * P, 0);
+        for (size_t u = 0; u < vars.size(); ++u)
+            if (live[u]) {
+                ++c[vars[u].pos];
+                ++c[P + vars[u].piece];
+            }
+        std::sort(c.begin(), c.end());
+        return c;
+    }
+
+    // situation_profile's comparisons; a profile carries its level.  The
+    // total is accumulated in 32-bit arithmetic, as the reference's is
+    // (the reference's min / max start profiles fill every value with 0 or
+    // 0xFFFFFFFF, whose total wraps).
+    struct Prof {
+        uint32_t level;
+        std::vector<uint32_t> v;
+        uint32_t total() const {
+            uint32_t t = 0;
+            for (uint32_t x : v) t += x;
+            return t;
+        }
+        uint32_t max() const { return v.back(); }
+        uint32_t min() const { return 2 * level < v.size() ? v[2 * level] : 0; }
+        bool valid() const { return level < v.size() / 2 ? v[2 * level] != 0 : true; }
+    };
+    static bool less(int criterion, const Prof &a, const Prof &b) {
+        switch (criterion) {
+            case 0: return a.total() < b.total();
+            case 1: return std::make_pair(a.max(), a.total()) < std::make_pair(b.max(), b.total());
+            case 2: return std::make_pair(a.min(), a.total()) < std::make_pair(b.min(), b.total());
+            default: return a.v < b.v;   // vector_op, and rvector_op, which the reference compares the same way
+        }
+    }
+
+    // compute_glutton for one criterion: from the level-0 capability, place
+    // the situation's pieces one at a time, each time the one whose
+    // capability is the extremum (the first invalid one at once); the total
+    // after each placement (0 for an invalid one), padded to P + 1 values.
+    std::vector<uint32_t> glutton(const std::vector<char> &live0, const std::vector<uint32_t> &placed_vars,
+                                  int criterion, bool want_max) const {
+        std::vector<uint32_t> out;
+        std::vector<char> live = live0;
+        out.push_back(Prof{0, profile(live)}.total());
+        std::vector<char> used(placed_vars.size(), 0);
+        for (size_t level = 0; level < placed_vars.size(); ++level) {
+            Prof ext{0, std::vector<uint32_t>(2 * P, want_max ? 0u : 0xFFFFFFFFu)};
+            uint32_t ext_score = 0;
+            int chosen = -1;
+            std::vector<char> chosen_live;
+            for (size_t c = 0; c < placed_vars.size(); ++c) {
+                if (used[c]) continue;
+                std::vector<char> next = live;
+                apply(next, placed_vars[c]);
+                Prof cur{(uint32_t)level + 1, profile(next)};
+                const bool ok = cur.valid();
+                const uint32_t score = ok ? cur.total() : 0;
+                if (!ok || want_max != less(criterion, cur, ext)) {
+                    ext = cur;
+                    ext_score = score;
+                    chosen = (int)c;
+                    chosen_live = next;
+                    if (!ok) break;
+                }
+            }
+            used[chosen] = 1;
+            live = chosen_live;
+            out.push_back(ext_score);
+        }
+        while (out.size() < P + 1) out.push_back(0);
+        return out;
+    }
+};
+
+int oracle_situation_profile(uint32_t width, uint32_t height, const uint32_t *colours, const uint16_t *situation,
+                             uint32_t *level, uint32_t *glutton, uint32_t *surface, uint32_t *row_index,
+                             uint32_t *n_variables) {
+    Model m;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!situation || !level || !glutton || !surface || !row_index) return ORACLE_BAD_ARGUMENT;
+    Profiler pr(m);
+    const uint32_t P = m.P;
+    if (n_variables) *n_variables = (uint32_t)pr.vars.size();
+    // the situation's placements in strategy order
+    std::vector<uint32_t> placed;
+    std::vector<uint32_t> placed_index;
+    for (uint32_t i = 0; i < P; ++i) {
+        const uint32_t cell = pr.order[i].second * width + pr.order[i].first;
+        const uint16_t b = situation[cell];
+        if (b == 0xFFFF) continue;
+        if ((uint32_t)(b >> 2) >= P) return ORACLE_BAD_ARGUMENT;
+        const int32_t v = pr.var_at[((size_t)cell * P + (b >> 2)) * 4 + (b & 3)];
+        if (v < 0) return ORACLE_BAD_ARGUMENT;
+        placed.push_back((uint32_t)v);
+        placed_index.push_back(i);
+    }
+    *level = (uint32_t)placed.size();
+    std::vector<char> live0(pr.vars.size(), 1);
+    for (int criterion = 0; criterion < 5; ++criterion)
+        for (int mx = 0; mx < 2; ++mx) {
+            const std::vector<uint32_t> g = pr.glutton(live0, placed, criterion, mx != 0);
+            std::copy(g.begin(), g.end(), glutton + (size_t)(2 * criterion + mx) * (P + 1));
+        }
+    // the profile before each placement in strategy order, then at the end
+    std::vector<char> live = live0;
+    for (size_t k = 0; k < placed.size(); ++k) {
+        const std::vector<uint32_t> pf = pr.profile(live);
+        std::copy(pf.begin(), pf.end(), surface + k * 2 * P);
+        row_index[k] = placed_index[k];
+        pr.apply(live, placed[k]);
+    }
+    const std::vector<uint32_t> pf = pr.profile(live);
+    std::copy(pf.begin(), pf.end(), surface + placed.size() * 2 * P);
+    row_index[placed.size()] = P;
+    return ORACLE_OK;
+}

@@ -152,6 +152,27 @@ int oracle_glutton_max(uint32_t width, uint32_t height, const uint32_t *colours,
 int oracle_backtracker(uint32_t width, uint32_t height, const uint32_t *colours, int order, uint16_t *board,
                        uint32_t *found, uint32_t *trace, uint64_t trace_capacity, uint64_t *trace_length);
 
+/*
+ * situation_profile restatement (include/feature_situation_profile.h
+ * :150-483, include/situation_profile.h, include/situation_capability.h) over
+ * the variables and transitions of the system equation
+ * (include/system_equation_for_CUDA.h:70-173, include/emp_variable_generator.h
+ * :561-621), positions in the basic order.  situation: row-major board of the
+ * placed pieces (0xFFFF empty); level: pieces placed.
+ *   glutton  [10][P + 1]  the glutton profiles (total of the capability after
+ *            each placement) for total, max, min, vector_op, rvector_op, each
+ *            searched for the min then the max
+ *   surface  [level + 1][2P]  the sorted capability profile before each
+ *            placement of the situation in strategy order, then at the end
+ *   row_index  [level + 1]  strategy index of each surface row's placement
+ *            (P for the last row)
+ *   n_variables  variables of the system equation
+ * Returns ORACLE_BAD_ARGUMENT when a placement is not a variable.
+ */
+int oracle_situation_profile(uint32_t width, uint32_t height, const uint32_t *colours, const uint16_t *situation,
+                             uint32_t *level, uint32_t *glutton, uint32_t *surface, uint32_t *row_index,
+                             uint32_t *n_variables);
+
 #ifdef __cplusplus
 }
 #endif

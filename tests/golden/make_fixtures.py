@@ -2,9 +2,10 @@
 
 Run in a checkout that has the reference mounted (default /root/reference).
 Only data is copied: the example puzzle definitions, the reference's expected
-output files (results.bin, stats tables, solution lists) and the expected
-stdout strings of each reference test (test.info "expected_stdout_string"
-lines, stored as JSON).  No reference source is copied.
+output files (results.bin, stats tables, solution lists, capability-profile
+plots) and the expected stdout strings of each reference test (test.info
+"expected_stdout_string" lines, stored as JSON).  No reference source is
+copied.
 """
 import json
 import os
@@ -50,6 +51,30 @@ def main():
     with open(os.path.join(OUT, "expected_stdout.json"), "w") as f:
         json.dump(expected, f, indent=1, sort_keys=True)
     print("copied", len(copies), "files; expected strings for", len(expected), "tests")
+    collect_situation_profiles()
+
+
+def collect_situation_profiles():
+    """tests/situation_profile_<n>: every reference output file (VTK line
+    plots and surfaces of the capability profiles) under
+    situation_profile/<n>/, and the --initial_situation argument of each test
+    in situation_profile/situations.json."""
+    situations = {}
+    root = os.path.join(OUT, "situation_profile")
+    for d in sorted(os.listdir(os.path.join(REF, "tests"))):
+        if not d.startswith("situation_profile_"):
+            continue
+        n = d[len("situation_profile_"):]
+        os.makedirs(os.path.join(root, n), exist_ok=True)
+        refdir = os.path.join(REF, "tests", d, "references")
+        for f in sorted(os.listdir(refdir)):
+            shutil.copyfile(os.path.join(refdir, f), os.path.join(root, n, f))
+        for line in open(os.path.join(REF, "tests", d, "test.info")).read().splitlines():
+            if line.startswith("args:") and "--initial_situation=" in line:
+                situations[n] = line.split('--initial_situation="', 1)[1].split('"', 1)[0]
+    with open(os.path.join(root, "situations.json"), "w") as f:
+        json.dump(situations, f, indent=1, sort_keys=True)
+    print("situation profiles:", ", ".join(sorted(situations)))
 
 
 if __name__ == "__main__":

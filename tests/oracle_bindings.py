@@ -44,6 +44,8 @@ def lib():
         _lib.oracle_glutton_masks.restype = i32
         _lib.oracle_glutton_max.argtypes = [u32, u32, p, u32, p, u64, p, p, p, p]
         _lib.oracle_glutton_max.restype = i32
+        _lib.oracle_situation_profile.argtypes = [u32, u32, p, p, p, p, p, p, p]
+        _lib.oracle_situation_profile.restype = i32
     return _lib
 
 
@@ -212,3 +214,22 @@ def situation_string(board, P):
         else:
             out.append(str((v >> 2) + 1).rjust(w - 1) + "NESW"[v & 3])
     return "".join(out)
+
+
+def situation_profile(w, h, colours, board):
+    """Capability profiles of a situation (row-major board, 0xFFFF empty)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    board = np.ascontiguousarray(board, dtype=np.uint16)
+    P = w * h
+    level = np.zeros(1, np.uint32)
+    glutton = np.zeros((10, P + 1), np.uint32)
+    surface = np.zeros((P + 1, 2 * P), np.uint32)
+    row_index = np.zeros(P + 1, np.uint32)
+    nvar = np.zeros(1, np.uint32)
+    st = lib().oracle_situation_profile(w, h, _ptr(colours), _ptr(board), _ptr(level), _ptr(glutton), _ptr(surface),
+                                        _ptr(row_index), _ptr(nvar))
+    if st != OK:
+        raise ValueError("oracle_situation_profile status %d" % st)
+    L = int(level[0])
+    return {"level": L, "glutton": glutton, "surface": surface[:L + 1].copy(), "row_index": row_index[:L + 1].copy(),
+            "n_variables": int(nvar[0])}
