@@ -112,6 +112,10 @@ SIGNATURES = {
     "emp_hip_backtrack_trace": (_I32, [_P, _P, _U32, _U64, _P, _P, _P, _U64, _P]),
     "emp_hip_expand": (_I32, [_P, _P, _U32, _U32, _U64, _P, _P, _P, _P]),
     "emp_hip_glutton_max": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P]),
+    "emp_hip_glutton_stack_words": (_I32, [_U32, _P, _P]),
+    "emp_hip_glutton_max_stacks": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "emp_hip_glutton_stack_to_xml": (_I32, [_P, _P, _P, _P, _U64, _P]),
+    "emp_hip_glutton_stack_from_xml": (_I32, [_P, _P, _U64, _P, _P]),
     "emp_hip_spiral_prefixes": (_I32, [_P, _U32, _U32, _U64, _P, _U32, _P, _P]),
     "emp_hip_backtrack_variables": (_I32, [_P, _U32, _P]),
     "emp_hip_glutton_wide": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _U64, _P, _P, _P, _P, _P]),
@@ -228,6 +232,40 @@ def backtrack_variables(width, height, colours, order="basic"):
     n = ctypes.c_uint64()
     _check(lib().emp_hip_backtrack_variables(ctypes.byref(pz), _ORDERS[order], ctypes.byref(n)))
     return n.value
+
+
+def glutton_stack_words(n_pieces):
+    """(state words, mask words) of one saved glutton stack."""
+    sw, mw = ctypes.c_uint32(), ctypes.c_uint64()
+    _check(lib().emp_hip_glutton_stack_words(n_pieces, ctypes.byref(sw), ctypes.byref(mw)))
+    return sw.value, mw.value
+
+
+def glutton_stack_to_xml(width, height, colours, state, masks):
+    """XML text of one saved glutton stack (the reference converter's
+    element layout)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    state = np.ascontiguousarray(state, dtype=np.uint32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint64)
+    size = ctypes.c_uint64()
+    _check(lib().emp_hip_glutton_stack_to_xml(ctypes.byref(pz), _ptr(state), _ptr(masks), None, 0, ctypes.byref(size)))
+    buf = ctypes.create_string_buffer(size.value)
+    _check(lib().emp_hip_glutton_stack_to_xml(ctypes.byref(pz), _ptr(state), _ptr(masks), buf, size.value,
+                                              ctypes.byref(size)))
+    return buf.raw[:size.value].decode()
+
+
+def glutton_stack_from_xml(width, height, colours, xml):
+    """(state, masks) of an XML glutton stack."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    pz = Puzzle(width, height, colours.ctypes.data)
+    sw, mw = glutton_stack_words(width * height)
+    state = np.zeros(sw, dtype=np.uint32)
+    masks = np.zeros(mw, dtype=np.uint64)
+    data = xml.encode() if isinstance(xml, str) else bytes(xml)
+    _check(lib().emp_hip_glutton_stack_from_xml(ctypes.byref(pz), data, len(data), _ptr(state), _ptr(masks)))
+    return state, masks
 
 
 def resume_item(width, height, colours, board, order="spiral"):
@@ -370,26 +408,46 @@ class Context:
         return loss.value, g
 
     def glutton_max(self, width, height, colours, prefixes=None, push_budget=0, waves_per_cu=0,
-                    n_searches=None):
-        """One glutton_max search per prefix row (packed moves); returns
-        (results, boards [n, P] uint16, stats)."""
+                    n_searches=None, stacks=None, save_stacks=False):
+        """One glutton_max search per prefix row (packed moves), or per
+        saved stack (stacks = (state [n, state_words], masks [n, mask_words])
+        from an earlier call); returns (results, boards [n, P] uint16, stats),
+        and with save_stacks the stack of every search as it ended, as a
+        fourth element."""
         colours = np.ascontiguousarray(colours, dtype=np.uint32)
         pz = Puzzle(width, height, colours.ctypes.data)
-        if prefixes is None or len(prefixes) == 0 or np.asarray(prefixes).size == 0:
+        P = width * height
+        if stacks is not None:
+            in_state = np.ascontiguousarray(stacks[0], dtype=np.uint32)
+            in_masks = np.ascontiguousarray(stacks[1], dtype=np.uint64)
+            n = in_state.shape[0]
+            pre = np.zeros((n, 0), dtype=np.uint32)
+        elif prefixes is None or len(prefixes) == 0 or np.asarray(prefixes).size == 0:
             n = n_searches or 1
             pre = np.zeros((n, 0), dtype=np.uint32)
         else:
             pre = np.ascontiguousarray(np.asarray(prefixes, dtype=np.uint32))
             n = pre.shape[0]
+        out_state = out_masks = None
+        if save_stacks:
+            sw, mw = glutton_stack_words(P)
+            out_state = np.zeros((n, sw), dtype=np.uint32)
+            out_masks = np.zeros((n, mw), dtype=np.uint64)
         plen = pre.shape[1] if pre.ndim == 2 else 0
         res = (GluttonResult * n)()
         boards = np.zeros((n, width * height), dtype=np.uint16)
         opt = GluttonOptions(push_budget, waves_per_cu, 0)
         st = GluttonStats()
-        _check(lib().emp_hip_glutton_max(self._ctx, ctypes.byref(pz), _ptr(pre) if plen else None, plen, n,
-                                         ctypes.byref(opt), ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
+        _check(lib().emp_hip_glutton_max_stacks(
+            self._ctx, ctypes.byref(pz), _ptr(pre) if plen else None, plen, n, ctypes.byref(opt),
+            _ptr(in_state) if stacks is not None else None, _ptr(in_masks) if stacks is not None else None,
+            _ptr(out_state) if save_stacks else None, _ptr(out_masks) if save_stacks else None,
+            ctypes.byref(res), _ptr(boards), ctypes.byref(st)))
         results = [{"max_depth": r.max_depth, "status": r.status, "pushes": r.pushes} for r in res]
-        return results, boards, {f: getattr(st, f) for f, _ in GluttonStats._fields_}
+        stats = {f: getattr(st, f) for f, _ in GluttonStats._fields_}
+        if save_stacks:
+            return results, boards, stats, (out_state, out_masks)
+        return results, boards, stats
 
     def glutton_wide(self, width, height, colours, parents=None, cells=None, capacity=1 << 16):
         """glutton_wide's breadth step: each parent (rows of packed moves from

@@ -17,6 +17,7 @@
 #include "emp_enum_plan.h"
 #include "emp_enumerate.h"
 #include "emp_glutton.h"
+#include "emp_glutton_stack.h"
 #include "emp_glutton_wide.h"
 #include "emp_mlp.h"
 #include "emp_model.h"
@@ -620,10 +621,65 @@ int emp_hip_expand(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint3
 int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *prefixes,
                         uint32_t prefix_len, uint32_t n_searches, const emp_hip_glutton_options *options,
                         emp_hip_glutton_result *results, uint16_t *boards, emp_hip_glutton_stats *stats) {
+    return emp_hip_glutton_max_stacks(context, puzzle, prefixes, prefix_len, n_searches, options, nullptr, nullptr,
+                                      nullptr, nullptr, results, boards, stats);
+}
+
+int emp_hip_glutton_stack_words(uint32_t n_pieces, uint32_t *state_words, uint64_t *mask_words) {
+    return guarded([&] {
+        if (!state_words || !mask_words) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (!n_pieces || n_pieces > 256) return fail(EMP_HIP_UNSUPPORTED, "glutton_max handles 1 to 256 pieces");
+        *state_words = emp::glutton_state_words(n_pieces);
+        *mask_words = emp::glutton_mask_words(n_pieces);
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_glutton_stack_to_xml(const emp_hip_puzzle *puzzle, const uint32_t *state, const uint64_t *masks,
+                                 char *out, uint64_t capacity, uint64_t *size) {
+    return guarded([&] {
+        if (!puzzle || !state || !masks || !size) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        if (pz.size() > 256) return fail(EMP_HIP_UNSUPPORTED, "glutton_max handles at most 256 pieces");
+        std::string xml;
+        try {
+            xml = emp::glutton_stack_to_xml(pz, state, masks);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        *size = xml.size();
+        if (out) std::memcpy(out, xml.data(), std::min<uint64_t>(capacity, xml.size()));
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_glutton_stack_from_xml(const emp_hip_puzzle *puzzle, const char *xml, uint64_t size, uint32_t *state,
+                                   uint64_t *masks) {
+    return guarded([&] {
+        if (!puzzle || !xml || !state || !masks) return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
+        if (pz.size() > 256) return fail(EMP_HIP_UNSUPPORTED, "glutton_max handles at most 256 pieces");
+        try {
+            emp::glutton_stack_from_xml(pz, std::string(xml, xml + size), state, masks);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        return EMP_HIP_OK;
+    });
+}
+
+int emp_hip_glutton_max_stacks(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *prefixes,
+                               uint32_t prefix_len, uint32_t n_searches, const emp_hip_glutton_options *options,
+                               const uint32_t *in_state, const uint64_t *in_masks, uint32_t *out_state,
+                               uint64_t *out_masks, emp_hip_glutton_result *results, uint16_t *boards,
+                               emp_hip_glutton_stats *stats) {
     return guarded([&] {
         if (!context || !puzzle || !results || !boards || !n_searches)
             return fail(EMP_HIP_BAD_ARGUMENT, "null argument or no search");
         if (prefix_len && !prefixes) return fail(EMP_HIP_BAD_ARGUMENT, "null prefixes");
+        if ((in_state == nullptr) != (in_masks == nullptr) || (out_state == nullptr) != (out_masks == nullptr))
+            return fail(EMP_HIP_BAD_ARGUMENT, "a stack needs both its state and its masks");
+        if (in_state && prefix_len) return fail(EMP_HIP_BAD_ARGUMENT, "resumed searches play no prefix");
         if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
         emp::Puzzle pz(puzzle->width, puzzle->height, puzzle->colours);
         if (pz.size() > 256) return fail(EMP_HIP_UNSUPPORTED, "glutton_max handles at most 256 pieces");
@@ -636,8 +692,24 @@ int emp_hip_glutton_max(emp_hip_context *context, const emp_hip_puzzle *puzzle, 
         std::vector<uint32_t> pre(prefixes, prefixes + (size_t)prefix_len * n_searches);
         std::vector<emp::GluttonOutcome> out;
         emp::GluttonStats st;
+        emp::GluttonCheckpoints ck;
+        ck.in_state = in_state;
+        ck.in_masks = in_masks;
+        ck.out_state = out_state;
+        ck.out_masks = out_masks;
+        if (in_state) {
+            const uint32_t sw = emp::glutton_state_words(pz.size());
+            const uint64_t mw = emp::glutton_mask_words(pz.size());
+            for (uint32_t i = 0; i < n_searches; ++i) {
+                try {
+                    emp::validate_glutton_stack(pz, in_state + (size_t)i * sw, in_masks + (size_t)i * mw);
+                } catch (const std::invalid_argument &e) {
+                    return fail(EMP_HIP_BAD_ARGUMENT, "stack " + std::to_string(i) + ": " + e.what());
+                }
+            }
+        }
         emp::glutton_on_device(context->dev, context->stream, context->workspace, pz, pre, prefix_len, n_searches,
-                               cfg, out, st);
+                               cfg, out, st, ck);
         for (uint32_t i = 0; i < n_searches; ++i) {
             results[i].max_depth = out[i].max_depth;
             results[i].status = out[i].status;

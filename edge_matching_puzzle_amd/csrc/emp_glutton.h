@@ -41,12 +41,37 @@ struct GluttonStats {
     uint32_t waves = 0, lds_bytes = 0;
 };
 
+// A search's stack, as the reference's CUDA_glutton_stack_XML_converter
+// saves it (include/CUDA_glutton_stack_XML_converter.h:188-202), plus the
+// level the search never pops below.  State words: [0] size S (= P), [1]
+// pieces P, [2] level, [3] max, [4] base, then info_to_position [S],
+// position_to_info [P], played_info [S], max_played_info [S] (each move
+// pos | piece << 8 | o << 16) and the available pieces [8] (bit p of word
+// p / 32).  Masks: S(S+1)/2 slot masks of nw u64 (bit 4 * piece + o), level l
+// at offset (S(S+1) - (S-l)(S-l+1)) / 2 * nw holding S - l slots; levels
+// above `level` are zero.
+uint32_t glutton_state_words(uint32_t P);
+uint64_t glutton_mask_words(uint32_t P);
+// Throws std::invalid_argument unless the stack is one a search of this
+// puzzle can be in (sizes, indices in range, free positions and slots in
+// correspondence, placed moves on placed positions, no bit past 4P).
+void validate_glutton_stack(const Puzzle &puzzle, const uint32_t *state, const uint64_t *masks);
+
+// Host buffers of stack checkpoints, n_items stacks each (may be null).
+struct GluttonCheckpoints {
+    const uint32_t *in_state = nullptr;   // resume from these stacks instead of playing prefixes
+    const uint64_t *in_masks = nullptr;
+    uint32_t *out_state = nullptr;        // each search's stack when it ends
+    uint64_t *out_masks = nullptr;
+};
+
 // Runs one glutton_max search per prefix (prefix_len moves each, packed
 // pos | piece << 8 | o << 16, applied before the search; the search never
-// pops below them).
+// pops below them), or per saved stack (ck.in_*, prefix_len 0).
 void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &puzzle,
                        const std::vector<uint32_t> &prefixes, uint32_t prefix_len, uint32_t n_items,
-                       const GluttonConfig &cfg, std::vector<GluttonOutcome> &out, GluttonStats &stats);
+                       const GluttonConfig &cfg, std::vector<GluttonOutcome> &out, GluttonStats &stats,
+                       const GluttonCheckpoints &ck = GluttonCheckpoints{});
 
 }  // namespace emp
 

@@ -64,6 +64,15 @@ struct GlArgs {
     uint16_t *out_board;         // [n_items][P]
     unsigned long long *evals;   // [0] candidates evaluated, [1] of them scored, [2] removal rounds,
                                  // [3] free slots summed over the evaluations
+    // Stack checkpoints (layout: glutton_state_words / glutton_mask_words):
+    // with ck_in_* each search resumes from its saved stack instead of
+    // playing a prefix; with ck_out_* each search saves its stack at the end.
+    const uint32_t *ck_in_state;
+    const uint64_t *ck_in_masks;
+    uint32_t *ck_out_state;
+    uint64_t *ck_out_masks;
+    uint32_t ck_state_words;
+    uint64_t ck_mask_words;
 };
 
 // Per-wave LDS layout (offsets in bytes, all 8-byte aligned).
@@ -1348,6 +1357,58 @@ struct Wave {
         __syncthreads();
     }
 
+    // The stack as CUDA_glutton_stack_XML_converter::dump saves it
+    // (include/CUDA_glutton_stack_XML_converter.h:188-202), plus the level the
+    // search never pops below: state words [0] size S, [1] pieces P, [2]
+    // level, [3] max, [4] base, then info_to_position (pos_of_slot, S),
+    // position_to_info (slot_of_pos, P), played (S), max_played (S) and the
+    // available pieces (8 words of 32 bits); masks: every level's slot masks
+    // at level_offset(l), this level's the live ones, deeper levels zero.
+    __device__ void save(uint32_t *st, uint64_t *mk) {
+        const uint32_t S = a.S, P = a.P;
+        if (lane == 0) {
+            st[0] = S;
+            st[1] = P;
+            st[2] = level;
+            st[3] = max;
+            st[4] = base;
+        }
+        // moves past the level (and past max) are not part of the stack
+        for (uint32_t j = lane; j < S; j += kWave) {
+            st[5 + j] = pos_of_slot[j];
+            st[5 + S + P + j] = j < level ? played[j] : 0xFFFFFFFFu;
+            st[5 + 2 * S + P + j] = j < max ? max_played[j] : 0xFFFFFFFFu;
+        }
+        for (uint32_t q = lane; q < P; q += kWave) st[5 + S + q] = slot_of_pos[q];
+        if (lane < 8) st[5 + 3 * S + P + lane] = (uint32_t)(avail[lane >> 1] >> (32 * (lane & 1)));
+        const size_t here = level_offset(level), live = (size_t)n() * a.nw, total = level_offset(S);
+        for (size_t k = lane; k < here; k += kWave) mk[k] = stack[k];
+        for (size_t k = lane; k < live; k += kWave) mk[here + k] = M[k];
+        for (size_t k = here + live + lane; k < total; k += kWave) mk[k] = 0;
+        __syncthreads();
+    }
+
+    // Reverse of save() (after init()): the stack is as it was when saved.
+    // The masks are not known to be a fixpoint.
+    __device__ void restore(const uint32_t *st, const uint64_t *mk) {
+        const uint32_t S = a.S, P = a.P;
+        level = st[2];
+        max = st[3];
+        base = st[4];
+        for (uint32_t j = lane; j < S; j += kWave) {
+            pos_of_slot[j] = (uint16_t)st[5 + j];
+            played[j] = st[5 + S + P + j];
+            max_played[j] = st[5 + 2 * S + P + j];
+        }
+        for (uint32_t q = lane; q < P; q += kWave) slot_of_pos[q] = (uint16_t)st[5 + S + q];
+        if (lane < 4) avail[lane] = (uint64_t)st[5 + 3 * S + P + 2 * lane] | (uint64_t)st[5 + 3 * S + P + 2 * lane + 1] << 32;
+        const size_t here = level_offset(level), live = (size_t)n() * a.nw;
+        for (size_t k = lane; k < here; k += kWave) stack[k] = mk[k];
+        for (size_t k = lane; k < live; k += kWave) M[k] = mk[here + k];
+        push_known = false;
+        __syncthreads();
+    }
+
     // CUDA_glutton_max_stack::pop (:497-522), then reload the level's masks.
     __device__ uint32_t pop() {
         uint32_t slot = 0;
@@ -1471,7 +1532,10 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(Gl
         wv.init();
         uint32_t status = kExhausted;
         bool ok = true, dead = false;
-        for (uint32_t m = 0; m < a.prefix_len && ok && !dead; ++m) {
+        const bool resumed = a.ck_in_state != nullptr;
+        if (resumed)
+            wv.restore(a.ck_in_state + (size_t)item * a.ck_state_words, a.ck_in_masks + (size_t)item * a.ck_mask_words);
+        for (uint32_t m = 0; !resumed && m < a.prefix_len && ok && !dead; ++m) {
             const uint32_t mv = a.prefixes[(size_t)item * a.prefix_len + m];
             const uint32_t pos = mv & 0xFF, piece = (mv >> 8) & 0xFF, o = (mv >> 16) & 3;
             ok = pos < a.P && piece < a.P && wv.is_free(pos) && wv.piece_avail(piece);
@@ -1490,7 +1554,7 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(Gl
         if (!ok) status = kBadPrefix;
         else if (dead) status = kExhausted;
         else {
-            wv.base = wv.level;
+            if (!resumed) wv.base = wv.level;
             bool new_level = true;
             uint32_t pop_slot = 0;
             status = kSolved;
@@ -1530,6 +1594,8 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(Gl
             a.out_status[item] = status;
         }
         __syncthreads();
+        if (a.ck_out_state)
+            wv.save(a.ck_out_state + (size_t)item * a.ck_state_words, a.ck_out_masks + (size_t)item * a.ck_mask_words);
     }
 }
 
@@ -1597,12 +1663,67 @@ GluttonTables build_glutton_tables(const Puzzle &pz) {
     return t;
 }
 
+uint32_t glutton_state_words(uint32_t P) { return 13 + 4 * P; }
+
+uint64_t glutton_mask_words(uint32_t P) { return (uint64_t)P * (P + 1) / 2 * ((4 * P + 63) / 64); }
+
+void validate_glutton_stack(const Puzzle &pz, const uint32_t *st, const uint64_t *mk) {
+    const uint32_t P = pz.size(), S = P, nw = (4 * P + 63) / 64;
+    auto bad = [](const std::string &what) { throw std::invalid_argument("glutton stack: " + what); };
+    if (st[0] != S) bad("size " + std::to_string(st[0]) + " is not the puzzle's " + std::to_string(S));
+    if (st[1] != P) bad("nb_pieces " + std::to_string(st[1]) + " is not the puzzle's " + std::to_string(P));
+    const uint32_t level = st[2], max = st[3], base = st[4];
+    if (!(base <= level && level <= max && max <= S)) bad("base <= level <= max <= size does not hold");
+    const uint32_t *i2p = st + 5, *p2i = st + 5 + S, *played = st + 5 + S + P, *maxp = st + 5 + 2 * S + P;
+    const uint32_t *av = st + 5 + 3 * S + P;
+    const uint32_t n = S - level;
+    for (uint32_t j = 0; j < S; ++j)
+        if (i2p[j] >= P) bad("info_to_position[" + std::to_string(j) + "] out of range");
+    for (uint32_t q = 0; q < P; ++q)
+        if (p2i[q] >= S) bad("position_to_info[" + std::to_string(q) + "] out of range");
+    for (uint32_t j = 0; j < n; ++j)
+        if (p2i[i2p[j]] != j) bad("free info " + std::to_string(j) + " and its position disagree");
+    uint32_t free_positions = 0, n_avail = 0;
+    for (uint32_t q = 0; q < P; ++q) free_positions += p2i[q] < n;
+    if (free_positions != n) bad("free positions and level disagree");
+    for (uint32_t w = 0; w < 8; ++w)
+        for (uint32_t b = 0; b < 32; ++b)
+            if ((av[w] >> b) & 1) {
+                if (32 * w + b >= P) bad("available piece past the last piece");
+                ++n_avail;
+            }
+    if (n_avail != n) bad("available pieces and level disagree");
+    auto check_move = [&](uint32_t mv, const std::string &what, bool placed) {
+        const uint32_t pos = mv & 0xFF, piece = (mv >> 8) & 0xFF;
+        if ((mv >> 18) || pos >= P || piece >= P) bad(what + " is not a move");
+        if (placed && (p2i[pos] < n || ((av[piece / 32] >> (piece % 32)) & 1)))
+            bad(what + " is not on the board");
+    };
+    for (uint32_t l = 0; l < level; ++l) check_move(played[l], "played_info[" + std::to_string(l) + "]", true);
+    for (uint32_t l = 0; l < max; ++l) check_move(maxp[l], "max_played_info[" + std::to_string(l) + "]", false);
+    const uint64_t last = (4 * P) % 64 ? ~0ull << ((4 * P) % 64) : 0ull;
+    for (uint32_t l = 0; l <= level && l < S; ++l) {
+        const uint64_t off = ((uint64_t)S * (S + 1) - (uint64_t)(S - l) * (S - l + 1)) / 2 * nw;
+        for (uint32_t j = 0; j < S - l; ++j)
+            if (mk[off + (uint64_t)j * nw + nw - 1] & last) bad("mask bit past the last piece");
+    }
+}
+
 void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws, const Puzzle &pz,
                        const std::vector<uint32_t> &prefixes, uint32_t prefix_len, uint32_t n_items,
-                       const GluttonConfig &cfg, std::vector<GluttonOutcome> &out, GluttonStats &stats) {
+                       const GluttonConfig &cfg, std::vector<GluttonOutcome> &out, GluttonStats &stats,
+                       const GluttonCheckpoints &ck) {
     if (pz.size() > 256) throw Unsupported("glutton_max handles at most 256 pieces");
     if ((uint64_t)prefixes.size() != (uint64_t)prefix_len * n_items)
         throw std::invalid_argument("prefix array size does not match n_items * prefix_len");
+    if ((ck.in_state == nullptr) != (ck.in_masks == nullptr) || (ck.out_state == nullptr) != (ck.out_masks == nullptr))
+        throw std::invalid_argument("a checkpoint needs both its state and its masks");
+    if (ck.in_state && prefix_len) throw std::invalid_argument("resumed searches play no prefix");
+    const uint32_t ck_state = glutton_state_words(pz.size());
+    const uint64_t ck_masks = glutton_mask_words(pz.size());
+    if (ck.in_state)
+        for (uint32_t i = 0; i < n_items; ++i)
+            validate_glutton_stack(pz, ck.in_state + (size_t)i * ck_state, ck.in_masks + (size_t)i * ck_masks);
     auto wall0 = std::chrono::steady_clock::now();
     GluttonTables t = build_glutton_tables(pz);
     const uint32_t S = t.P;
@@ -1626,6 +1747,13 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     uint32_t *d_status = ws.get<uint32_t>("gl_status", n_items);
     uint16_t *d_board = ws.get<uint16_t>("gl_board", (size_t)n_items * t.P);
     unsigned long long *d_evals = ws.get<unsigned long long>("gl_evals", 4);
+    const bool any_ck = ck.in_state || ck.out_state;
+    uint32_t *d_ck_state = any_ck ? ws.get<uint32_t>("gl_ck_state", (size_t)n_items * ck_state) : nullptr;
+    uint64_t *d_ck_masks = any_ck ? ws.get<uint64_t>("gl_ck_masks", (size_t)n_items * ck_masks) : nullptr;
+    if (ck.in_state) {
+        GL_CHECK(hipMemcpyAsync(d_ck_state, ck.in_state, (size_t)n_items * ck_state * 4, hipMemcpyHostToDevice, stream));
+        GL_CHECK(hipMemcpyAsync(d_ck_masks, ck.in_masks, (size_t)n_items * ck_masks * 8, hipMemcpyHostToDevice, stream));
+    }
     GL_CHECK(hipMemcpyAsync(d_cm, t.cm.data(), t.cm.size() * 8, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_pcol, t.piece_col.data(), t.piece_col.size() * 4, hipMemcpyHostToDevice, stream));
     GL_CHECK(hipMemcpyAsync(d_init, t.init_masks.data(), t.init_masks.size() * 8, hipMemcpyHostToDevice, stream));
@@ -1666,6 +1794,12 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     a.out_status = d_status;
     a.out_board = d_board;
     a.evals = d_evals;
+    a.ck_in_state = ck.in_state ? d_ck_state : nullptr;
+    a.ck_in_masks = ck.in_state ? d_ck_masks : nullptr;
+    a.ck_out_state = ck.out_state ? d_ck_state : nullptr;
+    a.ck_out_masks = ck.out_state ? d_ck_masks : nullptr;
+    a.ck_state_words = ck_state;
+    a.ck_mask_words = ck_masks;
 
     hipEvent_t ev0, ev1;
     GL_CHECK(hipEventCreate(&ev0));
@@ -1689,6 +1823,10 @@ void glutton_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &ws,
     GL_CHECK(hipMemcpyAsync(pushes.data(), d_pushes, n_items * 8, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(boards.data(), d_board, boards.size() * 2, hipMemcpyDeviceToHost, stream));
     GL_CHECK(hipMemcpyAsync(evals, d_evals, sizeof(evals), hipMemcpyDeviceToHost, stream));
+    if (ck.out_state) {
+        GL_CHECK(hipMemcpyAsync(ck.out_state, d_ck_state, (size_t)n_items * ck_state * 4, hipMemcpyDeviceToHost, stream));
+        GL_CHECK(hipMemcpyAsync(ck.out_masks, d_ck_masks, (size_t)n_items * ck_masks * 8, hipMemcpyDeviceToHost, stream));
+    }
     GL_CHECK(hipStreamSynchronize(stream));
     out.assign(n_items, GluttonOutcome{});
     stats = GluttonStats{};

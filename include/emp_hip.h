@@ -414,6 +414,47 @@ int emp_hip_glutton_wide(emp_hip_context *context, const emp_hip_puzzle *puzzle,
                          uint32_t *parent_status, uint64_t capacity, uint32_t *child_parent, uint32_t *child_move,
                          uint64_t *child_masks, uint32_t *child_status, uint64_t *n_children);
 
+/* ------------------------------------------------------------------------ *
+ * glutton stack checkpoints.  The reference dumps and reloads the whole
+ * stack of a glutton run (CUDA_glutton_stack_XML_converter,
+ * include/CUDA_glutton_stack_XML_converter.h:165-516, written every 1024
+ * steps by include/CUDA_glutton_max.h:1556-1564); here a long search is cut
+ * at its push budget and resumed from its saved stack.  A stack is state
+ * words (u32) and masks (u64) in the layout documented in
+ * edge_matching_puzzle_amd/csrc/emp_glutton.h (glutton_state_words).
+ * ------------------------------------------------------------------------ */
+
+/* Words of one saved stack for n_pieces pieces. */
+int emp_hip_glutton_stack_words(uint32_t n_pieces, uint32_t *state_words, uint64_t *mask_words);
+
+/* emp_hip_glutton_max with stacks.  in_state / in_masks (n_searches stacks;
+ * prefixes null, prefix_len 0): each search resumes from its stack instead
+ * of playing a prefix.  out_state / out_masks: each search's stack is saved
+ * when it ends (push budget, solved or exhausted).  A search resumed from
+ * the stack its push budget cut saved continues exactly as the uncut search:
+ * the same moves, max depth, status and board, pushes counted from the
+ * resume.  EMP_HIP_BAD_ARGUMENT for a stack no search of this puzzle can be
+ * in.  Replaces: the reload of CUDA_glutton_stack_XML_converter::load
+ * (:158-186) into a stack the kernel continues. */
+int emp_hip_glutton_max_stacks(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *prefixes,
+                               uint32_t prefix_len, uint32_t n_searches, const emp_hip_glutton_options *options,
+                               const uint32_t *in_state, const uint64_t *in_masks, uint32_t *out_state,
+                               uint64_t *out_masks, emp_hip_glutton_result *results, uint16_t *boards,
+                               emp_hip_glutton_stats *stats);
+
+/* XML form of one stack: the reference converter's element layout
+ * (CUDA_glutton_stack root with size / level / nb_pieces / max, the five
+ * arrays, position_infos with one level per stack level of 32-word
+ * position_info elements) plus a base attribute.  to_xml writes at most
+ * capacity bytes to out (null: size only) and the full size to *size;
+ * from_xml reads size bytes, EMP_HIP_BAD_ARGUMENT with the loader's message
+ * when an element is missing or the stack is inconsistent.  No device work.
+ * Replace: CUDA_glutton_stack_XML_converter::dump (:188-202) / load (:158-186). */
+int emp_hip_glutton_stack_to_xml(const emp_hip_puzzle *puzzle, const uint32_t *state, const uint64_t *masks,
+                                 char *out, uint64_t capacity, uint64_t *size);
+int emp_hip_glutton_stack_from_xml(const emp_hip_puzzle *puzzle, const char *xml, uint64_t size, uint32_t *state,
+                                   uint64_t *masks);
+
 /* Seeded prefixes for glutton searches: each row is the n_fixed given moves
  * followed by one placement of each of the first `length` positions of the
  * spiral order, drawn by random dives that only place matching pieces
