@@ -128,15 +128,39 @@ __global__ void out_layer_kernel(const uint16_t *H2, const float *params, const 
 }
 
 // Gradients of w3 and b3: gw3[n] = sum_r H2[r][n] ds[r], gb3 = sum_r ds[r].
-__global__ void out_grad_kernel(const uint16_t *H2, const float *ds, uint32_t rows, float *grads) {
+// Column reductions over the batch rows in two stages: block b sums rows
+// [kSumRows * b, kSumRows * (b + 1)) of every column (thread n: column n)
+// into partial[b][n], then one block adds the partials in block order --
+// the same order on every run.
+constexpr uint32_t kSumRows = 32;
+
+// partial[b][n] = sum of D[r][n] (times ds[r] when ds is given); with ds,
+// partial[b][kMlpHidden] = sum of ds[r].
+__global__ void colsum_partial_kernel(const uint16_t *D, const float *ds, uint32_t rows, float *partial) {
+    const uint32_t n = threadIdx.x, r0 = blockIdx.x * kSumRows;
+    const uint32_t r1 = r0 + kSumRows < rows ? r0 + kSumRows : rows;
+    float g = 0.f, gb = 0.f;
+    for (uint32_t r = r0; r < r1; ++r) {
+        const float w = ds ? ds[r] : 1.f;
+        g += from_bf16(D[(size_t)r * kMlpHidden + n]) * w;
+        gb += w;
+    }
+    partial[(size_t)blockIdx.x * (kMlpHidden + 1) + n] = g;
+    if (n == 0) partial[(size_t)blockIdx.x * (kMlpHidden + 1) + kMlpHidden] = gb;
+}
+
+// grads[off + n] = sum over the blocks of partial[b][n]; with bias_off set,
+// grads[bias_off] = sum of partial[b][kMlpHidden].
+__global__ void colsum_final_kernel(const float *partial, uint32_t blocks, float *grads, uint32_t off,
+                                    int32_t bias_off) {
     const uint32_t n = threadIdx.x;
     float g = 0.f, gb = 0.f;
-    for (uint32_t r = 0; r < rows; ++r) {
-        g += from_bf16(H2[(size_t)r * kMlpHidden + n]) * ds[r];
-        gb += ds[r];
+    for (uint32_t b = 0; b < blocks; ++b) {
+        g += partial[(size_t)b * (kMlpHidden + 1) + n];
+        gb += partial[(size_t)b * (kMlpHidden + 1) + kMlpHidden];
     }
-    grads[kMlpOffW3 + n] = g;
-    if (n == 0) grads[kMlpOffB3] = gb;
+    grads[off + n] = g;
+    if (bias_off >= 0 && n == 0) grads[bias_off] = gb;
 }
 
 // dZ2 = (ds w3^T) * [H2 > 0] in BF16.
@@ -147,13 +171,6 @@ __global__ void dz2_kernel(const uint16_t *H2, const float *ds, const float *par
     dZ2[at] = from_bf16(H2[at]) > 0.f ? to_bf16(ds[r] * params[kMlpOffW3 + n]) : (uint16_t)0;
 }
 
-// Column sums of a BF16 [rows][kMlpHidden] matrix into grads[off + n].
-__global__ void colsum_kernel(const uint16_t *D, uint32_t rows, float *grads, uint32_t off) {
-    const uint32_t n = threadIdx.x;
-    float g = 0.f;
-    for (uint32_t r = 0; r < rows; ++r) g += from_bf16(D[(size_t)r * kMlpHidden + n]);
-    grads[off + n] = g;
-}
 
 __global__ void adam_kernel(float *p, const float *g, float *m, float *v, uint32_t n, float lr_t, float b1, float b2,
                             float eps) {
@@ -178,7 +195,7 @@ struct Net {
     uint32_t cap;   // rows per batch buffer
     float *params, *grads, *m, *v;
     uint16_t *W1b, *W2b, *Xb, *H1, *H2, *dZ1, *dZ2;
-    float *yb, *wb, *score, *ds, *loss;
+    float *yb, *wb, *score, *ds, *loss, *partial;
     uint32_t *idx;
 
     Net(Workspace &ws, uint32_t rows) : cap(rows) {
@@ -199,6 +216,15 @@ struct Net {
         ds = ws.get<float>("mlp_ds", rows);
         loss = ws.get<float>("mlp_loss", 1);
         idx = ws.get<uint32_t>("mlp_idx", rows);
+        partial = ws.get<float>("mlp_partial", (size_t)((rows + kSumRows - 1) / kSumRows) * (kMlpHidden + 1));
+    }
+
+    // grads[off + n] = column n of D summed over `rows` (weighted by ds, and
+    // grads[bias_off] = the sum of ds, when ds is given)
+    void colsum(hipStream_t s, const uint16_t *D, const float *w, uint32_t rows, uint32_t off, int32_t bias_off) {
+        const uint32_t blocks = (rows + kSumRows - 1) / kSumRows;
+        hipLaunchKernelGGL(colsum_partial_kernel, dim3(blocks), dim3(kMlpHidden), 0, s, D, w, rows, partial);
+        hipLaunchKernelGGL(colsum_final_kernel, dim3(1), dim3(kMlpHidden), 0, s, partial, blocks, grads, off, bias_off);
     }
 
     void refresh(hipStream_t s) {
@@ -220,9 +246,9 @@ struct Net {
     void backward(hipStream_t s, uint32_t rows, uint32_t padded, float wsum) {
         hipLaunchKernelGGL(out_layer_kernel, dim3((padded + 255) / 256), dim3(256), 0, s, H2, params, yb, wb, padded,
                            wsum, score, ds, loss);
-        hipLaunchKernelGGL(out_grad_kernel, dim3(1), dim3(kMlpHidden), 0, s, H2, ds, padded, grads);
+        colsum(s, H2, ds, padded, kMlpOffW3, (int32_t)kMlpOffB3);
         hipLaunchKernelGGL(dz2_kernel, dim3(padded), dim3(kMlpHidden), 0, s, H2, ds, params, padded, dZ2);
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kMlpHidden), 0, s, dZ2, padded, grads, kMlpOffB2);
+        colsum(s, dZ2, nullptr, padded, kMlpOffB2, -1);
         GemmArgs g{};
         // dW2 = H1^T dZ2
         g.A = H1; g.B = dZ2; g.M = kMlpHidden; g.N = kMlpHidden; g.K = padded; g.lda = kMlpHidden; g.ldb = kMlpHidden;
@@ -233,7 +259,7 @@ struct Net {
         g.A = dZ2; g.B = W2b; g.M = padded; g.N = kMlpHidden; g.K = kMlpHidden; g.lda = kMlpHidden; g.ldb = kMlpHidden;
         g.out_bf16 = dZ1; g.ldc = kMlpHidden; g.mask = H1;
         gemm<false, true, kMaskBf16>(s, g);
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kMlpHidden), 0, s, dZ1, padded, grads, kMlpOffB1);
+        colsum(s, dZ1, nullptr, padded, kMlpOffB1, -1);
         // dW1 = X^T dZ1
         g = GemmArgs{};
         g.A = Xb; g.B = dZ1; g.M = kMlpInPad; g.N = kMlpHidden; g.K = padded; g.lda = kMlpInPad; g.ldb = kMlpHidden;
