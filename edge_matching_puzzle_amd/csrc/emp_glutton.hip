@@ -1508,6 +1508,14 @@ struct Wave {
         return !emptied;
     }
 
+    // A choice of best() that can be played: its slot free, its bit one of
+    // the slot's options.  Anything else is an internal error, stopped here
+    // before apply() indexes with it.
+    __device__ bool playable(uint32_t b) const {
+        const uint32_t slot = b >> 16, bit = b & 0xFFFF;
+        return b != 0xFFFFFFFFu && slot < n() && bit < 4 * a.P && ((M[slot * a.nw + (bit >> 6)] >> (bit & 63)) & 1);
+    }
+
     __device__ bool slot_empty(uint32_t slot) {
         uint64_t any = 0;
         for (uint32_t w = lane; w < a.nw; w += kWave) any |= M[slot * a.nw + w];
@@ -1571,8 +1579,8 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(Gl
                     new_level = false;
                     continue;
                 }
-                const uint32_t b = wv.best();
-                if (b == 0xFFFFFFFFu || !wv.apply(b >> 16, b & 0xFFFF)) { status = kInternal; break; }
+                const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)wv.best());
+                if (!wv.playable(b) || !wv.apply(b >> 16, b & 0xFFFF)) { status = kInternal; break; }
                 ++pushes;
                 new_level = true;
                 if (a.push_budget && pushes >= a.push_budget && wv.level < a.S) { status = kLimit; break; }
@@ -1583,10 +1591,10 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) glutton_kernel(Gl
         uint16_t *board = a.out_board + (size_t)item * a.P;
         for (uint32_t q = wv.lane; q < a.P; q += kWave) board[q] = 0xFFFF;
         __syncthreads();
-        const uint32_t cnt = full ? a.S : wv.max;
+        const uint32_t cnt = full ? a.S : (wv.max < a.S ? wv.max : a.S);
         for (uint32_t k = wv.lane; k < cnt; k += kWave) {
             const uint32_t pl = full ? wv.played[k] : wv.max_played[k];
-            board[pl & 0xFF] = (uint16_t)((((pl >> 8) & 0xFF) << 2) | ((pl >> 16) & 3));
+            if ((pl & 0xFF) < a.P) board[pl & 0xFF] = (uint16_t)((((pl >> 8) & 0xFF) << 2) | ((pl >> 16) & 3));
         }
         if (wv.lane == 0) {
             a.out_max_depth[item] = wv.max;
