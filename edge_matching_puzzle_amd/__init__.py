@@ -37,7 +37,7 @@ class EnumOptions(ctypes.Structure):
                 ("node_limit", ctypes.c_uint64), ("max_solutions", ctypes.c_uint64),
                 ("use_graph", ctypes.c_uint32), ("queue_items", ctypes.c_uint32),
                 ("time_limit_ms", ctypes.c_double), ("count_work", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("schedule", ctypes.c_uint32)]
 
 
 class EnumResult(ctypes.Structure):
@@ -55,7 +55,7 @@ class EnumStats(ctypes.Structure):
                 ("lds_tables", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
                 ("threads", ctypes.c_uint32), ("candidate_sets", ctypes.c_uint64),
                 ("group_ms", ctypes.c_double), ("work_bytes", ctypes.c_uint64),
-                ("work_ops", ctypes.c_uint64)]
+                ("work_ops", ctypes.c_uint64), ("tickets", ctypes.c_uint64)]
 
 
 class GluttonOptions(ctypes.Structure):
@@ -78,6 +78,8 @@ class GluttonStats(ctypes.Structure):
 GLUTTON_SOLVED, GLUTTON_EXHAUSTED, GLUTTON_BUDGET, GLUTTON_BAD_PREFIX = range(4)
 
 NO_MOVE = 0xFFFFFFFF
+# emp_hip_enum_options.schedule
+SCHEDULES = {"auto": 0, "passes": 1, "persistent": 2}
 WIDE_VALID, WIDE_EMPTIED, WIDE_BAD_MOVE = range(3)
 
 
@@ -526,10 +528,11 @@ class Context:
         return boards[:min(n.value, capacity)].copy(), n.value, nodes.value, sols.value
 
     def _options(self, order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
-                 target_items, node_limit, queue_items):
+                 target_items, node_limit, queue_items, schedule="auto"):
         opt = EnumOptions()
         lib().emp_hip_enum_default_options(ctypes.byref(opt))
         opt.order = _ORDERS[order]
+        opt.schedule = SCHEDULES[schedule]
         opt.waves_per_cu, opt.waves_per_block = waves_per_cu, waves_per_block
         opt.steps_per_round, opt.step_budget = steps_per_round, step_budget
         opt.target_items, opt.node_limit, opt.max_solutions = target_items, node_limit, max_solutions
@@ -562,7 +565,8 @@ class Context:
             it_arr[:, :3] = items
         opt = self._options(order, max_solutions, launch.get("waves_per_cu", 0), launch.get("waves_per_block", 0),
                             launch.get("steps_per_round", 0), launch.get("step_budget", 0),
-                            launch.get("target_items", 0), launch.get("node_limit", 0), launch.get("queue_items", 0))
+                            launch.get("target_items", 0), launch.get("node_limit", 0), launch.get("queue_items", 0),
+                            launch.get("schedule", "auto"))
         opt.time_limit_ms = time_limit_ms
         res = EnumResult()
         boards = np.zeros((max(max_solutions, 1), P), dtype=np.uint16)
@@ -597,7 +601,8 @@ class Context:
         n, L = rows.shape
         opt = self._options(order, 0, launch.get("waves_per_cu", 0), launch.get("waves_per_block", 0),
                             launch.get("steps_per_round", 0), launch.get("step_budget", 0),
-                            launch.get("target_items", 0), node_limit, launch.get("queue_items", 0))
+                            launch.get("target_items", 0), node_limit, launch.get("queue_items", 0),
+                            launch.get("schedule", "auto"))
         opt.time_limit_ms = time_limit_ms
         res = (EnumResult * n)()
         st = EnumStats()
@@ -610,7 +615,7 @@ class Context:
     def enumerate(self, puzzles, order="spiral", max_solutions=1024, waves_per_cu=0,
                   waves_per_block=0, steps_per_round=0, step_budget=0, target_items=0,
                   node_limit=0, queue_items=0, ranks=None, time_limit_ms=0.0, use_graph=False,
-                  noalloc=False, count_work=False):
+                  noalloc=False, count_work=False, schedule="auto"):
         """Exhaustive enumeration of each (width, height, colours) puzzle.
 
         ranks (optional, one uint16 [P, T] array per puzzle) gives the visit
@@ -618,7 +623,10 @@ class Context:
         stats): one dict per puzzle with n_nodes, n_solutions, complete,
         solve_ms, max_depth and boards ([n_stored, P] uint16 row-major).
         noalloc: through emp_hip_enumerate_noalloc (device memory from the
-        library's fixed scratch buffer, no allocation)."""
+        library's fixed scratch buffer, no allocation).  schedule:
+        "persistent" (one launch, hand-overs to idle lanes), "passes"
+        (bounded launches, items split into the next one's queue) or "auto"
+        (the first for one puzzle, the second for batches)."""
         n = len(puzzles)
         keep = []
         arr = (Puzzle * n)()
@@ -632,7 +640,7 @@ class Context:
             boards.append(b)
             board_ptrs[i] = b.ctypes.data if max_solutions else None
         opt = self._options(order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
-                            target_items, node_limit, queue_items)
+                            target_items, node_limit, queue_items, schedule)
         opt.time_limit_ms = time_limit_ms
         opt.use_graph = 1 if use_graph else 0
         opt.count_work = 1 if count_work else 0

@@ -82,6 +82,9 @@ emp::EnumConfig config_from(const emp_hip_enum_options &opt) {
     cfg.time_limit_ms = opt.time_limit_ms;
     cfg.use_graph = opt.use_graph != 0;
     cfg.count_work = opt.count_work != 0;
+    cfg.schedule = opt.schedule == EMP_HIP_SCHEDULE_PASSES       ? emp::EnumConfig::kPasses
+                   : opt.schedule == EMP_HIP_SCHEDULE_PERSISTENT ? emp::EnumConfig::kPersistent
+                                                                 : emp::EnumConfig::kAuto;
     // Device-side store of up to max_solutions boards per search.
     cfg.solution_capacity = opt.max_solutions;
     return cfg;
@@ -108,6 +111,7 @@ void copy_stats(const emp::EnumStats &st, emp_hip_enum_stats *stats) {
     stats->expand_launches = st.expand_launches;
     stats->items_processed = st.items_processed;
     stats->splits = st.splits;
+    stats->tickets = st.tickets;
     stats->candidate_sets = st.candidate_sets;
     stats->group_ms = st.group_ms;
     stats->work_bytes = st.work_bytes;

@@ -77,8 +77,13 @@ struct EnumConfig {
     uint32_t max_expansions = 0;
     uint32_t queue_items = 0;       // capacity of each work queue, 0 = sized from the grid
     double time_limit_ms = 0;       // stop every search after this device time (0 = none)
-    bool use_graph = false;         // replay the DFS passes from two captured hipGraphs
+    bool use_graph = false;         // replay the DFS launch(es) from captured hipGraphs
     bool count_work = false;        // accumulate the algorithmic bytes / ops of every placement
+    // How the DFS work is shared out: one persistent launch with hand-overs,
+    // DFS passes (items split into the next pass's queue), or the first for
+    // one puzzle and the second for batches of several.
+    enum Schedule { kAuto, kPasses, kPersistent };
+    Schedule schedule = kAuto;
 };
 
 struct EnumOutcome {
@@ -123,6 +128,7 @@ struct EnumStats {
     double group_ms = 0;           // grouping items by puzzle before the passes
     uint64_t work_bytes = 0;       // count_work: algorithmic LDS bytes of the placements
     uint64_t work_ops = 0;         // count_work: algorithmic integer ops of the placements
+    uint64_t tickets = 0;          // persistent launch: items searched (starting items and hand-overs)
 };
 
 // A search that starts from a given situation of puzzle `table`: the

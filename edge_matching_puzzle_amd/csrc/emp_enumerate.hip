@@ -107,7 +107,50 @@ struct DfsArgs {
     // placement with M = ceil(4 P / 64) u64 words and n the placed neighbours
     // of the next strategy index (0 after the last one).
     unsigned long long *work;
+    // One persistent launch (persist != 0) instead of passes.  Work items
+    // live in a ring of 2^q_shift slots (in_hdr / in_stack, written through
+    // q_hdr / q_stack), ticket t in slot t mod 2^q_shift: tickets below the
+    // initial q_tail hold the starting items, later ones are filled by lanes
+    // that hand over the rest of their floor level -- after step_budget steps
+    // since their last hand-over, drain_budget while lanes wait for work
+    // (q_head > q_tail), not while q_backlog items wait for lanes.  An idle
+    // lane takes the next ticket (q_head).  Slot states (q_ready): 2g + 1 =
+    // holds the item of generation g (t >> q_shift), 2g + 2 = that item was
+    // read; a slot is filled for generation g only in state 2g (0 for g = 0):
+    // a lane whose slot is still in use keeps the item owed and writes it in
+    // a later round (its floor levels do not change meanwhile), so no lane
+    // waits.  q_pending counts the items reserved, queued or being searched;
+    // the launch ends when it reaches zero.
+    uint32_t persist;              // kernel choice: the persistent form
+    uint32_t limits;               // kernel choice: node limits checked while items run
+    uint32_t q_shift;
+    uint32_t q_backlog;
+    unsigned int *q_head;
+    unsigned int *q_tail;
+    unsigned int *q_pending;
+    unsigned int *q_ready;
+    uint64_t *q_hdr;
+    uint16_t *q_stack;
+    unsigned int *q_stalled;       // set when a wave waited kStallTicks with nothing to do
 };
+
+// Waits until this wave's vector memory operations issued so far are
+// complete (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima).
+// The work items of a persistent launch are written and read with
+// agent-scope relaxed atomics, which go to the coherence point past the XCD's
+// L2, so ordering them needs only this wait -- an agent-scope release or
+// acquire would write back or invalidate the whole L2 of the XCD, and a
+// workgroup-scope fence emits no wait for global memory.
+__device__ __forceinline__ void wave_settle() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// A wave of a persistent launch that has waited this long (device clock,
+// 100 MHz) for work with items still pending gives up and reports it: the
+// search ends with an error instead of waiting without bound.
+constexpr uint64_t kStallTicks = 2000000000ull;   // 20 s
+
+// A persistent launch's lanes read the queue counters (and may hand over)
+// every kQueueRounds rounds of steps_per_round steps.
+constexpr uint32_t kQueueRounds = 8;
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
 // transitions whose rank at this index is above the rank of `last`.
@@ -487,7 +530,10 @@ __global__ void expand_kernel(DfsArgs a) {
 // so their blocks stay at 4 waves.
 constexpr uint32_t dfs_max_threads(int w) { return w <= 4 ? 1024u : 256u; }
 
-template <int W, bool LDS_TABLE, bool RANKED>
+// LIMITS: the launch has node limits, checked while items run as well as
+// when they are taken (compiled only where needed: the check's live state
+// costs registers in the step loop).
+template <int W, bool LDS_TABLE, bool RANKED, bool PERSIST, bool LIMITS>
 __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     using STK = typename std::conditional<(W <= 4), uint8_t, uint16_t>::type;
     extern __shared__ uint64_t smem[];
@@ -514,6 +560,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 
     bool active = false;
     bool queue_empty = false;
+    bool waiting = false;                    // persistent launch: holds a ticket not started yet
+    uint32_t ticket = 0;
+    bool owed = false;                       // persistent launch: a reserved ticket whose slot was in use
+    uint32_t owed_ticket = 0, owed_floor = 0;
+    uint64_t give_mark = 0;                  // steps of the item at its last hand-over
+    uint32_t rounds = 0;                     // rounds of this lane (queue checks every kQueueRounds)
+    uint64_t idle_since = 0;                 // persistent launch: the wave's wait for work began
     bool fresh = false;       // c not computed yet for the item's level
     uint32_t puzzle = 0, k = 0, kmin = 0, P = 0, depth = 0;
     uint32_t item = 0;   // input index of the item (ordered results)
@@ -522,6 +575,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     const uint16_t *rank_base = nullptr;   // ranked: this puzzle's rank rows
     uint32_t rank_stride = 0;
     uint64_t steps = 0, nodes = 0;
+    // Under a node limit the lane adds its progress to the puzzle's count
+    // every `pub_step` nodes (0: the puzzle has no limit), so the many items
+    // of one puzzle that run side by side (grouped batches, a persistent
+    // launch) stop close to the limit instead of each running its whole
+    // budget past it.  pub_at: the low 32 bits of `nodes` when last added
+    // (0: nothing added yet; fewer than 2^32 nodes pass between two additions).
+    uint32_t pub_at = 0, pub_step = 0;
     uint64_t work_bytes = 0, work_ops = 0;   // instrumented runs only
     uint32_t words_alg = 0;                  // M = ceil(4 P / 64) of the item's puzzle
     uint64_t c[W];
@@ -539,33 +599,53 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             atomicAdd(&a.work[0], (unsigned long long)work_bytes);
             atomicAdd(&a.work[1], (unsigned long long)work_ops);
         }
-        atomicAdd(&a.nodes[puzzle], (unsigned long long)nodes);
+        // the nodes not added yet (under LIMITS fewer than 2^32 pass between
+        // two additions)
+        atomicAdd(&a.nodes[puzzle], LIMITS ? (unsigned long long)((uint32_t)nodes - pub_at) : (unsigned long long)nodes);
         atomicAdd(a.candidate_sets, (unsigned long long)steps);
         atomicMax(&a.max_depth[puzzle], depth);
         atomicMax(&a.last_tick[puzzle], (unsigned long long)wall_clock64());
+        // the item is done: after its hand-overs, so the count never drops
+        // to zero while work can still appear
+        if (PERSIST) atomicSub(a.q_pending, 1u);
         active = false;
     };
 
-    while (true) {
-        // Refill idle lanes with one queue atomic per wave.
-        const uint64_t idle = __ballot(!active);
-        if (idle && !queue_empty) {
-            const uint32_t n_idle = (uint32_t)__popcll(idle);
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(a.in_head, n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= n_in) {
-                queue_empty = true;
-                // Every item is taken: from now on busy lanes split early so
-                // the pass ends soon and the rest is shared out next pass.
-                if (lane == leader) atomicOr(&a.flags[3], 1u);
-            }
-            if (!active) {
-                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
-                const uint32_t idx = base + rank;
-                if (idx < n_in) {
-                    const uint64_t hdr = a.in_hdr[idx];
+    // Transition j of input item idx.  In a persistent launch the item may
+    // have been written by another CU during the launch, so it is read from
+    // L2 (rows start on 8-byte boundaries: stride is a multiple of 4).
+    auto item_move = [&](uint32_t idx, uint32_t j) -> uint32_t {
+        const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
+        if (!PERSIST) return src[j];
+        const uint32_t w = __hip_atomic_load(const_cast<uint32_t *>(reinterpret_cast<const uint32_t *>(src)) + (j >> 1),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return (w >> (16 * (j & 1))) & 0xFFFFu;
+    };
+    // Writes the rest of floor level f of this lane's search (the candidates
+    // at index f above S(f), with S(0..f-1) placed) as the item of ticket tk,
+    // if its slot is free for tk's generation; false when the slot is still
+    // in use.  The item goes through the coherence point (stores that skip
+    // the XCD's L2) and is settled before its state says so.
+    auto write_handover = [&](uint32_t tk, uint32_t f) -> bool {
+        const uint32_t slot = tk & ((1u << a.q_shift) - 1), g = tk >> a.q_shift;
+        if (__hip_atomic_load(&a.q_ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != (g ? 2 * g : 0u))
+            return false;
+        uint32_t *db = reinterpret_cast<uint32_t *>(a.q_stack + (size_t)slot * a.stride);
+        for (uint32_t q = 0; q < f; q += 2)
+            __hip_atomic_store(db + (q >> 1), (uint32_t)S(q) | (q + 1 < f ? (uint32_t)S(q + 1) << 16 : 0u),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.q_hdr + slot, make_hdr(puzzle, f, f, (int)S(f)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        wave_settle();
+        __hip_atomic_store(&a.q_ready[slot], 2 * g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    };
+    // Starts input item idx on this lane; false when it is not searched here
+    // (a dead slot, its puzzle over the node limit, past the deadline).
+    auto take_item = [&](uint32_t idx) -> bool {
+                    const uint64_t hdr = PERSIST ? __hip_atomic_load(const_cast<uint64_t *>(a.in_hdr) + idx,
+                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                                   : a.in_hdr[idx];
                     kmin = hdr_floor(hdr);
                     puzzle = hdr_puzzle(hdr);
                     bool take = kmin != kDeadItem;
@@ -586,7 +666,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                             if (o < a.out_cap) {
                                 const uint32_t lv = hdr_level(hdr);
                                 for (uint32_t j = 0; j < lv; ++j)
-                                    a.out_stack[(size_t)o * a.stride + j] = a.in_stack[(size_t)idx * a.stride + j];
+                                    a.out_stack[(size_t)o * a.stride + j] = (uint16_t)item_move(idx, j);
                                 a.out_hdr[o] = hdr & ~kPendingMask;
                             } else {
                                 atomicOr(&a.flags[0], 1u);
@@ -600,7 +680,13 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         last = hdr_last(hdr);
                         k = hdr_level(hdr);
                         steps = 0;
+                        give_mark = 0;
                         nodes = hdr_pending(hdr);
+                        {
+                            const unsigned long long lim = a.node_limit[puzzle];
+                            pub_step = lim ? (uint32_t)min(4096ull, max(64ull, lim >> 9)) : 0u;
+                            pub_at = 0;
+                        }
                         item = idx;
                         work_bytes = work_ops = 0;
                         words_alg = (4 * P + 63) / 64;
@@ -610,12 +696,11 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                             rank_base = a.ranks + a.rank_off[puzzle];
                             rank_stride = a.rank_stride[puzzle];
                         }
-                        const uint16_t *src = a.in_stack + (size_t)idx * a.stride;
                         for (int K = 0; K < 3; ++K)
                             for (uint32_t w = 0, nw = tv.nat(K); w < nw; ++w)
                                 avail[(tv.abase(K) + w) * kWave + lane] = tv.b[tv.off_valid(K) + w];
                         for (uint32_t j = 0; j < k; ++j) {
-                            const uint32_t t = src[j];
+                            const uint32_t t = item_move(idx, j);
                             S(j) = (STK)t;
                             const uint32_t bb = t & ~3u;
                             A(desc_abase(tv.nbr(j)), bb >> 6) &= ~(0xFull << (bb & 63));
@@ -638,7 +723,75 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
                         atomicAdd(&a.flags[2], 1u);
                         active = true;
                     }
+                    return take;
+    };
+
+    while (true) {
+        if (PERSIST) {
+            // Idle lanes take tickets, one atomic per wave; a lane holding a
+            // ticket starts once its item is written.
+            if (owed && !active && write_handover(owed_ticket, owed_floor)) owed = false;
+            const uint64_t idle = __ballot(!active && !waiting && !owed);
+            if (idle) {
+                const uint32_t n_idle = (uint32_t)__popcll(idle);
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(a.q_head, n_idle);
+                base = __shfl(base, leader);
+                if (!active && !waiting && !owed) {
+                    ticket = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+                    waiting = true;
                 }
+            }
+            if (waiting) {
+                const uint32_t slot = ticket & ((1u << a.q_shift) - 1), gen = ticket >> a.q_shift;
+                if (__hip_atomic_load(&a.q_ready[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 2 * gen + 1) {
+                    __atomic_signal_fence(__ATOMIC_ACQUIRE);   // the item's reads stay after the state's
+                    waiting = false;
+                    const bool took = take_item(slot);
+                    // the slot is free for the next generation once read (every
+                    // value read from it is in use by now, so the reads are done)
+                    __hip_atomic_store(&a.q_ready[slot], 2 * gen + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (!took) atomicSub(a.q_pending, 1u);
+                }
+            }
+            if (!__ballot(active)) {
+                // Nothing to search in this wave: the launch is over once no
+                // item is queued or being searched anywhere (every later ticket
+                // would stay empty), else wait for a hand-over.
+                uint32_t pend = 0;
+                if (lane == 0) pend = __hip_atomic_load(a.q_pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__shfl(pend, 0) == 0) break;
+                const uint64_t now = wall_clock64();
+                if (!idle_since) idle_since = now;
+                if (now - idle_since > kStallTicks) {
+                    if (lane == 0) atomicOr(a.q_stalled, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+                continue;
+            }
+            idle_since = 0;
+            if (!active) continue;
+        } else {
+        // Refill idle lanes with one queue atomic per wave.
+        const uint64_t idle = __ballot(!active);
+        if (idle && !queue_empty) {
+            const uint32_t n_idle = (uint32_t)__popcll(idle);
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(a.in_head, n_idle);
+            base = __shfl(base, leader);
+            if (base + n_idle >= n_in) {
+                queue_empty = true;
+                // Every item is taken: from now on busy lanes split early so
+                // the pass ends soon and the rest is shared out next pass.
+                if (lane == leader) atomicOr(&a.flags[3], 1u);
+            }
+            if (!active) {
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
+                const uint32_t idx = base + rank;
+                if (idx < n_in) (void)take_item(idx);
             }
         }
         if (!__ballot(active)) {
@@ -646,13 +799,22 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             continue;
         }
         if (!active) continue;
+        }
 
-        if (a.deadline && wall_clock64() > a.deadline) {
-            // out of time: the rest of this item is not searched here; with
-            // keep_rest it is requeued as one item (level k above `last`,
-            // which is what the candidates held in c stand for)
+        bool over = false;
+        if (LIMITS && pub_step && (uint32_t)nodes - pub_at >= pub_step) {
+            const uint32_t add = (uint32_t)nodes - pub_at;
+            const unsigned long long before = atomicAdd(&a.nodes[puzzle], (unsigned long long)add);
+            over = before + add >= a.node_limit[puzzle];
+            pub_at = (uint32_t)nodes;
+        }
+        if (over || (a.deadline && wall_clock64() > a.deadline)) {
+            // out of time, or the puzzle's node limit reached: the rest of
+            // this item is not searched here; with keep_rest it is requeued
+            // as one item (level k above `last`, which is what the candidates
+            // held in c stand for)
             atomicOr(&a.truncated[puzzle], 1u);
-            atomicOr(a.timed_out, 1u);
+            if (!over || a.keep_rest) atomicOr(a.timed_out, 1u);
             if (a.keep_rest) {
                 const uint32_t o = atomicAdd(a.out_count, 1u);
                 if (o < a.out_cap) {
@@ -666,8 +828,60 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             finish();
             continue;
         }
-        const bool drain = __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-        const uint64_t budget = drain ? a.drain_budget : a.step_budget;
+        if (PERSIST) {
+            // Hand over the rest of the floor level (its candidates above the
+            // transition this lane holds there) as a new item and go on one
+            // level deeper, after step_budget steps since the last hand-over
+            // -- drain_budget while lanes wait for work -- unless the queue
+            // already holds q_backlog items nobody has taken.  The queue's
+            // counters are read every kQueueRounds rounds: each read is a
+            // round trip to the coherence point that the whole wave waits
+            // for (C1: 106.2 G nodes/s reading them every round, 111.2 every
+            // 4th, 112.8 every 8th).
+            if (owed && write_handover(owed_ticket, owed_floor)) owed = false;
+            const int first = __builtin_ctzll(__ballot(true));
+            // the first active lane's count decides, so the wave agrees
+            const bool check = ((uint32_t)__shfl((int)++rounds, first) & (kQueueRounds - 1)) == 0;
+            uint32_t hq = 0, qq = 0;
+            if (check && lane == first) {
+                hq = __hip_atomic_load(a.q_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                qq = __hip_atomic_load(a.q_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            hq = __shfl(hq, first);
+            qq = __shfl(qq, first);
+            const bool hungry = hq > qq;
+            const bool room = check && (hungry || qq - hq < a.q_backlog);
+            const bool give = room && !owed && k > kmin &&
+                              steps - give_mark >= (hungry ? a.drain_budget : a.step_budget);
+            const uint64_t gm = __ballot(give);
+            if (gm) {
+                const uint32_t ng = (uint32_t)__popcll(gm);
+                const int leader = __ffsll((unsigned long long)gm) - 1;
+                uint32_t base = 0;
+                if (lane == leader) {
+                    // counted (and the count settled) before any of them can
+                    // be taken and finished
+                    atomicAdd(a.q_pending, ng);
+                    wave_settle();
+                    base = atomicAdd(a.q_tail, ng);
+                    atomicAdd(&a.flags[1], ng);
+                }
+                base = __shfl(base, leader);
+                if (give) {
+                    const uint32_t tk = base + (uint32_t)__popcll(gm & ((1ull << lane) - 1));
+                    if (!write_handover(tk, kmin)) {
+                        owed = true;
+                        owed_ticket = tk;
+                        owed_floor = kmin;
+                    }
+                    kmin += 1;
+                    give_mark = steps;
+                }
+            }
+        }
+        const bool drain =
+            !PERSIST && __hip_atomic_load(&a.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        const uint64_t budget = PERSIST ? ~0ull : (drain ? a.drain_budget : a.step_budget);
         for (uint32_t r = 0; r < a.steps_per_round; ++r) {
             // One candidate set per iteration, of level tk above tl:
             //   fresh item   tk = k,     tl = last
@@ -864,26 +1078,33 @@ void launch_expand(const DfsArgs &a, hipStream_t s) {
     else hipLaunchKernelGGL((expand_kernel<W, false>), dim3(blocks), dim3(threads), 0, s, a);
 }
 
-template <int W>
-const void *dfs_fn(bool lds, bool ranked) {
+template <int W, bool PERSIST, bool LIMITS>
+const void *dfs_fn_p(bool lds, bool ranked) {
     if (ranked)
-        return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, true>)
-                   : reinterpret_cast<const void *>(&dfs_kernel<W, false, true>);
-    return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, false>)
-               : reinterpret_cast<const void *>(&dfs_kernel<W, false, false>);
+        return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, true, PERSIST, LIMITS>)
+                   : reinterpret_cast<const void *>(&dfs_kernel<W, false, true, PERSIST, LIMITS>);
+    return lds ? reinterpret_cast<const void *>(&dfs_kernel<W, true, false, PERSIST, LIMITS>)
+               : reinterpret_cast<const void *>(&dfs_kernel<W, false, false, PERSIST, LIMITS>);
 }
 
-// VGPRs of the DFS kernel for this row width and table placement.
-uint32_t dfs_vgprs(int w, bool lds, bool ranked) {
+template <int W>
+const void *dfs_fn(bool lds, bool ranked, bool persist, bool limits) {
+    if (persist) return limits ? dfs_fn_p<W, true, true>(lds, ranked) : dfs_fn_p<W, true, false>(lds, ranked);
+    return limits ? dfs_fn_p<W, false, true>(lds, ranked) : dfs_fn_p<W, false, false>(lds, ranked);
+}
+
+// VGPRs of the DFS kernel for this row width, table placement, schedule and
+// node-limit form.
+uint32_t dfs_vgprs(int w, bool lds, bool ranked, bool persist, bool limits) {
     const void *fn = nullptr;
     switch (w) {
-        case 1: fn = dfs_fn<1>(lds, ranked); break;
-        case 2: fn = dfs_fn<2>(lds, ranked); break;
-        case 3: fn = dfs_fn<3>(lds, ranked); break;
-        case 4: fn = dfs_fn<4>(lds, ranked); break;
-        case 6: fn = dfs_fn<6>(lds, ranked); break;
-        case 8: fn = dfs_fn<8>(lds, ranked); break;
-        default: fn = dfs_fn<16>(lds, ranked); break;
+        case 1: fn = dfs_fn<1>(lds, ranked, persist, limits); break;
+        case 2: fn = dfs_fn<2>(lds, ranked, persist, limits); break;
+        case 3: fn = dfs_fn<3>(lds, ranked, persist, limits); break;
+        case 4: fn = dfs_fn<4>(lds, ranked, persist, limits); break;
+        case 6: fn = dfs_fn<6>(lds, ranked, persist, limits); break;
+        case 8: fn = dfs_fn<8>(lds, ranked, persist, limits); break;
+        default: fn = dfs_fn<16>(lds, ranked, persist, limits); break;
     }
     hipFuncAttributes attr;
     if (hipFuncGetAttributes(&attr, fn) != hipSuccess || attr.numRegs <= 0) {
@@ -893,14 +1114,25 @@ uint32_t dfs_vgprs(int w, bool lds, bool ranked) {
     return (uint32_t)attr.numRegs;
 }
 
+template <int W, bool PERSIST, bool LIMITS>
+void launch_dfs_p(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes, hipStream_t s) {
+    if (a.ranks) {
+        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, true, PERSIST, LIMITS>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+        else hipLaunchKernelGGL((dfs_kernel<W, false, true, PERSIST, LIMITS>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    } else {
+        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, false, PERSIST, LIMITS>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+        else hipLaunchKernelGGL((dfs_kernel<W, false, false, PERSIST, LIMITS>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    }
+}
+
 template <int W>
 void launch_dfs(const DfsArgs &a, bool lds, uint32_t blocks, uint32_t threads, size_t lds_bytes, hipStream_t s) {
-    if (a.ranks) {
-        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-        else hipLaunchKernelGGL((dfs_kernel<W, false, true>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+    if (a.persist) {
+        if (a.limits) launch_dfs_p<W, true, true>(a, lds, blocks, threads, lds_bytes, s);
+        else launch_dfs_p<W, true, false>(a, lds, blocks, threads, lds_bytes, s);
     } else {
-        if (lds) hipLaunchKernelGGL((dfs_kernel<W, true, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
-        else hipLaunchKernelGGL((dfs_kernel<W, false, false>), dim3(blocks), dim3(threads), lds_bytes, s, a);
+        if (a.limits) launch_dfs_p<W, false, true>(a, lds, blocks, threads, lds_bytes, s);
+        else launch_dfs_p<W, false, false>(a, lds, blocks, threads, lds_bytes, s);
     }
 }
 
@@ -1050,6 +1282,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         }
     }
 
+    // One persistent launch (not with the fixed scratch buffer, whose room
+    // goes to the pass queues, nor for ordered results, which record the
+    // items of each pass), or DFS passes.
+    const bool persist = !ws.is_arena() && !record &&
+                         (cfg.schedule == EnumConfig::kPersistent || (cfg.schedule == EnumConfig::kAuto && n_tab == 1));
+
     // Launch geometry: as many resident waves as registers (512 VGPRs per
     // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
     // wave's stacks) allow; the block size that fits the most waves wins.
@@ -1062,7 +1300,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     const bool lds_table = n_tab == 1 && (size_t)max_table * 8 + 4 * per_wave_bytes <= dev.lds_per_block;
     const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
     const size_t lds_cu = dev.lds_per_cu ? dev.lds_per_cu : dev.lds_per_block;
-    const uint32_t vgprs = dfs_vgprs(maxw, lds_table, ranks != nullptr);
+    const uint32_t vgprs = dfs_vgprs(maxw, lds_table, ranks != nullptr, persist, cfg.node_limit != 0);
     const uint32_t reg_waves = 4 * std::min<uint32_t>(8, 512 / ((vgprs + 7) & ~7u));
     auto resident = [&](uint32_t w) -> uint32_t {
         const size_t bytes = table_bytes + w * per_wave_bytes;
@@ -1161,10 +1399,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         record ? ws.get<unsigned long long>("sol_local", (size_t)n * std::max(sol_per_puzzle, 1u)) : nullptr;
     // Batches of several tables read them through the caches: group the
     // items by puzzle before each pass (roots of one table gain nothing).
-    // Not under a node limit: a puzzle's count grows only as its items
-    // finish, and grouped items are taken together, so the limit would be
-    // overshot many times over (48 node-limited C2 instances: 8x the nodes).
-    const bool group_items = !lds_table && n_tab > 1 && cfg.node_limit == 0;
+    // Under a node limit the items publish their progress to the puzzle's
+    // count as they run, so the grouped items of one puzzle, taken together,
+    // still stop near the limit.
+    const bool group_items = !lds_table && n_tab > 1;
     unsigned int *d_group = group_items ? ws.get<unsigned int>("group_count", n) : nullptr;
 
     // Work queues.  A pass emits at most two items per input item and always
@@ -1189,6 +1427,26 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                             ws.get<uint16_t>("stack1", (size_t)cap * stride)};
     uint64_t *d_group_hdr = group_items ? ws.get<uint64_t>("group_hdr", cap) : nullptr;
     uint16_t *d_group_stack = group_items ? ws.get<uint16_t>("group_stack", (size_t)cap * stride) : nullptr;
+    // The persistent launch's ring of 2^q_shift work-item slots: the queue at
+    // the launch first, then one slot per hand-over.
+    uint32_t q_cap = 0, q_shift = 0;
+    uint64_t *d_qhdr = nullptr;
+    uint16_t *d_qstack = nullptr;
+    unsigned int *d_qready = nullptr, *d_qcount = nullptr;
+    if (persist) {
+        // A power of two holding the starting queue and eight items per lane.
+        // At most a quarter of it waits untaken (plus one item per lane from
+        // waves that give at the same moment), so a slot's previous item has
+        // been claimed -- by a lane that waits for it, not one that owes a
+        // write -- before its slot is reserved again.
+        const uint64_t want_cap = std::max<uint64_t>(cap, lanes * 8);
+        while ((1ull << q_shift) < want_cap) ++q_shift;
+        q_cap = 1u << q_shift;
+        d_qhdr = ws.get<uint64_t>("ticket_hdr", q_cap);
+        d_qstack = ws.get<uint16_t>("ticket_stack", (size_t)q_cap * stride);
+        d_qready = ws.get<unsigned int>("ticket_ready", q_cap);
+        d_qcount = ws.get<unsigned int>("ticket_counts", 4);
+    }
 
     // Root items: index 0 holds its first candidate (one node each), or the
     // given situations (their placements are not counted).
@@ -1254,6 +1512,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.rank_stride = d_rank_stride;
     a.timed_out = d_timed_out;
     a.keep_rest = keep_rest ? 1u : 0u;
+    a.limits = cfg.node_limit != 0 ? 1u : 0u;
     a.work = cfg.count_work ? d_work : nullptr;
     if (cfg.count_work) {
         const unsigned long long zero2[2] = {0, 0};
@@ -1310,7 +1569,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             if (pinned) (void)hipHostFree(pinned);
         }
     } graphs;
-    if (cfg.use_graph) {
+    if (cfg.use_graph && !persist) {
         if (ws.is_arena()) throw std::invalid_argument("graph replay allocates: not available with a fixed scratch buffer");
         EMP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&graphs.pinned), 8 * sizeof(unsigned int)));
         for (int dir = 0; dir < 2; ++dir) {
@@ -1332,6 +1591,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     }
     int cur = 0;
     bool expansion_grows = true;
+    uint32_t flat_expansions = 0;   // expansions in a row that did not grow the queue
     uint64_t n_items = n_init;
     // ordered results: each DFS pass's input items (their keys) and the
     // executions recorded so far
@@ -1351,6 +1611,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
         // multiply it (resumed searches pass through an expansion unchanged).
         const bool expand = n_items < expand_to && expansions < max_expansions && expansion_grows;
         const bool grouped = !expand && group_items && n_items > 1;
+        const bool one_launch = persist && !expand;
         if (grouped) {
             const uint32_t ni = (uint32_t)n_items, g = (ni + 255) / 256;
             EMP_CHECK(hipEventRecord(eg0, stream));
@@ -1364,6 +1625,96 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             EMP_CHECK(hipMemcpyAsync(d_stack[cur], d_group_stack, (size_t)ni * stride * 2, hipMemcpyDeviceToDevice,
                                      stream));
             EMP_CHECK(hipEventRecord(eg1, stream));
+        }
+        if (one_launch) {
+            // The queue becomes tickets 0..n-1 of the ticket array; one launch
+            // searches them and everything handed over from them.
+            const uint32_t ni = (uint32_t)n_items;
+            if (ni > q_cap)
+                throw Unsupported("the persistent search starts from " + std::to_string(ni) + " items, more than its " +
+                                  std::to_string(q_cap) + " slots");
+            EMP_CHECK(hipMemcpyAsync(d_qhdr, d_hdr[cur], (size_t)ni * 8, hipMemcpyDeviceToDevice, stream));
+            EMP_CHECK(hipMemcpyAsync(d_qstack, d_stack[cur], (size_t)ni * stride * 2, hipMemcpyDeviceToDevice, stream));
+            hipLaunchKernelGGL(fill_u32, dim3((ni + 255) / 256), dim3(256), 0, stream, d_qready, ni, 1u);
+            if (q_cap > ni)
+                hipLaunchKernelGGL(fill_u32, dim3((q_cap - ni + 255) / 256), dim3(256), 0, stream, d_qready + ni,
+                                   q_cap - ni, 0u);
+            hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
+            const unsigned int qc[4] = {0u, ni, ni, 0u};   // head, tail, pending, stalled
+            EMP_CHECK(hipMemcpyAsync(d_qcount, qc, sizeof(qc), hipMemcpyHostToDevice, stream));
+            DfsArgs pa = a;
+            pa.persist = 1;
+            pa.q_shift = q_shift;
+            pa.q_backlog = std::max<uint32_t>(q_cap / 4, 1u);
+            pa.q_head = d_qcount + 0;
+            pa.q_tail = d_qcount + 1;
+            pa.q_pending = d_qcount + 2;
+            pa.q_ready = d_qready;
+            pa.q_hdr = d_qhdr;
+            pa.q_stack = d_qstack;
+            pa.q_stalled = d_qcount + 3;
+            pa.in_hdr = d_qhdr;
+            pa.in_stack = d_qstack;
+            pa.n_in = ni;
+            pa.n_in_dev = nullptr;
+            pa.out_hdr = d_hdr[1 - cur];   // the rest of a cut run
+            pa.out_stack = d_stack[1 - cur];
+            pa.out_cap = cap;
+            pa.exec_base = 0;
+            EMP_CHECK(hipEventRecord(ev0, stream));
+            dispatch_dfs(maxw, pa, lds_table, blocks, threads, lds_bytes, stream);
+            EMP_CHECK(hipGetLastError());
+            EMP_CHECK(hipEventRecord(ev1, stream));
+            unsigned int counters[8], qend[4];
+            EMP_CHECK(hipMemcpyAsync(counters, d_counters, sizeof(counters), hipMemcpyDeviceToHost, stream));
+            EMP_CHECK(hipMemcpyAsync(qend, d_qcount, sizeof(qend), hipMemcpyDeviceToHost, stream));
+            EMP_CHECK(hipStreamSynchronize(stream));
+            float ms = 0;
+            EMP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            if (grouped) {
+                float gms = 0;
+                EMP_CHECK(hipEventElapsedTime(&gms, eg0, eg1));
+                stats.group_ms += gms;
+            }
+            stats.dfs_kernel_ms += ms;
+            ++stats.dfs_launches;
+            stats.splits += counters[3];
+            stats.items_processed += counters[4];
+            stats.tickets = qend[1];
+            if (qend[3]) throw DeviceError("a wave of the persistent search waited 20 s for work and gave up");
+            if (qend[2] != 0) throw DeviceError("the persistent search ended with work pending");
+            if (counters[2]) throw DeviceError("rest queue overflow in the persistent search");
+            if (counters[7])
+                throw std::invalid_argument("a situation places an identical piece while a lower-numbered twin is "
+                                            "still available (such a piece is never a candidate)");
+            if (keep_rest) {
+                unsigned int cut = 0;
+                EMP_CHECK(hipMemcpyAsync(&cut, d_timed_out, 4, hipMemcpyDeviceToHost, stream));
+                EMP_CHECK(hipStreamSynchronize(stream));
+                if (cut) {
+                    const uint32_t m = std::min<uint32_t>(counters[1], cap);
+                    std::vector<uint64_t> rh(m);
+                    std::vector<uint16_t> rs((size_t)m * stride);
+                    EMP_CHECK(hipMemcpyAsync(rh.data(), d_hdr[1 - cur], (size_t)m * 8, hipMemcpyDeviceToHost, stream));
+                    EMP_CHECK(hipMemcpyAsync(rs.data(), d_stack[1 - cur], (size_t)m * stride * 2,
+                                             hipMemcpyDeviceToHost, stream));
+                    EMP_CHECK(hipStreamSynchronize(stream));
+                    work->rest.clear();
+                    for (uint32_t i = 0; i < m; ++i) {
+                        const uint64_t h = rh[i];
+                        EnumItem it;
+                        it.floor = (uint32_t)(h >> 20) & 0x7FF;
+                        it.level = (uint32_t)(h >> 31) & 0x7FF;
+                        it.last = (int)((uint32_t)(h >> 42) & 0xFFF) - 1;
+                        it.stack.assign(rs.begin() + (size_t)i * stride, rs.begin() + (size_t)i * stride + it.level);
+                        work->rest.push_back(std::move(it));
+                    }
+                }
+            }
+            if (verbose)
+                std::fprintf(stderr, "emp_hip: persistent in=%u tickets=%u handovers=%u %.3f ms\n", ni, qend[1],
+                             counters[3], ms);
+            break;
         }
         if (!cfg.use_graph || expand)
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
@@ -1427,7 +1778,15 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                 expansions = max_expansions;
                 continue;
             }
-            expansion_grows = counters[1] >= n_items + n_items / 4;
+            // DFS passes take over once an expansion stops multiplying the
+            // queue (they split items themselves).  A persistent launch
+            // without a node limit starts from what the expansions made, so
+            // they go on through a few levels that do not grow (corners,
+            // frame turns); under a node limit breadth would spend the budget
+            // on shallow levels, so they stop as for passes.
+            flat_expansions = counters[1] > n_items ? 0 : flat_expansions + 1;
+            expansion_grows = persist && !cfg.node_limit ? counters[1] > 0 && flat_expansions < 4
+                                                         : counters[1] >= n_items + n_items / 4;
         } else {
             expansion_grows = true;
             stats.dfs_kernel_ms += ms;

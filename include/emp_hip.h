@@ -103,9 +103,9 @@ typedef struct {
                                   at most 16 (4 for boards with more than 64
                                   pieces of one kind)                          */
     uint32_t steps_per_round;  /* steps between work refills, 0 = default       */
-    uint64_t step_budget;      /* steps per work item per pass before it is
-                                  requeued (and split in two while the queue
-                                  has room), 0 = default                      */
+    uint64_t step_budget;      /* passes only: steps per work item per pass
+                                  before it is requeued (and split in two
+                                  while the queue has room), 0 = default      */
     uint64_t target_items;     /* prefix-split until this many items, 0 = auto  */
     uint64_t node_limit;       /* per puzzle: stop taking its work items once
                                   this many nodes were counted, 0 = none      */
@@ -119,8 +119,24 @@ typedef struct {
     uint32_t count_work;       /* 1: count the algorithmic work of every
                                   placement into the stats (an instrumented
                                   run, for the roofline; plain order only)    */
-    uint32_t reserved;
+    uint32_t schedule;         /* EMP_HIP_SCHEDULE_*                            */
 } emp_hip_enum_options;
+
+/* How the DFS work is shared out.
+ * PERSISTENT: one launch; work items sit in a device-side ring, idle lanes
+ *   take the next one, and a lane hands over the rest of the shallowest level
+ *   of its search as a new item every step_budget steps (512 while lanes wait
+ *   for work), so the search spreads without host round trips.
+ * PASSES: launches of bounded length; items over their step budget split
+ *   into the next launch's queue, which batches regroup by puzzle before
+ *   each launch (the form emp_hip_enumerate_noalloc always uses).
+ * AUTO (0, the default): PERSISTENT for one puzzle (or several roots of
+ *   one), PASSES for batches of several puzzles, whose tables are read
+ *   through the caches and gain from the regrouping.  Ordered results
+ *   (emp_hip_enumerate_items with node_index) always use PASSES. */
+#define EMP_HIP_SCHEDULE_AUTO 0u
+#define EMP_HIP_SCHEDULE_PASSES 1u
+#define EMP_HIP_SCHEDULE_PERSISTENT 2u
 
 typedef struct {
     uint64_t n_nodes;          /* placements (the reference's node count)      */
@@ -153,6 +169,8 @@ typedef struct {
                                   per placement (M = ceil(4P/64) u64 words, n
                                   the placed neighbours of the next index)    */
     uint64_t work_ops;         /* count_work: integer ops, 2 M (n + 1) + 2 M  */
+    uint64_t tickets;          /* persistent schedule: items searched (the
+                                  starting items and the hand-overs)         */
 } emp_hip_enum_stats;
 
 void emp_hip_enum_default_options(emp_hip_enum_options *options);

@@ -71,13 +71,56 @@ def test_c0_batched(gpu_context):
 
 
 def test_forced_splits_and_tiny_budget(gpu_context):
-    """A step budget of 64 splits nearly every item many times over."""
+    """DFS passes: a step budget of 64 splits nearly every item many times over."""
     w, h = 6, 6
     c, _ = emp.generate(1, w, h, 3, 6)
     print("case: forced splits 6x6", flush=True)
-    (r,), st = gpu_context.enumerate([(w, h, c)], step_budget=64, target_items=1)
+    (r,), st = gpu_context.enumerate([(w, h, c)], step_budget=64, target_items=1, schedule="passes")
     assert st["splits"] > 1000
     _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_persistent_from_one_item(gpu_context):
+    """One persistent launch from a single starting item: every other lane's
+    work comes from hand-overs (the rest of a busy lane's floor level), and
+    the enumeration is the oracle's."""
+    w, h = 6, 6
+    c, _ = emp.generate(1, w, h, 3, 6)
+    print("case: persistent, one starting item 6x6", flush=True)
+    (r,), st = gpu_context.enumerate([(w, h, c)], target_items=1, schedule="persistent")
+    assert st["dfs_launches"] == 1 and st["splits"] > 100 and st["tickets"] > st["splits"]
+    _check_same(r, o.enumerate_(w, h, c))
+
+
+def test_persistent_ring_wraps(gpu_context):
+    """One wave per CU and a hand-over every 64 steps: the work-item ring (a
+    power of two below 16 items per lane) is reused several times over in
+    one launch, every slot refilled only after its item was read, and the
+    enumeration stays exact."""
+    w, h = 8, 8
+    c, _ = emp.generate(11, w, h, 6, 16)
+    ref = o.enumerate_(w, h, c)
+    print("case: persistent, ring reuse 8x8 B=6 I=16 seed 11", ref["n_nodes"], flush=True)
+    (r,), st = gpu_context.enumerate([(w, h, c)], target_items=1, waves_per_cu=1, step_budget=64,
+                                     schedule="persistent")
+    lanes = st["blocks"] * st["threads"]
+    assert st["dfs_launches"] == 1 and st["tickets"] > 16 * lanes, (st["tickets"], lanes)
+    _check_same(r, ref)
+
+
+def test_persistent_matches_passes_on_a_batch(gpu_context):
+    """The two schedules give the same node counts and solution sets on a
+    batch of uneven puzzles."""
+    rng = np.random.default_rng(5)
+    batch = [(w, w, emp.generate(3000 + s, w, w, 3, int(rng.integers(5, 9)))[0])
+             for s, w in enumerate([4, 5, 6, 5, 6, 6, 4, 5])]
+    print("case: persistent vs passes, batch of 8", flush=True)
+    p, st_p = gpu_context.enumerate(batch, max_solutions=4096, schedule="persistent")
+    q, st_q = gpu_context.enumerate(batch, max_solutions=4096, schedule="passes")
+    assert st_p["dfs_launches"] == 1 and st_q["dfs_launches"] >= 1
+    for a, b in zip(p, q):
+        assert (a["n_nodes"], a["n_solutions"], a["complete"]) == (b["n_nodes"], b["n_solutions"], True)
+        assert _sorted_boards(a["boards"]) == _sorted_boards(b["boards"])
 
 
 def test_full_queue_stops_splitting_and_stays_exact(gpu_context):
@@ -86,7 +129,8 @@ def test_full_queue_stops_splitting_and_stays_exact(gpu_context):
     w, h = 6, 6
     c, _ = emp.generate(1, w, h, 3, 6)
     print("case: tiny queue 6x6", flush=True)
-    (r,), st = gpu_context.enumerate([(w, h, c)], step_budget=64, target_items=64, queue_items=256)
+    (r,), st = gpu_context.enumerate([(w, h, c)], step_budget=64, target_items=64, queue_items=256,
+                                     schedule="passes")
     _check_same(r, o.enumerate_(w, h, c))
 
 
@@ -229,7 +273,7 @@ def test_graph_replays_match_eager(gpu_context):
         print("case: graph replay,", name, flush=True)
         eager, st_e = gpu_context.enumerate(puzzles, max_solutions=4096, **kw)
         for rep in range(2):
-            graph, st_g = gpu_context.enumerate(puzzles, max_solutions=4096, use_graph=True, **kw)
+            graph, st_g = gpu_context.enumerate(puzzles, max_solutions=4096, use_graph=True, schedule="passes", **kw)
             assert st_g["dfs_launches"] >= 1
             for (pw, ph, pc), e, g in zip(puzzles, eager, graph):
                 assert (g["n_nodes"], g["n_solutions"], g["complete"]) == (e["n_nodes"], e["n_solutions"], True)
@@ -249,7 +293,7 @@ def test_node_limit_holds_when_no_item_can_split(gpu_context):
     limit = 200_000
     print("case: node limit with a full queue", flush=True)
     res, st = gpu_context.enumerate(puzzles, max_solutions=4, node_limit=limit, step_budget=1 << 10,
-                                    queue_items=len(puzzles), time_limit_ms=30_000.0)
+                                    queue_items=len(puzzles), time_limit_ms=30_000.0, schedule="passes")
     assert st["total_ms"] < 15_000
     assert sum(not r["complete"] for r in res) >= 1
     assert all(r["n_nodes"] >= limit for r in res if not r["complete"])
@@ -262,7 +306,8 @@ def test_queue_of_one_item_per_search_completes_exactly(gpu_context):
     enumeration stays exact and the passes stay short."""
     puzzles = _c0_puzzles(4, 300)
     print("case: one queue slot per search, budget 2^10", flush=True)
-    res, st = gpu_context.enumerate(puzzles, step_budget=1 << 10, queue_items=len(puzzles), target_items=1)
+    res, st = gpu_context.enumerate(puzzles, step_budget=1 << 10, queue_items=len(puzzles), target_items=1,
+                                    schedule="passes")
     assert st["dfs_launches"] > 4
     for (w, h, c), r in zip(puzzles, res):
         _check_same(r, o.enumerate_(w, h, c))
@@ -277,7 +322,7 @@ def test_small_step_budget_under_node_limit_is_bounded(gpu_context):
     limit = 2_000_000
     print("case: step budget 2^14 under a node limit", flush=True)
     res, st = gpu_context.enumerate(puzzles, max_solutions=4, node_limit=limit, step_budget=1 << 14,
-                                    time_limit_ms=60_000.0)
+                                    time_limit_ms=60_000.0, schedule="passes")
     assert st["total_ms"] < 20_000
     assert all(r["complete"] or r["n_nodes"] >= limit for r in res)
     for (w, h, c), r in list(zip(puzzles, res))[:24]:
