@@ -5,12 +5,14 @@ Default line (the metric's plain leg, with the batched leg beside it):
   pieces, 4 border colours, 8 interior colours, generator seed 7), corner
   kind-index 0 fixed at (0,0), spiral order -- the plain backtracker
   configuration the metric is quoted on.  Each step runs the whole
-  enumeration on the GPU (tables built and uploaded by the call).
+  enumeration on the GPU (tables built and uploaded by the call, breadth
+  expansions, then one persistent DFS launch).
   batched: one step = 4096 synthetic puzzles of side 5..10, 3..5 border and
   4..12 interior colours in one enumeration (DFS passes replayed from
-  hipGraphs), each stopped at 2e7 nodes (the cells with more than 10^9
-  expected solutions cannot be enumerated by anyone; the count of complete and
-  cut instances is reported).
+  hipGraphs, items regrouped by puzzle before each pass), each stopped near
+  2e7 nodes (the cells with more than 10^9 expected solutions cannot be
+  enumerated by anyone; the count of complete and cut instances is
+  reported).
 
 Prints one JSON line:
   value            search nodes per second (placements, the reference's count)
@@ -141,7 +143,7 @@ def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LI
         "ms_per_step": elapsed * 1e3 / steps,
         "config": {"workload": "batched enumeration: %d synthetic puzzles, side 5..10, border colours 3..5, "
                                "interior colours 4..12, one enumeration, DFS passes replayed from hipGraphs, "
-                               "each instance stopped at %d nodes" % (len(puzzles), limit),
+                               "each instance stopped near %d nodes" % (len(puzzles), limit),
                    "completed": len(done), "cut_at_node_limit": len(res) - len(done),
                    "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99), "nodes_per_step": nodes // steps,
                    "dfs_launches_per_step": launches // steps, "kernel_ms_per_step": kernel_ms / steps,
@@ -417,6 +419,8 @@ def main():
                                "seed %d, corner kind-index 0 fixed, spiral order" % args.seed,
                    "nodes_per_step": r["n_nodes"], "solutions": r["n_solutions"],
                    "dfs_launches_per_step": st["dfs_launches"], "grid": [st["blocks"], st["threads"]],
+                   "schedule": "persistent" if st["tickets"] else "passes", "work_items": st["tickets"],
+                   "handovers": st["splits"],
                    "lds_tables": bool(st["lds_tables"]), "tuned": tuned or None},
         "cpu_baseline": cpu,
     }
