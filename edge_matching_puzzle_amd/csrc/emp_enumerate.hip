@@ -59,6 +59,12 @@ struct DfsArgs {
     const uint16_t *in_stack;      // [n_in][stride]
     uint32_t n_in;
     unsigned int *in_head;
+    // Batches (items grouped by puzzle): the input is cut into kRegions
+    // contiguous regions, region r taken first by the blocks with blockIdx
+    // mod kRegions == r -- the blocks the dispatcher places on XCD r -- so
+    // each XCD's L2 holds the tables of its own run of puzzles; a wave moves
+    // on to the next region once its own is taken.  Null: one head (in_head).
+    unsigned int *region_head;     // [kRegions] items taken from each region
     // output queue (residual items)
     uint64_t *out_hdr;
     uint16_t *out_stack;
@@ -151,6 +157,10 @@ constexpr uint64_t kStallTicks = 2000000000ull;   // 20 s
 // A persistent launch's lanes read the queue counters (and may hand over)
 // every kQueueRounds rounds of steps_per_round steps.
 constexpr uint32_t kQueueRounds = 8;
+
+// Input regions of a batched pass: one per XCD (MI355X has eight, and the
+// dispatcher deals workgroups to them round-robin).
+constexpr uint32_t kRegions = 8;
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
 // transitions whose rank at this index is above the rank of `last`.
@@ -560,6 +570,8 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
 
     bool active = false;
     bool queue_empty = false;
+    uint32_t region = blockIdx.x % kRegions;   // batched passes: the input region taken from
+    uint32_t regions_left = kRegions;          // regions not yet seen taken
     bool waiting = false;                    // persistent launch: holds a ticket not started yet
     uint32_t ticket = 0;
     bool owed = false;                       // persistent launch: a reserved ticket whose slot was in use
@@ -779,10 +791,24 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
         if (idle && !queue_empty) {
             const uint32_t n_idle = (uint32_t)__popcll(idle);
             const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(a.in_head, n_idle);
-            base = __shfl(base, leader);
-            if (base + n_idle >= n_in) {
+            uint32_t base = 0, end = n_in;
+            bool taken_all = false;
+            if (a.region_head) {
+                // items [lo, hi) of the wave's current region
+                const uint32_t lo = (uint32_t)((uint64_t)n_in * region / kRegions);
+                end = (uint32_t)((uint64_t)n_in * (region + 1) / kRegions);
+                if (lane == leader) base = lo + atomicAdd(&a.region_head[region], n_idle);
+                base = __shfl(base, leader);
+                if (base + n_idle >= end) {
+                    region = region + 1 == kRegions ? 0u : region + 1;
+                    taken_all = --regions_left == 0;
+                }
+            } else {
+                if (lane == leader) base = atomicAdd(a.in_head, n_idle);
+                base = __shfl(base, leader);
+                taken_all = base + n_idle >= n_in;
+            }
+            if (taken_all) {
                 queue_empty = true;
                 // Every item is taken: from now on busy lanes split early so
                 // the pass ends soon and the rest is shared out next pass.
@@ -791,7 +817,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             if (!active) {
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1));
                 const uint32_t idx = base + rank;
-                if (idx < n_in) (void)take_item(idx);
+                if (idx < end) (void)take_item(idx);
             }
         }
         if (!__ballot(active)) {
@@ -1404,6 +1430,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // still stop near the limit.
     const bool group_items = !lds_table && n_tab > 1;
     unsigned int *d_group = group_items ? ws.get<unsigned int>("group_count", n) : nullptr;
+    unsigned int *d_region_head = group_items ? ws.get<unsigned int>("region_head", kRegions) : nullptr;
 
     // Work queues.  A pass emits at most two items per input item and always
     // has room for one; a full queue only stops items splitting in two.
@@ -1501,6 +1528,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.node_limit = d_limit;
     a.truncated = d_trunc;
     a.in_head = d_counters + 0;
+    a.region_head = d_region_head;
     a.out_count = d_counters + 1;
     a.flags = d_counters + 2;
     a.sol_stack = d_sol_stack;
@@ -1583,6 +1611,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             ga.n_in_dev = d_n_in;
             EMP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
+            if (d_region_head) hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_region_head, kRegions, 0u);
             dispatch_dfs(maxw, ga, lds_table, blocks, threads, lds_bytes, stream);
             (void)hipMemcpyAsync(graphs.pinned, d_counters, 8 * sizeof(unsigned int), hipMemcpyDeviceToHost, stream);
             EMP_CHECK(hipStreamEndCapture(stream, &graphs.g[dir]));
@@ -1716,8 +1745,10 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
                              counters[3], ms);
             break;
         }
-        if (!cfg.use_graph || expand)
+        if (!cfg.use_graph || expand) {
             hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_counters, 7u, 0u);
+            if (d_region_head) hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, d_region_head, kRegions, 0u);
+        }
         if (expand) {
             EMP_CHECK(hipMemcpyAsync(d_nodes_save, d_nodes, n * 8, hipMemcpyDeviceToDevice, stream));
             EMP_CHECK(hipMemcpyAsync(d_sols_save, d_sols, n * 8, hipMemcpyDeviceToDevice, stream));
