@@ -161,6 +161,11 @@ constexpr uint32_t kQueueRounds = 8;
 // Input regions of a batched pass: one per XCD (MI355X has eight, and the
 // dispatcher deals workgroups to them round-robin).
 constexpr uint32_t kRegions = 8;
+// Items a wave of a batched pass takes from its region at a time.
+#ifndef EMP_RUN_ITEMS
+#define EMP_RUN_ITEMS 64
+#endif
+constexpr uint32_t kRunItems = EMP_RUN_ITEMS;
 
 // Plain order: transitions above `last` in index order.  Ranked order: the
 // transitions whose rank at this index is above the rank of `last`.
@@ -572,6 +577,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     bool queue_empty = false;
     uint32_t region = blockIdx.x % kRegions;   // batched passes: the input region taken from
     uint32_t regions_left = kRegions;          // regions not yet seen taken
+    uint32_t run_next = 0, run_end = 0;        // the wave's run of items: [run_next, run_end) not taken
     bool waiting = false;                    // persistent launch: holds a ticket not started yet
     uint32_t ticket = 0;
     bool owed = false;                       // persistent launch: a reserved ticket whose slot was in use
@@ -794,15 +800,28 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             uint32_t base = 0, end = n_in;
             bool taken_all = false;
             if (a.region_head) {
-                // items [lo, hi) of the wave's current region
-                const uint32_t lo = (uint32_t)((uint64_t)n_in * region / kRegions);
-                end = (uint32_t)((uint64_t)n_in * (region + 1) / kRegions);
-                if (lane == leader) base = lo + atomicAdd(&a.region_head[region], n_idle);
-                base = __shfl(base, leader);
-                if (base + n_idle >= end) {
-                    region = region + 1 == kRegions ? 0u : region + 1;
-                    taken_all = --regions_left == 0;
+                // The wave takes a run of kRunItems consecutive items of its
+                // region and refills its lanes from that run only, so its
+                // lanes hold items of one or two puzzles and a CU's waves
+                // read few tables.
+                while (run_next >= run_end && regions_left) {
+                    const uint32_t lo = (uint32_t)((uint64_t)n_in * region / kRegions);
+                    const uint32_t hi = (uint32_t)((uint64_t)n_in * (region + 1) / kRegions);
+                    uint32_t b = 0;
+                    if (lane == leader) b = lo + atomicAdd(&a.region_head[region], kRunItems);
+                    b = __shfl(b, leader);
+                    if (b < hi) {
+                        run_next = b;
+                        run_end = min(b + kRunItems, hi);
+                    } else {
+                        region = region + 1 == kRegions ? 0u : region + 1;
+                        --regions_left;
+                    }
                 }
+                base = run_next;
+                end = run_end;
+                run_next = min(run_next + n_idle, run_end);
+                taken_all = run_next >= run_end && !regions_left;
             } else {
                 if (lane == leader) base = atomicAdd(a.in_head, n_idle);
                 base = __shfl(base, leader);
