@@ -32,7 +32,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <exception>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -161,6 +163,9 @@ constexpr uint32_t kQueueRounds = 8;
 // Input regions of a batched pass: one per XCD (MI355X has eight, and the
 // dispatcher deals workgroups to them round-robin).
 constexpr uint32_t kRegions = 8;
+// Host threads that build the tables of a large batch.
+constexpr uint32_t kTableThreads = 16;
+
 // Items a wave of a batched pass takes from its region at a time.
 #ifndef EMP_RUN_ITEMS
 #define EMP_RUN_ITEMS 64
@@ -1103,16 +1108,28 @@ __global__ void __launch_bounds__(1024) exclusive_scan_u32(unsigned int *v, uint
     }
 }
 
+// kScatterLanes lanes per item: the first takes the item's slot, then the
+// lanes copy the placed part of its stack row (the transitions of indices
+// below its level) in u64 words (rows start on 8-byte boundaries: stride is
+// a multiple of 4).
+constexpr uint32_t kScatterLanes = 8;
 __global__ void scatter_items(const uint64_t *hdr, const uint16_t *stack, uint32_t n_items, uint32_t stride,
                               unsigned int *offset, uint64_t *out_hdr, uint16_t *out_stack) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items) return;
-    const uint64_t h = hdr[i];
-    const uint32_t o = atomicAdd(&offset[hdr_puzzle(h)], 1u);
-    out_hdr[o] = h;
-    const uint16_t *src = stack + (size_t)i * stride;
-    uint16_t *dst = out_stack + (size_t)o * stride;
-    for (uint32_t j = 0; j < stride; ++j) dst[j] = src[j];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = t / kScatterLanes, sub = t % kScatterLanes;
+    const bool on = i < n_items;
+    const uint64_t h = on ? hdr[i] : 0ull;
+    uint32_t o = 0;
+    if (on && sub == 0) {
+        o = atomicAdd(&offset[hdr_puzzle(h)], 1u);
+        out_hdr[o] = h;
+    }
+    o = __shfl(o, 0, kScatterLanes);
+    if (!on) return;
+    const uint32_t words = min((hdr_level(h) + 3) / 4, stride / 4);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(stack + (size_t)i * stride);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(out_stack + (size_t)o * stride);
+    for (uint32_t j = sub; j < words; j += kScatterLanes) dst[j] = src[j];
 }
 
 template <int W>
@@ -1282,13 +1299,33 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     for (uint32_t i = 0; i < n_tab; ++i) natural = std::max(natural, natural_row_words(*puzzles[i]));
     if (natural > 16) throw Unsupported("candidate bitfields wider than 1024 bits");
     const uint32_t row = round_words(natural);
-    std::vector<EnumTable> tables;
-    tables.reserve(n_tab);
+    // Large batches build their tables on several host threads; an error is
+    // reported for the first puzzle (in batch order) that has one.
+    std::vector<EnumTable> tables(n_tab);
+    {
+        std::vector<std::exception_ptr> errors(n_tab);
+        auto build = [&](uint32_t first, uint32_t step) {
+            for (uint32_t i = first; i < n_tab; i += step) {
+                try {
+                    tables[i] = build_enum_table(*puzzles[i], cfg.order, row);
+                } catch (...) {
+                    errors[i] = std::current_exception();
+                }
+            }
+        };
+        const uint32_t n_threads =
+            n_tab < 64 ? 1u : std::max(1u, std::min<uint32_t>(kTableThreads, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (uint32_t t = 1; t < n_threads; ++t) pool.emplace_back(build, t, n_threads);
+        build(0, n_threads);
+        for (std::thread &th : pool) th.join();
+        for (const std::exception_ptr &e : errors)
+            if (e) std::rethrow_exception(e);
+    }
     std::vector<uint32_t> table_off(n_tab);
     uint32_t total_words = 0, max_avail = 1, max_P = 0, max_table = 0;
     for (uint32_t i = 0; i < n_tab; ++i) {
-        tables.push_back(build_enum_table(*puzzles[i], cfg.order, row));
-        EnumTableHeader h = tables.back().header();
+        EnumTableHeader h = tables[i].header();
         table_off[i] = total_words;
         total_words += h.total_words;
         max_avail = std::max(max_avail, h.avail_words);
@@ -1666,7 +1703,8 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             hipLaunchKernelGGL(fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, d_group, n, 0u);
             hipLaunchKernelGGL(items_per_puzzle, dim3(g), dim3(256), 0, stream, d_hdr[cur], ni, d_group);
             hipLaunchKernelGGL(exclusive_scan_u32, dim3(1), dim3(1024), 0, stream, d_group, n);
-            hipLaunchKernelGGL(scatter_items, dim3(g), dim3(256), 0, stream, d_hdr[cur], d_stack[cur], ni, stride,
+            hipLaunchKernelGGL(scatter_items, dim3((uint32_t)(((uint64_t)ni * kScatterLanes + 255) / 256)), dim3(256), 0,
+                               stream, d_hdr[cur], d_stack[cur], ni, stride,
                                d_group, d_group_hdr, d_group_stack);
             EMP_CHECK(hipGetLastError());
             EMP_CHECK(hipMemcpyAsync(d_hdr[cur], d_group_hdr, (size_t)ni * 8, hipMemcpyDeviceToDevice, stream));
