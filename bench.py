@@ -8,8 +8,9 @@ Default line (the metric's plain leg, with the batched leg beside it):
   enumeration on the GPU (tables built and uploaded by the call, breadth
   expansions, then one persistent DFS launch).
   batched: one step = 4096 synthetic puzzles of side 5..10, 3..5 border and
-  4..12 interior colours in one enumeration (DFS passes replayed from
-  hipGraphs, items regrouped by puzzle before each pass), each stopped near
+  4..12 interior colours in one enumeration (tables built on host threads,
+  DFS passes replayed from hipGraphs, items regrouped by puzzle before each
+  pass and taken by each XCD from its own region in per-wave runs), each stopped near
   2e7 nodes (the cells with more than 10^9 expected solutions cannot be
   enumerated by anyone; the count of complete and cut instances is
   reported).
