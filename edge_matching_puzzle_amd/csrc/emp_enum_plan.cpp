@@ -127,10 +127,13 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
             }
         }
     }
-    const uint32_t abase_of[3] = {0, h.words[0], h.words[0] + h.words[1]};
+    // Offsets of the kinds' words in a lane's available-transition region:
+    // corners first, interior last, so that reads of `row` words at any
+    // kind's offset stay within words[2] + words[1] + row words.
+    const uint32_t abase_of[3] = {h.words[2] + h.words[1], h.words[2], 0};
     if (h.avail_words * 16 > 4096) throw std::domain_error("identical-piece chains index at most 4096 piece slots");
     std::vector<uint32_t> pred_list;
-    for (int k = 2; k >= 0; --k)
+    for (int k = 0; k < 3; ++k)   // kinds by decreasing offset: highest slot first
         for (int ki = (int)n_kind[k] - 1; ki >= 0; --ki)
             if (pred[k][ki] >= 0)
                 pred_list.push_back((abase_of[k] * 16 + (uint32_t)ki) | (abase_of[k] * 16 + (uint32_t)pred[k][ki]) << 16);
@@ -257,7 +260,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
             rec_out[1 + n] = stk_off | col_off << 16 | (cm_block * 8) << 32;
             ++n;
         }
-        const uint64_t abase = k == 0 ? 0 : (k == 1 ? h.words[0] : h.words[0] + h.words[1]);
+        const uint64_t abase = abase_of[k];
         const uint64_t uni = h.off_dup[k] + (uint64_t)h.ndup[k] * R;
         rec_out[0] = uni << 32 | abase << 8 | (uint64_t)k << 3 | n;
     }

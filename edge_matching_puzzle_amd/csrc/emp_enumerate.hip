@@ -284,7 +284,7 @@ struct TableView {
     }
     // natural words of kind K (per-lane available-transition layout)
     __device__ __forceinline__ uint32_t nat(int k) const { return sel3(k, n0, n1, n2); }
-    __device__ __forceinline__ uint32_t abase(int k) const { return sel3(k, 0u, n0, n0 + n1); }
+    __device__ __forceinline__ uint32_t abase(int k) const { return sel3(k, n2 + n1, n2, 0u); }
     __device__ __forceinline__ uint32_t ndup(int k) const { return sel3(k, d0, d1, d2); }
     __device__ __forceinline__ uint32_t off_cm(int k) const { return sel3(k, cm0, cm1, cm2); }
     __device__ __forceinline__ uint32_t off_valid(int k) const { return sel3(k, va0, va1, va2); }
@@ -612,8 +612,9 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     for (int w = 0; w < W; ++w) c[w] = 0;
 
     // Available word w of kind K.  Words past the kind's own read the next
-    // kind's words or the region's padding of W - 1 words; the base mask
-    // zeroes them.
+    // kind's words or, past the interior kind's (the last), the rest of the
+    // region (sized for W words from the interior kind's offset); the base
+    // mask zeroes them.
     auto A = [&](uint32_t ab, uint32_t w) -> uint64_t & { return avail[(ab + w) * kWave + lane]; };
     auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
     auto finish = [&]() {
@@ -1323,12 +1324,15 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             if (e) std::rethrow_exception(e);
     }
     std::vector<uint32_t> table_off(n_tab);
-    uint32_t total_words = 0, max_avail = 1, max_P = 0, max_table = 0;
+    // max_reach: one past the last word a read of W words at a kind's offset
+    // touches (the interior kind's words come last in the region)
+    uint32_t total_words = 0, max_avail = 1, max_P = 0, max_table = 0, max_reach = 0;
     for (uint32_t i = 0; i < n_tab; ++i) {
         EnumTableHeader h = tables[i].header();
         table_off[i] = total_words;
         total_words += h.total_words;
         max_avail = std::max(max_avail, h.avail_words);
+        max_reach = std::max(max_reach, h.words[2] + h.words[1] + row);
         max_P = std::max(max_P, h.P);
         max_table = std::max(max_table, h.total_words);
     }
@@ -1374,9 +1378,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // SIMD lane, 4 SIMDs per CU) and LDS (the table once per block plus each
     // wave's stacks) allow; the block size that fits the most waves wins.
     const bool narrow = row <= 4;   // every kind has at most 256 transitions: u8 stack
-    // available words of the kinds plus W - 1 words of padding for reads
-    // past the last kind's own
-    const uint32_t avail_region = max_avail + (uint32_t)maxw - 1;
+    // available words of the kinds, and room for reads of W words at the
+    // interior kind's offset when it has fewer words than W
+    const uint32_t avail_region = std::max(max_avail, max_reach);
     const size_t per_wave_bytes = (size_t)avail_region * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
     // One table goes to LDS when it leaves room for four waves per block.
     const bool lds_table = n_tab == 1 && (size_t)max_table * 8 + 4 * per_wave_bytes <= dev.lds_per_block;
