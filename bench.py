@@ -30,7 +30,10 @@ Prints one JSON line:
   cpu_baseline     the CPU oracle (single thread) on the first nodes of the
                    same DFS.
   batched          the same fields for the batched leg (cpu_baseline: the
-                   oracle in a pool of host processes, one instance each).
+                   oracle in a pool of host processes, one instance each);
+                   its launch is picked by the autotuner on the batch itself
+                   before timing (tuned, autotune_trials; --no-autotune: the
+                   default launch).
   learned          the learned-ordering leg at a reduced size: a scorer
                    trained on 8 seeds, then the mean max depth on 4 held-out
                    16x16 seeds in equal device time (300 ms) for the plain,
@@ -114,18 +117,38 @@ def _roofline(nodes, kernel_ms, work, launches, peaks=None):
     return r
 
 
-def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LIMIT):
-    """The batched leg: `instances` uneven puzzles in one enumeration."""
+# Launch space the batched leg is tuned over (the middle of each axis is the
+# library's default launch, where the coordinate search starts).
+BATCHED_SPACE = {
+    "waves_per_cu": [12, 16],
+    "waves_per_block": [2, 4],
+    "steps_per_round": [16, 32, 64],
+    "target_per_lane": [2, 4, 8],
+    "step_budget": [1 << 14, 1 << 16, 1 << 18],
+}
+
+
+def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LIMIT, tune=True):
+    """The batched leg: `instances` uneven puzzles in one enumeration, its
+    launch picked by the autotuner on the workload itself (before timing)."""
     from edge_matching_puzzle_amd.workloads import c2_puzzles
     puzzles = c2_puzzles(instances)
+    tuned, trials = {}, None
+    if tune:
+        from edge_matching_puzzle_amd import autotune
+        best, history = autotune.tune(ctx, puzzles, limit, space=BATCHED_SPACE, budget_s=30.0, max_solutions=4,
+                                      use_graph=True, log=lambda m: print(m, file=sys.stderr, flush=True))
+        tuned = autotune.launch_options(ctx, best)
+        trials = [dict(s, nodes_per_s=r) for s, r in history]
+        print("batched autotune best %s" % json.dumps(best, sort_keys=True), file=sys.stderr, flush=True)
     for i in range(warmup):
-        ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True)
+        ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True, **tuned)
         print("batched warmup step %d done" % i, file=sys.stderr, flush=True)
     nodes = launches = 0
     kernel_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
-        res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True)
+        res, st = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True, **tuned)
         step_nodes = sum(r["n_nodes"] for r in res)
         nodes += step_nodes
         kernel_ms += st["dfs_kernel_ms"]
@@ -137,7 +160,7 @@ def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LI
     times = sorted(r["solve_ms"] for r in done)
     pct = lambda q: times[min(len(times) - 1, int(q * len(times)))] if times else None
     # algorithmic work of the same batch (instrumented run, not timed)
-    wres, wst = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True, count_work=True)
+    wres, wst = ctx.enumerate(puzzles, max_solutions=4, node_limit=limit, use_graph=True, count_work=True, **tuned)
     work = (sum(r["n_nodes"] for r in wres), wst["work_bytes"], wst["work_ops"])
     return {
         "value": nodes / elapsed, "unit": "nodes/s", "steps": steps, "warmup": warmup,
@@ -148,7 +171,8 @@ def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LI
                    "completed": len(done), "cut_at_node_limit": len(res) - len(done),
                    "solve_ms_p50": pct(0.5), "solve_ms_p99": pct(0.99), "nodes_per_step": nodes // steps,
                    "dfs_launches_per_step": launches // steps, "kernel_ms_per_step": kernel_ms / steps,
-                   "group_ms_per_step": st["group_ms"]},
+                   "group_ms_per_step": st["group_ms"], "launch": {"blocks": st["blocks"], "threads": st["threads"]},
+                   "tuned": tuned or None, "autotune_trials": trials},
         "roofline": _roofline(nodes, kernel_ms, work, launches),
         "cpu_baseline": cpu,
     }
@@ -156,7 +180,8 @@ def bench_batched(ctx, steps, warmup, cpu=None, instances=4096, limit=C2_NODE_LI
 
 def bench_c2(args, ctx):
     """The batched workload alone, as a full line."""
-    b = bench_batched(ctx, args.steps, args.warmup, getattr(args, "cpu_c2", None), args.instances, args.node_limit)
+    b = bench_batched(ctx, args.steps, args.warmup, getattr(args, "cpu_c2", None), args.instances, args.node_limit,
+                      tune=not args.no_autotune)
     return {"metric": METRIC, "value": b["value"], "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": b["config"],
@@ -343,6 +368,7 @@ def main():
     ap.add_argument("--cpu-nodes", type=int, default=600_000_000,
                     help="nodes of the CPU baseline sample (about 10 s on one core)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batched leg: default launch, no tuning before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported" % args.gpus,
@@ -427,7 +453,7 @@ def main():
     }
     if args.batched_steps:
         line["batched"] = bench_batched(ctx, args.batched_steps, 1, getattr(args, "cpu_c2", None),
-                                        args.instances, args.node_limit)
+                                        args.instances, args.node_limit, tune=not args.no_autotune)
     if args.learned_ms > 0:
         line["learned"] = bench_learned(ctx, args.learned_ms)
     # The same two peaks measured on this device by stream kernels (after the
