@@ -122,6 +122,7 @@ SIGNATURES = {
     "emp_hip_backtrack_variables": (_I32, [_P, _U32, _P]),
     "emp_hip_glutton_wide": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P, _U64, _P, _P, _P, _P, _P]),
     "emp_hip_situation_profile": (_I32, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "emp_hip_profile": (_I32, [_P, _P, _U64, _P, _P, _U64, _P, _P, _P, _P]),
     "emp_hip_enumerate_ranked": (_I32, [_P, _P, _U32, _P, _P, _P, _P, _P]),
     "emp_hip_probe": (_I32, [_P, _P, _P, _U32, _U32, _P, _P, _P]),
     "emp_hip_stream_copy": (_I32, [_P, _U64, _P]),
@@ -501,6 +502,28 @@ class Context:
         L = level.value
         return {"level": L, "glutton": glutton, "surface": surface[:L + 1].copy(),
                 "row_index": row_index[:L + 1].copy(), "n_variables": nvar.value}
+
+    PROFILE_STATS = ("low_min", "low_max", "high_min", "high_max", "total_min", "total_max")
+
+    def profile(self, width, height, colours, max_situations=1 << 24, solution_capacity=1 << 16):
+        """The profile feature's search: stats [6, 3, P + 1] (PROFILE_STATS;
+        series every situation / leading to a solution / the others; by
+        level), the solutions (boards, ordered), level_sizes [P + 1] and the
+        searches' starting situations."""
+        colours = np.ascontiguousarray(colours, dtype=np.uint32)
+        pz = Puzzle(width, height, colours.ctypes.data)
+        P = width * height
+        stats = np.zeros((6, 3, P + 1), dtype=np.uint32)
+        sols = np.zeros((solution_capacity, P), dtype=np.uint16)
+        sizes = np.zeros(P + 1, dtype=np.uint64)
+        starts = np.zeros((6, P), dtype=np.uint16)
+        n_sol, n_starts = ctypes.c_uint64(), ctypes.c_uint32()
+        _check(lib().emp_hip_profile(self._ctx, ctypes.byref(pz), max_situations, _ptr(stats), _ptr(sols),
+                                     solution_capacity, ctypes.byref(n_sol), _ptr(sizes), _ptr(starts),
+                                     ctypes.byref(n_starts)))
+        n = n_sol.value
+        return {"stats": stats, "solutions": sols[:min(n, solution_capacity)].copy(), "n_solutions": n,
+                "level_sizes": sizes, "starts": starts[:n_starts.value].copy()}
 
     def backtrack_trace(self, width, height, colours, order="basic", max_selections=0, trace_capacity=1 << 16):
         """The reference's CUDA backtracker: first solution and the selection

@@ -782,6 +782,38 @@ int emp_hip_situation_profile(emp_hip_context *context, const emp_hip_puzzle *pu
     });
 }
 
+int emp_hip_profile(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint64_t max_situations, uint32_t *stats,
+                    uint16_t *solutions, uint64_t solution_capacity, uint64_t *n_solutions, uint64_t *level_sizes,
+                    uint16_t *starts, uint32_t *n_starts) {
+    return guarded([&] {
+        if (!context || !puzzle || !stats || !n_solutions || (solution_capacity && !solutions))
+            return fail(EMP_HIP_BAD_ARGUMENT, "null argument");
+        if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
+        std::unique_ptr<emp::Puzzle> pz;
+        try {
+            pz.reset(new emp::Puzzle(puzzle->width, puzzle->height, puzzle->colours));
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_PUZZLE, e.what());
+        }
+        emp::SearchOutcome out;
+        try {
+            emp::profile_search_on_device(context->stream, context->workspace, *pz, max_situations, out);
+        } catch (const std::invalid_argument &e) {
+            return fail(EMP_HIP_BAD_ARGUMENT, e.what());
+        }
+        const uint32_t P = pz->size();
+        std::copy(out.stats.begin(), out.stats.end(), stats);
+        *n_solutions = out.n_solutions;
+        const uint64_t kept = std::min<uint64_t>(out.n_solutions, solution_capacity);
+        std::copy(out.solutions.begin(), out.solutions.begin() + kept * P, solutions);
+        if (level_sizes) std::copy(out.level_sizes.begin(), out.level_sizes.end(), level_sizes);
+        if (n_starts) *n_starts = (uint32_t)out.starts.size();
+        if (starts)
+            for (size_t i = 0; i < out.starts.size(); ++i) std::copy(out.starts[i].begin(), out.starts[i].end(), starts + i * P);
+        return EMP_HIP_OK;
+    });
+}
+
 int emp_hip_glutton_wide(emp_hip_context *context, const emp_hip_puzzle *puzzle, const uint32_t *parents,
                          uint32_t parent_len, uint32_t n_parents, const uint32_t *cells, uint64_t *parent_masks,
                          uint32_t *parent_status, uint64_t capacity, uint32_t *child_parent, uint32_t *child_move,

@@ -34,6 +34,22 @@ struct ProfileOutcome {
 void situation_profile_on_device(hipStream_t stream, Workspace &ws, const Puzzle &puzzle, const uint16_t *board,
                                  ProfileOutcome &out);
 
+struct SearchOutcome {
+    std::vector<uint32_t> stats;        // [6][3][P + 1]: low min/max, high min/max, total min/max; series
+                                        // all / leading to a solution / others; by level
+    std::vector<uint16_t> solutions;    // [n_solutions][P] row-major boards, in board order
+    uint64_t n_solutions = 0;
+    std::vector<uint64_t> level_sizes;  // [P + 1] situations of each level (first pass, all starts)
+    std::vector<std::vector<uint16_t>> starts;   // the searches' starting situations
+    uint64_t launches = 0;
+    double kernel_ms = 0;               // search_children_kernel time (HIP events)
+};
+
+// The profile feature's search (include/feature_profile.h:582-868) on the
+// device.  Throws Unsupported when a level holds more than max_situations.
+void profile_search_on_device(hipStream_t stream, Workspace &ws, const Puzzle &puzzle, uint64_t max_situations,
+                              SearchOutcome &out);
+
 }  // namespace emp
 
 #endif

@@ -122,6 +122,231 @@ __global__ void __launch_bounds__(kProfThreads) profile_kernel(ProfArgs a) {
     for (uint32_t i = tid; i < 2 * P; i += kProfThreads) a.out[(size_t)blockIdx.x * 2 * P + i] = val[i];
 }
 
+// ---------------------------------------------------------------------------
+// Profile search (feature_profile, include/feature_profile.h:582-868): level
+// by level, every live variable of every situation of a level is placed; a
+// child whose profile is valid joins the next level (each situation once)
+// and its profile's min, max and total update that level's extrema.
+//
+//   search_children_kernel  one block per parent: the parent's capability is
+//                           built in LDS from the level-0 one (one transition
+//                           per placed cell), then each thread takes capability
+//                           words and, for each live variable in them, counts
+//                           the child's 2P infos without storing the child's
+//                           capability (a child word is the parent's word
+//                           through the variable's transition); extrema are
+//                           reduced in LDS, then one global atomic per block
+//                           and value; valid children's boards are appended.
+//   search_dedup_kernel     one thread per appended board: an open-addressing
+//                           table of board indices (CAS on empty slots, full
+//                           board compare on a hit, linear probing) keeps the
+//                           first copy of each situation for the next level.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSearchThreads = 256;
+constexpr uint16_t kEmptyCell = 0xFFFF;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kStatValues = 6;        // low min, low max, high min, high max, total min, total max
+constexpr uint32_t kMaxPrecedes = 1024;    // solutions a parent precedes, listed in LDS
+
+struct SearchArgs {
+    uint32_t P;
+    const uint64_t *cap0;       // [P][16] level-0 capability
+    const uint64_t *cm;         // [(colour * 4 + side) * 16]
+    const uint32_t *piece_col;  // N | E << 8 | S << 16 | W << 24
+    const int32_t *nb;          // [P][4] neighbour cell per side, -1 off the board
+    const uint16_t *parents;    // [n][P] row-major boards, ((piece << 2) | o), 0xFFFF empty
+    uint32_t level;             // pieces placed in every parent
+    uint16_t *cand;             // [cand_cap][P] valid children
+    uint64_t cand_cap;
+    unsigned long long *n_cand;
+    uint32_t *stats;            // [3 series][P + 1 levels][kStatValues]
+    const uint16_t *sols;       // second pass: the solutions [n_sol][P]; null in the first
+    uint32_t n_sol;
+};
+
+// Word w of position j after placing (pos, piece, o) on a capability whose
+// word it was: the transition of the system equation's variable
+// (prepare_transitions, include/system_equation_for_CUDA.h:70-134): the
+// placed position loses every variable, the piece is cleared everywhere, and
+// at each neighbour only the variables showing the placed piece's colour on
+// the shared edge stay.  nbm[s] is the neighbour on side s (-1: none), nrow[s]
+// that side's colour-mask row.
+__device__ __forceinline__ uint64_t transition_word(uint64_t v, uint32_t j, uint32_t w, uint32_t pos, uint32_t piece,
+                                                   const int32_t (&nbm)[4], const uint64_t *const (&nrow)[4]) {
+    if (j == pos) return 0;
+    if ((w & 3) == (piece >> 6)) v &= ~(1ull << (piece & 63));
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s)
+        if (nbm[s] == (int32_t)j) v &= nrow[s][w];
+    return v;
+}
+
+__device__ __forceinline__ void placement_rows(const SearchArgs &a, uint32_t pos, uint32_t piece, uint32_t o,
+                                               int32_t (&nbm)[4], const uint64_t *(&nrow)[4]) {
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s) {
+        nbm[s] = a.nb[pos * 4 + s];
+        const uint32_t col = (a.piece_col[piece] >> (8 * ((s + o) & 3))) & 0xFF;
+        nrow[s] = a.cm + ((size_t)col * 4 + ((s + 2) & 3)) * kCapWords;
+    }
+}
+
+__global__ void __launch_bounds__(kSearchThreads) search_children_kernel(SearchArgs a) {
+    __shared__ uint64_t cap[256 * kCapWords];
+    __shared__ uint16_t board[256];
+    __shared__ uint32_t st[3 * kStatValues];
+    __shared__ uint32_t precedes[kMaxPrecedes];
+    __shared__ uint32_t n_precedes;
+    const uint32_t P = a.P, tid = threadIdx.x, n_words = P * kCapWords, L1 = a.level + 1;
+    const uint16_t *pb = a.parents + (size_t)blockIdx.x * P;
+    for (uint32_t t = tid; t < P; t += kSearchThreads) board[t] = pb[t];
+    for (uint32_t t = tid; t < n_words; t += kSearchThreads) cap[t] = a.cap0[t];
+    if (tid < 3 * kStatValues) st[tid] = (tid % 2 == 0) ? 0xFFFFFFFFu : 0u;
+    if (tid == 0) n_precedes = 0;
+    __syncthreads();
+    // the parent's capability: one transition per placed cell
+    for (uint32_t cell = 0; cell < P; ++cell) {
+        const uint32_t bv = board[cell];
+        if (bv == kEmptyCell) continue;
+        int32_t nbm[4];
+        const uint64_t *nrow[4];
+        placement_rows(a, cell, bv >> 2, bv & 3, nbm, nrow);
+        for (uint32_t t = tid; t < n_words; t += kSearchThreads)
+            cap[t] = transition_word(cap[t], t / kCapWords, t % kCapWords, cell, bv >> 2, nbm, nrow);
+        __syncthreads();
+    }
+    // second pass: the solutions the parent precedes (its placed cells equal theirs)
+    bool scan_all = false;
+    if (a.sols) {
+        for (uint32_t s = tid; s < a.n_sol; s += kSearchThreads) {
+            const uint16_t *sb = a.sols + (size_t)s * P;
+            bool pre = true;
+            for (uint32_t c = 0; c < P && pre; ++c) pre = board[c] == kEmptyCell || board[c] == sb[c];
+            if (pre) {
+                const uint32_t i = atomicAdd(&n_precedes, 1u);
+                if (i < kMaxPrecedes) precedes[i] = s;
+            }
+        }
+        __syncthreads();
+        scan_all = n_precedes > kMaxPrecedes;
+    }
+    for (uint32_t t = tid; t < n_words; t += kSearchThreads) {
+        const uint32_t pos = t / kCapWords, w = t % kCapWords;
+        for (uint64_t bits = cap[t]; bits; bits &= bits - 1) {
+            const uint32_t piece = (w & 3) * 64 + (uint32_t)__builtin_ctzll(bits), o = w >> 2;
+            int32_t nbm[4];
+            const uint64_t *nrow[4];
+            placement_rows(a, pos, piece, o, nbm, nrow);
+            // the child's 2P counts: by position, then by piece
+            uint32_t zeros = 0, mn = 0xFFFFFFFFu, mx = 0, total = 0;
+            auto count = [&](uint32_t c) {
+                zeros += c == 0;
+                mn = c ? min(mn, c) : mn;
+                mx = max(mx, c);
+                total += c;
+            };
+            for (uint32_t j = 0; j < P; ++j) {
+                uint32_t c = 0;
+                for (uint32_t ww = 0; ww < kCapWords; ++ww)
+                    c += (uint32_t)__popcll(transition_word(cap[j * kCapWords + ww], j, ww, pos, piece, nbm, nrow));
+                count(c);
+            }
+            for (uint32_t q = 0; q < P; ++q) {
+                const uint32_t qw = q >> 6, qb = q & 63;
+                uint32_t c = 0;
+                for (uint32_t j = 0; j < P; ++j)
+#pragma unroll
+                    for (uint32_t oo = 0; oo < 4; ++oo) {
+                        const uint32_t ww = oo * 4 + qw;
+                        c += (uint32_t)(transition_word(cap[j * kCapWords + ww], j, ww, pos, piece, nbm, nrow) >> qb) & 1u;
+                    }
+                count(c);
+            }
+            // every placed position and piece has no variable left; a valid
+            // profile leaves no other empty (include/situation_profile.h:170-178)
+            if (L1 < P && zeros != 2 * L1) continue;
+            const uint32_t pmin = L1 < P ? mn : 0u;
+            const uint32_t cv = piece << 2 | o;
+            uint32_t series = 0;
+            if (a.sols) {
+                bool leads = L1 == P;
+                if (scan_all) {
+                    for (uint32_t s = 0; s < a.n_sol && !leads; ++s) {
+                        const uint16_t *sb = a.sols + (size_t)s * P;
+                        bool pre = sb[pos] == cv;
+                        for (uint32_t c = 0; c < P && pre; ++c) pre = board[c] == kEmptyCell || board[c] == sb[c];
+                        leads = pre;
+                    }
+                } else {
+                    for (uint32_t i = 0; i < n_precedes && !leads; ++i) leads = a.sols[(size_t)precedes[i] * P + pos] == cv;
+                }
+                series = leads ? 1u : 2u;
+            }
+            uint32_t *sv = st + series * kStatValues;
+            atomicMin(&sv[0], pmin);
+            atomicMax(&sv[1], pmin);
+            atomicMin(&sv[2], mx);
+            atomicMax(&sv[3], mx);
+            atomicMin(&sv[4], total);
+            atomicMax(&sv[5], total);
+            const unsigned long long slot = atomicAdd(a.n_cand, 1ull);
+            if (slot < a.cand_cap) {
+                uint16_t *dst = a.cand + slot * P;
+                for (uint32_t c = 0; c < P; ++c) dst[c] = c == pos ? (uint16_t)cv : board[c];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 3 * kStatValues) {
+        const uint32_t series = tid / kStatValues, k = tid % kStatValues;
+        const uint32_t v = st[tid];
+        uint32_t *g = a.stats + ((size_t)series * (P + 1) + L1) * kStatValues + k;
+        if (k % 2 == 0) {
+            if (v != 0xFFFFFFFFu) atomicMin(g, v);
+        } else if (st[series * kStatValues + k - 1] != 0xFFFFFFFFu) {
+            atomicMax(g, v);
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t board_hash(const uint16_t *b, uint32_t P) {
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (uint32_t c = 0; c < P; ++c) {
+        h ^= b[c];
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+    }
+    return h;
+}
+
+__global__ void search_dedup_kernel(const uint16_t *cand, uint64_t n, uint32_t P, uint32_t *table, uint32_t mask,
+                                    uint16_t *next, uint64_t next_cap, unsigned long long *n_next) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint16_t *b = cand + i * P;
+    uint32_t h = (uint32_t)board_hash(b, P) & mask;
+    // at most half the table is used, so an empty slot ends every probe
+    while (true) {
+        const uint32_t old = atomicCAS(&table[h], kNoSlot, (uint32_t)i);
+        if (old == kNoSlot) {
+            const unsigned long long o = atomicAdd(n_next, 1ull);
+            if (o < next_cap)
+                for (uint32_t c = 0; c < P; ++c) next[o * P + c] = b[c];
+            return;
+        }
+        const uint16_t *ob = cand + (uint64_t)old * P;
+        bool same = true;
+        for (uint32_t c = 0; c < P && same; ++c) same = ob[c] == b[c];
+        if (same) return;
+        h = (h + 1) & mask;
+    }
+}
+
+__global__ void fill_words(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 #define PF_CHECK(call)                                                                    \
     do {                                                                                  \
         hipError_t err_ = (call);                                                         \
@@ -345,6 +570,264 @@ void situation_profile_on_device(hipStream_t stream, Workspace &ws, const Puzzle
     }
     out.launches = dv.launches;
     out.kernel_ms = dv.kernel_ms;
+}
+
+// Profile search (feature_profile::template_run, include/feature_profile.h
+// :686-868): see the kernels above.  The starts: the empty situation, or at
+// 25 pieces the four corners placed -- corner 0 at (0, 0), the other three at
+// (W-1, 0), (0, H-1), (W-1, H-1) in the six orders of next_permutation, each
+// in its first orientation that is a variable there (the corners of a 5x5
+// board share no edge, so a corner's variables there are those of the
+// level-0 capability) -- from which the search runs the remaining 21 levels.
+// The first pass records the extrema of all situations and takes the last
+// level's set of the first start whose search leaves the solutions unset as
+// the solutions; the second runs each start that precedes a solution again,
+// splitting the extrema between situations that lead to a solution and the
+// others.  Every series also holds the level-0 profile.
+void profile_search_on_device(hipStream_t stream, Workspace &ws, const Puzzle &pz, uint64_t max_situations,
+                              SearchOutcome &out) {
+    const uint32_t P = pz.size(), W = pz.width(), H = pz.height();
+    if (P > 256) throw Unsupported("capability profiles hold at most 256 pieces");
+    if (max_situations == 0 || max_situations > (1ull << 31))
+        throw std::invalid_argument("max_situations must be between 1 and 2^31");
+    const std::vector<uint64_t> init = backtrack_capability(pz, kBasic);   // basic order: position = cell
+    std::vector<uint64_t> cm((size_t)(pz.max_colour() + 1) * 4 * kCapWords, 0);
+    std::vector<uint32_t> pcol(P);
+    std::vector<int32_t> nb((size_t)P * 4, -1);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const uint32_t i = y * W + x;
+            if (y > 0) nb[i * 4 + 0] = (int32_t)(i - W);
+            if (x + 1 < W) nb[i * 4 + 1] = (int32_t)(i + 1);
+            if (y + 1 < H) nb[i * 4 + 2] = (int32_t)(i + W);
+            if (x > 0) nb[i * 4 + 3] = (int32_t)(i - 1);
+        }
+    for (uint32_t q = 0; q < P; ++q) {
+        const Piece &p = pz.piece(q);
+        pcol[q] = p.colour[0] | p.colour[1] << 8 | p.colour[2] << 16 | p.colour[3] << 24;
+        for (unsigned o = 0; o < 4; ++o)
+            for (unsigned sd = 0; sd < 4; ++sd) {
+                const uint32_t b = 256 * o + q;
+                cm[((size_t)shown(p, sd, o) * 4 + sd) * kCapWords + b / 64] |= 1ull << (b % 64);
+            }
+    }
+    const size_t cap_words = (size_t)P * kCapWords;
+    uint64_t *d_init = ws.get<uint64_t>("ps_init", cap_words);
+    uint64_t *d_cm = ws.get<uint64_t>("ps_cm", cm.size());
+    uint32_t *d_pcol = ws.get<uint32_t>("ps_pcol", P);
+    int32_t *d_nb = ws.get<int32_t>("ps_nb", nb.size());
+    const size_t stat_words = (size_t)3 * (P + 1) * kStatValues;
+    uint32_t *d_stats = ws.get<uint32_t>("ps_stats", stat_words);
+    unsigned long long *d_counts = ws.get<unsigned long long>("ps_counts", 2);
+    PF_CHECK(hipMemcpyAsync(d_init, init.data(), cap_words * 8, hipMemcpyHostToDevice, stream));
+    PF_CHECK(hipMemcpyAsync(d_cm, cm.data(), cm.size() * 8, hipMemcpyHostToDevice, stream));
+    PF_CHECK(hipMemcpyAsync(d_pcol, pcol.data(), P * 4, hipMemcpyHostToDevice, stream));
+    PF_CHECK(hipMemcpyAsync(d_nb, nb.data(), nb.size() * 4, hipMemcpyHostToDevice, stream));
+    {
+        std::vector<uint32_t> st0(stat_words);
+        for (size_t i = 0; i < stat_words; ++i) st0[i] = (i % kStatValues) % 2 == 0 ? 0xFFFFFFFFu : 0u;
+        PF_CHECK(hipMemcpyAsync(d_stats, st0.data(), stat_words * 4, hipMemcpyHostToDevice, stream));
+    }
+    hipEvent_t e0, e1;
+    PF_CHECK(hipEventCreate(&e0));
+    PF_CHECK(hipEventCreate(&e1));
+    struct Events {
+        hipEvent_t a, b;
+        ~Events() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } events{e0, e1};
+
+    // level-0 profile: the counts of the level-0 capability, by position and by piece
+    std::vector<uint32_t> prof0(2 * P, 0);
+    for (uint32_t j = 0; j < P; ++j)
+        for (uint32_t w = 0; w < kCapWords; ++w) {
+            const uint64_t v = init[(size_t)j * kCapWords + w];
+            prof0[j] += (uint32_t)__builtin_popcountll(v);
+            for (uint64_t m = v; m; m &= m - 1) ++prof0[P + (w & 3) * 64 + (uint32_t)__builtin_ctzll(m)];
+        }
+    uint32_t p0_min = 0xFFFFFFFFu, p0_max = 0, p0_total = 0;
+    for (uint32_t c : prof0) {
+        p0_min = std::min(p0_min, c);
+        p0_max = std::max(p0_max, c);
+        p0_total += c;
+    }
+
+    // starts
+    std::vector<std::vector<uint16_t>> starts;
+    uint32_t levels = P;
+    const std::vector<uint16_t> empty(P, kEmptyCell);
+    auto variable = [&](uint32_t cell, uint32_t piece, uint32_t o) {
+        const uint32_t b = 256 * o + piece;
+        return (init[(size_t)cell * kCapWords + b / 64] >> (b % 64)) & 1;
+    };
+    if (P != 25) {
+        starts.push_back(empty);
+    } else {
+        levels = P - 4;
+        const std::vector<uint32_t> &corners = pz.pieces_of(kCorner);
+        auto set_corner = [&](std::vector<uint16_t> &b, uint32_t piece, uint32_t x, uint32_t y) {
+            for (uint32_t o = 0; o < 4; ++o)
+                if (variable(y * W + x, piece, o)) {
+                    b[y * W + x] = (uint16_t)(piece << 2 | o);
+                    break;
+                }
+        };
+        std::vector<uint16_t> c0 = empty;
+        set_corner(c0, corners[0], 0, 0);
+        const uint32_t cx[3] = {W - 1, 0, W - 1}, cy[3] = {0, H - 1, H - 1};
+        std::vector<uint32_t> others{1, 2, 3};
+        do {
+            std::vector<uint16_t> b = c0;
+            for (uint32_t i = 0; i < 3; ++i) set_corner(b, corners[others[i]], cx[i], cy[i]);
+            starts.push_back(b);
+        } while (std::next_permutation(others.begin(), others.end()));
+    }
+    out.starts = starts;
+
+    uint64_t cand_cap = 1u << 16, next_cap = std::min<uint64_t>(max_situations, 1u << 16);
+    out.level_sizes.assign(P + 1, 0);
+    out.launches = 0;
+    out.kernel_ms = 0;
+    // one breadth search from `start`; returns the last level's set
+    auto search = [&](const std::vector<uint16_t> &start, const uint16_t *d_sols, uint32_t n_sol, bool record) {
+        uint64_t n_cur = 1;
+        uint32_t level = 0;
+        for (uint32_t c : start) level += c != kEmptyCell;
+        uint16_t *d_parents = ws.get<uint16_t>("ps_parents", (size_t)P);
+        PF_CHECK(hipMemcpyAsync(d_parents, start.data(), P * 2, hipMemcpyHostToDevice, stream));
+        for (uint32_t l = 0; l < levels && n_cur; ++l, ++level) {
+            if (n_cur > 0xFFFFFFFFull) throw Unsupported("a level of the profile search holds more than 2^32 situations");
+            SearchArgs a{};
+            a.P = P;
+            a.cap0 = d_init;
+            a.cm = d_cm;
+            a.piece_col = d_pcol;
+            a.nb = d_nb;
+            a.level = level;
+            a.stats = d_stats;
+            a.sols = d_sols;
+            a.n_sol = n_sol;
+            a.n_cand = d_counts;
+            uint64_t n_cand = 0;
+            while (true) {
+                a.parents = d_parents;
+                a.cand = ws.get<uint16_t>("ps_cand", (size_t)cand_cap * P);
+                a.cand_cap = cand_cap;
+                PF_CHECK(hipMemsetAsync(d_counts, 0, 16, stream));
+                PF_CHECK(hipEventRecord(e0, stream));
+                hipLaunchKernelGGL(search_children_kernel, dim3((uint32_t)n_cur), dim3(kSearchThreads), 0, stream, a);
+                PF_CHECK(hipGetLastError());
+                PF_CHECK(hipEventRecord(e1, stream));
+                PF_CHECK(hipMemcpyAsync(&n_cand, d_counts, 8, hipMemcpyDeviceToHost, stream));
+                PF_CHECK(hipStreamSynchronize(stream));
+                float ms = 0;
+                PF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                out.kernel_ms += ms;
+                ++out.launches;
+                if (n_cand <= cand_cap) break;
+                // the children did not fit: the level runs again with room for
+                // all of them (its extrema are minima and maxima, which a second
+                // run leaves as they are; the parents keep their own buffer)
+                if (n_cand > (1ull << 31)) throw Unsupported("the profile search makes more than 2^31 children in a level");
+                cand_cap = n_cand;
+            }
+            // distinct children
+            uint32_t tbits = 1;
+            while ((1ull << tbits) < 2 * std::max<uint64_t>(n_cand, 1)) ++tbits;
+            uint32_t *d_table = ws.get<uint32_t>("ps_table", (size_t)1 << tbits);
+            hipLaunchKernelGGL(fill_words, dim3(1024), dim3(256), 0, stream, d_table, (uint64_t)1 << tbits, kNoSlot);
+            uint16_t *d_next = ws.get<uint16_t>("ps_next", (size_t)next_cap * P);
+            uint64_t n_next = 0;
+            while (true) {
+                PF_CHECK(hipMemsetAsync(d_counts + 1, 0, 8, stream));
+                if (n_cand)
+                    hipLaunchKernelGGL(search_dedup_kernel, dim3((uint32_t)((n_cand + 255) / 256)), dim3(256), 0, stream,
+                                       a.cand, n_cand, P, d_table, (uint32_t)((1ull << tbits) - 1), d_next, next_cap,
+                                       d_counts + 1);
+                PF_CHECK(hipGetLastError());
+                PF_CHECK(hipMemcpyAsync(&n_next, d_counts + 1, 8, hipMemcpyDeviceToHost, stream));
+                PF_CHECK(hipStreamSynchronize(stream));
+                ++out.launches;
+                if (n_next <= next_cap) break;
+                if (n_next > max_situations)
+                    throw Unsupported("a level of the profile search holds " + std::to_string(n_next) +
+                                      " situations, more than max_situations (" + std::to_string(max_situations) + ")");
+                next_cap = n_next;
+                d_next = ws.get<uint16_t>("ps_next", (size_t)next_cap * P);
+                hipLaunchKernelGGL(fill_words, dim3(1024), dim3(256), 0, stream, d_table, (uint64_t)1 << tbits, kNoSlot);
+            }
+            if (record) out.level_sizes[level + 1] += n_next;
+            // the next level's parents
+            d_parents = ws.get<uint16_t>("ps_parents", (size_t)std::max<uint64_t>(n_next, 1) * P);
+            PF_CHECK(hipMemcpyAsync(d_parents, d_next, (size_t)n_next * P * 2, hipMemcpyDeviceToDevice, stream));
+            n_cur = n_next;
+        }
+        std::vector<uint16_t> last((size_t)n_cur * P);
+        if (n_cur) PF_CHECK(hipMemcpyAsync(last.data(), d_parents, last.size() * 2, hipMemcpyDeviceToHost, stream));
+        PF_CHECK(hipStreamSynchronize(stream));
+        return std::make_pair(last, n_cur);
+    };
+
+    // first pass: every situation; the solutions
+    bool have_solutions = false;
+    std::vector<uint16_t> sols;
+    uint64_t n_sol = 0;
+    for (const std::vector<uint16_t> &st : starts) {
+        out.level_sizes[0] += 1;
+        auto r = search(st, nullptr, 0, true);
+        if (!have_solutions && r.second) {
+            sols = std::move(r.first);
+            n_sol = r.second;
+            have_solutions = true;
+        }
+    }
+    // the first pass wrote series 0 only; the second writes series 1 and 2
+    if (n_sol > 0xFFFFFFFFull) throw Unsupported("more than 2^32 solutions");
+    if (n_sol) {
+        uint16_t *d_sols = ws.get<uint16_t>("ps_sols", (size_t)n_sol * P);
+        PF_CHECK(hipMemcpyAsync(d_sols, sols.data(), sols.size() * 2, hipMemcpyHostToDevice, stream));
+        for (const std::vector<uint16_t> &st : starts) {
+            bool precedes = false;
+            uint32_t lv = 0;
+            for (uint16_t c : st) lv += c != kEmptyCell;
+            for (uint64_t k = 0; k < n_sol && !precedes && lv < P; ++k) {
+                bool pre = true;
+                for (uint32_t c = 0; c < P && pre; ++c) pre = st[c] == kEmptyCell || st[c] == sols[k * P + c];
+                precedes = pre;
+            }
+            if (precedes) (void)search(st, d_sols, (uint32_t)n_sol, false);
+        }
+    }
+    std::vector<uint32_t> dst(stat_words);
+    PF_CHECK(hipMemcpyAsync(dst.data(), d_stats, stat_words * 4, hipMemcpyDeviceToHost, stream));
+    PF_CHECK(hipStreamSynchronize(stream));
+    // [series][level][value] -> [value][series][level], level 0 from the
+    // level-0 profile, minima never updated -> 0 (simple_stat::get_min)
+    out.stats.assign((size_t)kStatValues * 3 * (P + 1), 0);
+    for (uint32_t series = 0; series < 3; ++series)
+        for (uint32_t l = 0; l <= P; ++l)
+            for (uint32_t k = 0; k < kStatValues; ++k) {
+                uint32_t v = dst[((size_t)series * (P + 1) + l) * kStatValues + k];
+                if (l == 0) {
+                    const uint32_t p = k < 2 ? p0_min : (k < 4 ? p0_max : p0_total);
+                    v = k % 2 == 0 ? std::min(v, p) : std::max(v, p);
+                }
+                if (k % 2 == 0 && v == 0xFFFFFFFFu) v = 0;
+                out.stats[((size_t)k * 3 + series) * (P + 1) + l] = v;
+            }
+    // solutions, in board order (the reference keeps them in a set)
+    std::vector<size_t> order((size_t)n_sol);
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return std::lexicographical_compare(sols.begin() + x * P, sols.begin() + (x + 1) * P, sols.begin() + y * P,
+                                            sols.begin() + (y + 1) * P);
+    });
+    out.solutions.resize((size_t)n_sol * P);
+    for (size_t i = 0; i < order.size(); ++i)
+        std::copy(sols.begin() + order[i] * P, sols.begin() + (order[i] + 1) * P, out.solutions.begin() + i * P);
+    out.n_solutions = n_sol;
 }
 
 }  // namespace emp

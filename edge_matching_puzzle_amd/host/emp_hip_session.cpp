@@ -47,4 +47,16 @@ std::string situation_string(const puzzle_description &p, const uint16_t *board)
     return std::string(text.data());
 }
 
+void write_vtk_table(const std::string &file_name, const std::string &type, const std::string &header,
+                     const std::vector<std::string> &names, const std::vector<std::vector<uint32_t>> &series) {
+    std::ofstream f(file_name);
+    if (!f) throw logic_exception("Unable to create VTK " + type + " file \"" + file_name + "\"", __LINE__, __FILE__);
+    f << type << '\n' << header << '\n';
+    for (size_t i = 0; i < names.size(); ++i) f << names[i] << (i + 1 < names.size() ? " " : "\n");
+    for (const auto &ser : series) {
+        for (uint32_t v : ser) f << v << ' ';
+        f << '\n';
+    }
+}
+
 }  // namespace edge_matching_puzzle

@@ -57,6 +57,12 @@ class device_session {
 
 std::string situation_string(const puzzle_description &p, const uint16_t *board);
 
+// VTK table files (include/VTK_table_based_dumper.h): type, then title, axis
+// names, series length and count, then the series names, then one line per
+// series, every value followed by a space.
+void write_vtk_table(const std::string &file_name, const std::string &type, const std::string &header,
+                     const std::vector<std::string> &names, const std::vector<std::vector<uint32_t>> &series);
+
 }  // namespace edge_matching_puzzle
 
 #endif

@@ -4,6 +4,7 @@
 #include "feature_CUDA_glutton_max.h"
 #include "feature_CUDA_glutton_wide.h"
 #include "feature_new_strategy.h"
+#include "feature_profile.h"
 #include "feature_situation_profile.h"
 
 namespace edge_matching_puzzle {
@@ -27,6 +28,7 @@ std::unique_ptr<feature_if> factory_feature::create_feature(const std::string &p
     if (!p_initial_situation.empty())
         throw logic_exception("--initial_situation is only used by new_strategy and situation_profile", __LINE__,
                               __FILE__);
+    if (p_name == "profile") return std::unique_ptr<feature_if>(new feature_profile(p_puzzle));
     if (p_name == "CUDA_backtracker")
         return std::unique_ptr<feature_if>(new feature_CUDA_backtracker(p_puzzle, p_strategy));
     if (p_name == "CUDA_enumerate")

@@ -10,21 +10,6 @@ namespace edge_matching_puzzle {
 
 namespace {
 
-// VTK table files (include/VTK_table_based_dumper.h): type, then title, axis
-// names, series length and count, then the series names, then one line per
-// series, every value followed by a space.
-void write_table(const std::string &file_name, const std::string &type, const std::string &header,
-                 const std::vector<std::string> &names, const std::vector<std::vector<uint32_t>> &series) {
-    std::ofstream f(file_name);
-    if (!f) throw logic_exception("Unable to create VTK " + type + " file \"" + file_name + "\"", __LINE__, __FILE__);
-    f << type << '\n' << header << '\n';
-    for (size_t i = 0; i < names.size(); ++i) f << names[i] << (i + 1 < names.size() ? " " : "\n");
-    for (const auto &s : series) {
-        for (uint32_t v : s) f << v << ' ';
-        f << '\n';
-    }
-}
-
 std::string capitalised(const std::string &s) { return std::string(1, (char)std::toupper(s[0])) + s.substr(1); }
 
 }  // namespace
@@ -66,11 +51,11 @@ void feature_situation_profile::run() {
             names.push_back(capitalised(crit) + "_" + mm);
             all.push_back(s);
             const std::string title = "Level_" + crit + "_" + mm + "_glutton";
-            write_table(root + "level_" + crit + "_" + mm + "_glutton.txt", "line_plot",
+            write_vtk_table(root + "level_" + crit + "_" + mm + "_glutton.txt", "line_plot",
                         title + " Step " + capitalised(crit) + " " + std::to_string(P + 1) + " 1",
                         {capitalised(mm)}, {s});
         }
-    write_table(root + "level_glutton.txt", "line_plot", "Level_glutton Step Total " + std::to_string(P + 1) + " 10",
+    write_vtk_table(root + "level_glutton.txt", "line_plot", "Level_glutton Step Total " + std::to_string(P + 1) + " 10",
                 names, all);
 
     // treat_situation_capability: each profile on stdout (3 wide), in the
@@ -96,7 +81,7 @@ void feature_situation_profile::run() {
             f << '\n';
         }
     }
-    write_table(root + "line_plot_vtk.txt", "line_plot",
+    write_vtk_table(root + "line_plot_vtk.txt", "line_plot",
                 "Profile_evolution Step Possibilities " + std::to_string(2 * P) + " " + std::to_string(level + 1),
                 level_names, lines);
 }

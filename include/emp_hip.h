@@ -338,6 +338,33 @@ int emp_hip_situation_profile(emp_hip_context *context, const emp_hip_puzzle *pu
                               uint32_t *n_variables);
 
 /*
+ * The profile feature's search: level by level, every situation the system
+ * equation's capability allows (pieces placed at any free position, in any
+ * order; a situation joins its level when every free position and every
+ * unplaced piece keeps a variable, and only once), with per-level extrema of
+ * the capability profiles (the 2P bit counts: "low" = the smallest count of a
+ * free position or piece, "high" = the largest, "total" = their sum).
+ *   stats        [6][3][P + 1] u32: low min, low max, high min, high max,
+ *                total min, total max; series: every situation, those that
+ *                precede a solution (and full boards), the others; by level.
+ *                A minimum never updated is 0, as the reference reports it.
+ *   solutions    [solution_capacity][P] the solutions (row-major boards, in
+ *                board order); *n_solutions is their full count
+ *   level_sizes  [P + 1] situations of each level (may be NULL)
+ *   starts       [6][P] the searches' starting situations (may be NULL;
+ *                *n_starts of them: the empty situation, or at 25 pieces the
+ *                six corner placements)
+ * EMP_HIP_UNSUPPORTED when a level holds more than max_situations (at most
+ * 2^31) or above 256 pieces.
+ * Replaces: the computation of feature_profile::template_run and wide
+ * (include/feature_profile.h:582-868); the host feature writes the six VTK
+ * histograms and solutions.txt.
+ */
+int emp_hip_profile(emp_hip_context *context, const emp_hip_puzzle *puzzle, uint64_t max_situations, uint32_t *stats,
+                    uint16_t *solutions, uint64_t solution_capacity, uint64_t *n_solutions, uint64_t *level_sizes,
+                    uint16_t *starts, uint32_t *n_starts);
+
+/*
  * Breadth expansion: every situation the enumeration visits with `depth`
  * pieces placed (indices 0..depth-1 of the order), level by level on the GPU.
  * boards: [capacity][P] row-major (cells beyond the depth empty); *n_situations

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -1347,5 +1348,176 @@ int oracle_situation_profile(uint32_t width, uint32_t height, const uint32_t *co
     const std::vector<uint32_t> pf = pr.profile(live);
     std::copy(pf.begin(), pf.end(), surface + placed.size() * 2 * P);
     row_index[placed.size()] = P;
+    return ORACLE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// profile: feature_profile::template_run and wide (include/feature_profile.h
+// :582-868).  A level-by-level search over every situation the capability
+// allows, placements in any order: from each situation of a level, every
+// live variable (a piece, orientation at a free position) is placed; a child
+// whose profile is valid joins the next level's set (include/situation_
+// profile.h:170-178: no free position or piece left without a variable) and
+// updates the per-level extrema of its profile's min, max and total.  The
+// first pass records them over all situations and takes the last level's
+// set as the solutions; the second splits them between situations that lead
+// to a solution (predecessors, include/emp_situation.h:567-584, and full
+// boards) and the others.  At 25 pieces the four corners are placed first:
+// corner 0 at (0, 0), the other three at the other corners in the six orders
+// of next_permutation, each in its first orientation that is a variable
+// there (:686-764), and the search runs the remaining 21 levels from each;
+// the solutions are then the last level's set of the first start that
+// leaves the recorded set empty before it (:766-776).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ExtremumStat {   // simple_stat (include/feature_profile.h:201-237)
+    bool updated = false;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    void update(uint32_t v) {
+        updated = true;
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+    }
+    uint32_t get_min() const { return updated ? mn : 0u; }
+    uint32_t get_max() const { return mx; }
+};
+
+struct LevelStats {     // profile_level_stats: low = profile min, high = profile max
+    ExtremumStat low, high, total;
+};
+
+using Board = std::vector<uint16_t>;
+
+struct ProfileSearch {
+    const Model &m;
+    const Profiler &pr;
+    uint64_t max_situations;
+    bool too_large = false;
+
+    std::vector<char> rebuild(const Board &b) const {
+        std::vector<char> live(pr.vars.size(), 1);
+        for (uint32_t cell = 0; cell < m.P; ++cell)
+            if (b[cell] != 0xFFFF) {
+                const int32_t v = pr.var_at[((size_t)cell * m.P + (b[cell] >> 2)) * 4 + (b[cell] & 3)];
+                pr.apply(live, (uint32_t)v);
+            }
+        return live;
+    }
+    static uint32_t level_of(const Board &b) {
+        uint32_t n = 0;
+        for (uint16_t c : b) n += c != 0xFFFF;
+        return n;
+    }
+    static bool predecessor(const Board &a, const Board &b) {
+        if (level_of(a) >= level_of(b)) return false;
+        for (size_t i = 0; i < a.size(); ++i)
+            if (a[i] != 0xFFFF && a[i] != b[i]) return false;
+        return true;
+    }
+    static void update(std::vector<LevelStats> &st, const Profiler::Prof &p) {
+        st[p.level].low.update(p.min());
+        st[p.level].high.update(p.max());
+        st[p.level].total.update(p.total());
+    }
+
+    void wide(std::set<Board> &solutions, std::vector<LevelStats> &lead, std::vector<LevelStats> &other,
+              const Board &start, uint32_t levels) {
+        std::set<Board> cur{start}, next;
+        for (uint32_t l = 0; l < levels && !too_large; ++l) {
+            next.clear();
+            for (const Board &s : cur) {
+                const std::vector<char> live = rebuild(s);
+                for (size_t v = 0; v < pr.vars.size(); ++v) {
+                    if (!live[v]) continue;
+                    std::vector<char> nl = live;
+                    pr.apply(nl, (uint32_t)v);
+                    Board c = s;
+                    const Profiler::Var &var = pr.vars[v];
+                    c[var.y * m.W + var.x] = (uint16_t)(var.piece << 2 | var.o);
+                    const Profiler::Prof p{level_of(c), pr.profile(nl)};
+                    if (!p.valid() || next.count(c)) continue;
+                    next.insert(c);
+                    if (next.size() > max_situations) too_large = true;
+                    bool leads = solutions.empty() || p.level == m.P;
+                    for (auto it = solutions.begin(); !leads && it != solutions.end(); ++it) leads = predecessor(c, *it);
+                    update(leads ? lead : other, p);
+                }
+            }
+            std::swap(cur, next);
+        }
+        if (solutions.empty()) solutions = cur;
+    }
+};
+
+}  // namespace
+
+int oracle_profile(uint32_t width, uint32_t height, const uint32_t *colours, uint64_t max_situations,
+                   uint32_t *stats, uint16_t *solutions, uint32_t capacity, uint32_t *n_solutions) {
+    Model m;
+    if (!build_model(width, height, colours, m)) return ORACLE_BAD_PUZZLE;
+    if (!stats || !n_solutions) return ORACLE_BAD_ARGUMENT;
+    const Profiler pr(m);
+    const uint32_t P = m.P;
+    ProfileSearch ps{m, pr, max_situations};
+    const std::vector<char> live0(pr.vars.size(), 1);
+    const Profiler::Prof p0{0, pr.profile(live0)};
+
+    std::vector<Board> starts;
+    uint32_t levels = P;
+    Board empty(P, 0xFFFF);
+    if (P != 25) {
+        starts.push_back(empty);
+    } else {
+        levels = P - 4;
+        auto set_corner = [&](Board &b, std::vector<char> &live, uint32_t piece, uint32_t x, uint32_t y) {
+            for (uint32_t o = 0; o < 4; ++o) {
+                const int32_t v = pr.var_at[((size_t)(y * width + x) * P + piece) * 4 + o];
+                if (v >= 0 && live[(size_t)v]) {
+                    b[y * width + x] = (uint16_t)(piece << 2 | o);
+                    pr.apply(live, (uint32_t)v);
+                    break;
+                }
+            }
+        };
+        const std::vector<uint32_t> &corners = m.by_kind[2];
+        Board c0 = empty;
+        std::vector<char> live_c0 = live0;
+        set_corner(c0, live_c0, corners[0], 0, 0);
+        const uint32_t cx[3] = {width - 1, 0, width - 1}, cy[3] = {0, height - 1, height - 1};
+        std::vector<uint32_t> others{1, 2, 3};
+        do {
+            Board b = c0;
+            std::vector<char> live = live_c0;
+            for (uint32_t i = 0; i < 3; ++i) set_corner(b, live, corners[others[i]], cx[i], cy[i]);
+            starts.push_back(b);
+        } while (std::next_permutation(others.begin(), others.end()));
+    }
+
+    std::vector<LevelStats> all(P + 1), lead(P + 1), other(P + 1);
+    ProfileSearch::update(all, p0);
+    std::set<Board> sols;
+    for (const Board &s : starts) ps.wide(sols, all, all, s, levels);
+    ProfileSearch::update(lead, p0);
+    ProfileSearch::update(other, p0);
+    for (const Board &s : starts) {
+        bool has = false;
+        for (auto it = sols.begin(); !has && it != sols.end(); ++it) has = ProfileSearch::predecessor(s, *it);
+        if (has) ps.wide(sols, lead, other, s, levels);
+    }
+    if (ps.too_large) return ORACLE_LIMIT_REACHED;
+
+    const std::vector<LevelStats> *series[3] = {&all, &lead, &other};
+    for (uint32_t k = 0; k < 3; ++k)
+        for (uint32_t l = 0; l <= P; ++l) {
+            const LevelStats &st = (*series[k])[l];
+            const uint32_t v[6] = {st.low.get_min(), st.low.get_max(), st.high.get_min(),
+                                   st.high.get_max(), st.total.get_min(), st.total.get_max()};
+            for (uint32_t s = 0; s < 6; ++s) stats[((size_t)s * 3 + k) * (P + 1) + l] = v[s];
+        }
+    *n_solutions = (uint32_t)sols.size();
+    uint32_t i = 0;
+    for (auto it = sols.begin(); it != sols.end() && i < capacity; ++it, ++i)
+        if (solutions) std::copy(it->begin(), it->end(), solutions + (size_t)i * P);
     return ORACLE_OK;
 }

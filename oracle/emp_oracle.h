@@ -173,6 +173,20 @@ int oracle_situation_profile(uint32_t width, uint32_t height, const uint32_t *co
                              uint32_t *level, uint32_t *glutton, uint32_t *surface, uint32_t *row_index,
                              uint32_t *n_variables);
 
+/*
+ * The profile feature (include/feature_profile.h:582-868): a level-by-level
+ * search over every situation the capability allows (placements in any
+ * order, valid profiles only, each situation once), with per-level extrema
+ * of the profiles' min ("low"), max ("high") and total.
+ *   stats      [6][3][P + 1]: low min, low max, high min, high max, total
+ *              min, total max; series all situations, situations leading to
+ *              a solution, the others; by level
+ *   solutions  [capacity][P] the solution set (row-major boards), ordered
+ * Returns ORACLE_LIMIT_REACHED when a level holds more than max_situations.
+ */
+int oracle_profile(uint32_t width, uint32_t height, const uint32_t *colours, uint64_t max_situations,
+                   uint32_t *stats, uint16_t *solutions, uint32_t capacity, uint32_t *n_solutions);
+
 #ifdef __cplusplus
 }
 #endif

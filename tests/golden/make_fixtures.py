@@ -52,6 +52,23 @@ def main():
         json.dump(expected, f, indent=1, sort_keys=True)
     print("copied", len(copies), "files; expected strings for", len(expected), "tests")
     collect_situation_profiles()
+    collect_profiles()
+
+
+def collect_profiles():
+    """tests/profile_<n>: the profile feature's six VTK histograms (per-level
+    extrema of the capability profiles) under profile/<n>/."""
+    root = os.path.join(OUT, "profile")
+    for d in sorted(os.listdir(os.path.join(REF, "tests"))):
+        if not d.startswith("profile_"):
+            continue
+        n = d[len("profile_"):]
+        os.makedirs(os.path.join(root, n), exist_ok=True)
+        refdir = os.path.join(REF, "tests", d, "references")
+        for f in sorted(os.listdir(refdir)):
+            if f.endswith("_vtk.txt"):
+                shutil.copyfile(os.path.join(refdir, f), os.path.join(root, n, f))
+    print("profiles:", ", ".join(sorted(os.listdir(root))))
 
 
 def collect_situation_profiles():

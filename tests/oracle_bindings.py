@@ -46,6 +46,8 @@ def lib():
         _lib.oracle_glutton_max.restype = i32
         _lib.oracle_situation_profile.argtypes = [u32, u32, p, p, p, p, p, p, p]
         _lib.oracle_situation_profile.restype = i32
+        _lib.oracle_profile.argtypes = [u32, u32, p, u64, p, p, u32, p]
+        _lib.oracle_profile.restype = i32
     return _lib
 
 
@@ -233,3 +235,21 @@ def situation_profile(w, h, colours, board):
     L = int(level[0])
     return {"level": L, "glutton": glutton, "surface": surface[:L + 1].copy(), "row_index": row_index[:L + 1].copy(),
             "n_variables": int(nvar[0])}
+
+
+PROFILE_STATS = ("low_min", "low_max", "high_min", "high_max", "total_min", "total_max")
+
+
+def profile(w, h, colours, max_situations=1 << 22, capacity=4096):
+    """The profile feature: per-level extrema [6 stats][3 series][P + 1]
+    (PROFILE_STATS; series all / leading to a solution / others) and the
+    solution set (boards, ordered)."""
+    colours = np.ascontiguousarray(colours, dtype=np.uint32)
+    P = w * h
+    stats = np.zeros((6, 3, P + 1), np.uint32)
+    sols = np.zeros((capacity, P), np.uint16)
+    n = np.zeros(1, np.uint32)
+    st = lib().oracle_profile(w, h, _ptr(colours), max_situations, _ptr(stats), _ptr(sols), capacity, _ptr(n))
+    if st != OK:
+        raise ValueError("oracle_profile status %d" % st)
+    return {"stats": stats, "solutions": sols[:min(int(n[0]), capacity)].copy(), "n_solutions": int(n[0])}

@@ -1,0 +1,190 @@
+"""The profile feature (include/feature_profile.h): a level-by-level search
+over every situation the system equation's capability allows, with per-level
+extrema of the capability profiles.  The oracle against the reference's own
+output files (tests/profile_3_3/references, copied under tests/golden), the
+HIP library against the oracle, and the command line's files against the
+reference's byte for byte."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import edge_matching_puzzle_amd as emp
+import oracle_bindings as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FILES = {"low_min": "profile_low_min_vtk.txt", "low_max": "profile_low_max_vtk.txt",
+         "high_min": "profile_high_min_vtk.txt", "high_max": "profile_high_max_vtk.txt",
+         "total_min": "profile_total_min_vtk.txt", "total_max": "profile_total_max_vtk.txt"}
+
+
+def _puzzle(name):
+    return o.parse_puzzle(os.path.join(GOLDEN, "puzzles", name + ".txt"))
+
+
+def _series(name, stat):
+    lines = open(os.path.join(GOLDEN, "profile", name, FILES[stat])).read().splitlines()
+    return [[int(v) for v in l.split()] for l in lines[3:6]]
+
+
+def test_oracle_matches_the_reference_histograms():
+    w, h, c = _puzzle("3_3")
+    r = o.profile(w, h, c)
+    for s, stat in enumerate(o.PROFILE_STATS):
+        assert r["stats"][s].tolist() == _series("3_3", stat), stat
+
+
+def test_oracle_solutions_are_the_reference_list():
+    w, h, c = _puzzle("3_3")
+    r = o.profile(w, h, c)
+    ref = open(os.path.join(GOLDEN, "profile_3_3__solutions.txt")).read().split()
+    assert sorted(emp.situation_string(w, h, b) for b in r["solutions"]) == ref
+
+
+def _brute_force_situations(w, h, c):
+    """Every situation whose placed pieces fit (edges match, frame sides face
+    the frame, each piece once), with no free cell and no unplaced piece left
+    without a fitting placement, by level: an independent restatement of the
+    profile feature's search space on tiny boards."""
+    P = w * h
+    c = np.asarray(c).reshape(P, 4)
+    shown = lambda q, s, ori: int(c[q][(s + ori) % 4])
+    sim = []
+    for q in range(P):
+        sim.append(2 if (c[q][0] == c[q][2] and c[q][1] == c[q][3] and c[q][0] == c[q][1])
+                   else (1 if (c[q][0] == c[q][2] and c[q][1] == c[q][3]) else 0))
+    kept = lambda q, ori: (ori & ((1 << sim[q]) - 1)) == 0
+
+    def frame_ok(cell, q, ori):
+        x, y = cell % w, cell // w
+        off = (y == 0, x == w - 1, y == h - 1, x == 0)
+        return all((shown(q, s, ori) == 0) == off[s] for s in range(4))
+
+    def fits(board, cell, q, ori):
+        if not kept(q, ori) or not frame_ok(cell, q, ori):
+            return False
+        x, y = cell % w, cell // w
+        for s, (dx, dy) in enumerate(((0, -1), (1, 0), (0, 1), (-1, 0))):
+            nx, ny = x + dx, y + dy
+            if 0 <= nx < w and 0 <= ny < h and board[ny * w + nx] is not None:
+                nq, no = board[ny * w + nx]
+                if shown(q, s, ori) != shown(nq, (s + 2) % 4, no):
+                    return False
+        return True
+
+    def live_ok(board):
+        used = {b[0] for b in board if b is not None}
+        free = [i for i in range(P) if board[i] is None]
+        for cell in free:
+            if not any(fits(board, cell, q, ori) for q in range(P) if q not in used for ori in range(4)):
+                return False
+        for q in range(P):
+            if q not in used and not any(fits(board, cell, q, ori) for cell in free for ori in range(4)):
+                return False
+        return True
+
+    levels = [{tuple([None] * P)}]
+    for _ in range(P):
+        nxt = set()
+        for s in levels[-1]:
+            used = {b[0] for b in s if b is not None}
+            for cell in range(P):
+                if s[cell] is not None:
+                    continue
+                for q in range(P):
+                    if q in used:
+                        continue
+                    for ori in range(4):
+                        if fits(s, cell, q, ori):
+                            b = list(s)
+                            b[cell] = (q, ori)
+                            if live_ok(b):
+                                nxt.add(tuple(b))
+        levels.append(nxt)
+    return levels
+
+
+def test_oracle_solutions_and_search_space_by_brute_force():
+    """On the 3x3 example: the oracle's solutions are exactly the full boards
+    of an independent search (all rotations of the puzzle's solutions), and
+    the level sizes that search finds match the oracle's statistics having a
+    value at exactly the levels that hold situations."""
+    w, h, c = _puzzle("3_3")
+    levels = _brute_force_situations(w, h, c)
+    r = o.profile(w, h, c)
+    full = {tuple((q << 2) | ori for q, ori in b) for b in levels[-1]}
+    assert {tuple(int(v) for v in b) for b in r["solutions"]} == full
+    assert r["n_solutions"] == len(full) == 8
+    total_max = r["stats"][o.PROFILE_STATS.index("total_max")][0]
+    for lvl in range(1, w * h):
+        assert (total_max[lvl] > 0) == (len(levels[lvl]) > 0)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_oracle_invariants_on_generated(seed):
+    """Properties fixed by the definitions: at level 0 every series holds the
+    level-0 profile (total = twice the variables); a full board's profile is
+    all zeros; min <= max per stat; the general series is the union of the
+    other two (its minima are the smaller, its maxima the larger)."""
+    c, _ = emp.generate(100 + seed, 3, 3, 2, 3)
+    r = o.profile(3, 3, c)
+    st = r["stats"]
+    n_var = emp.backtrack_variables(3, 3, c)
+    for k in range(3):
+        assert st[o.PROFILE_STATS.index("total_max"), k, 0] == 2 * n_var
+        assert st[o.PROFILE_STATS.index("total_min"), k, 0] == 2 * n_var
+    for a, b in (("low_min", "low_max"), ("high_min", "high_max"), ("total_min", "total_max")):
+        i, j = o.PROFILE_STATS.index(a), o.PROFILE_STATS.index(b)
+        assert (st[i] <= st[j]).all()
+    assert st[:, :, 9].max() == 0
+    for stat in o.PROFILE_STATS:
+        s = o.PROFILE_STATS.index(stat)
+        for lvl in range(1, 9):
+            parts = [v for v in (st[s, 1, lvl], st[s, 2, lvl]) if v]
+            if stat.endswith("max") and parts:
+                assert st[s, 0, lvl] == max(parts)
+    assert r["n_solutions"] >= 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["3_3"])
+def test_gpu_matches_the_reference_histograms(gpu_context, name):
+    w, h, c = _puzzle(name)
+    print("case: profile", name, flush=True)
+    r = gpu_context.profile(w, h, c)
+    for s, stat in enumerate(o.PROFILE_STATS):
+        assert r["stats"][s].tolist() == _series(name, stat), stat
+    ref = open(os.path.join(GOLDEN, "profile_3_3__solutions.txt")).read().split()
+    assert sorted(emp.situation_string(w, h, b) for b in r["solutions"]) == ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h,B,I,seed", [(3, 3, 2, 3, 1), (3, 3, 2, 3, 2), (4, 3, 2, 4, 3), (3, 4, 2, 3, 6),
+                                          (4, 4, 5, 8, 4)])
+def test_gpu_matches_the_oracle_on_generated(gpu_context, w, h, B, I, seed):
+    c, _ = emp.generate(200 + seed, w, h, B, I)
+    print("case: profile generated", w, h, B, I, seed, flush=True)
+    ref = o.profile(w, h, c)
+    r = gpu_context.profile(w, h, c)
+    assert (r["stats"] == ref["stats"]).all()
+    assert r["n_solutions"] == ref["n_solutions"]
+    assert sorted(map(tuple, r["solutions"].tolist())) == sorted(map(tuple, ref["solutions"].tolist()))
+    assert r["level_sizes"][0] >= 1
+
+
+@pytest.mark.gpu
+def test_cli_writes_the_reference_files(tmp_path):
+    """The command line of tests/profile_3_3/test.info writes the reference's
+    six histograms byte for byte and the solution list."""
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "edge_matching_puzzle_amd",
+                       "edge_matching_puzzle")
+    print("case: CLI profile 3_3", flush=True)
+    p = subprocess.run([exe, os.path.join(GOLDEN, "puzzles", "3_3.txt"), "ressources", "profile", "--enable_gui=0"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    for stat, f in FILES.items():
+        assert open(os.path.join(tmp_path, f), "rb").read() == \
+            open(os.path.join(GOLDEN, "profile", "3_3", f), "rb").read(), f
+    assert open(os.path.join(tmp_path, "solutions.txt")).read().split() == \
+        open(os.path.join(GOLDEN, "profile_3_3__solutions.txt")).read().split()
