@@ -775,7 +775,9 @@ void profile_search_on_device(hipStream_t stream, Workspace &ws, const Puzzle &p
     std::vector<uint16_t> sols;
     uint64_t n_sol = 0;
     for (const std::vector<uint16_t> &st : starts) {
-        out.level_sizes[0] += 1;
+        uint32_t lv = 0;
+        for (uint16_t c : st) lv += c != kEmptyCell;
+        out.level_sizes[lv] += 1;
         auto r = search(st, nullptr, 0, true);
         if (!have_solutions && r.second) {
             sols = std::move(r.first);

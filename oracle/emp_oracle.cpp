@@ -1394,14 +1394,44 @@ struct ProfileSearch {
     const Profiler &pr;
     uint64_t max_situations;
     bool too_large = false;
+    // live variables as bitsets: keep[v] = the variables v's transition
+    // leaves (Profiler::masks), at[i] = the variables of position i (i < P)
+    // or of piece i - P
+    size_t nw = 0;
+    std::vector<uint64_t> keep, at;
 
-    std::vector<char> rebuild(const Board &b) const {
-        std::vector<char> live(pr.vars.size(), 1);
+    ProfileSearch(const Model &model, const Profiler &profiler, uint64_t limit)
+        : m(model), pr(profiler), max_situations(limit) {
+        const size_t nv = pr.vars.size();
+        nw = (nv + 63) / 64;
+        keep.assign(nv * nw, 0);
+        at.assign((size_t)2 * m.P * nw, 0);
+        for (size_t v = 0; v < nv; ++v) {
+            for (size_t u = 0; u < nv; ++u)
+                if (!pr.masks(pr.vars[v], pr.vars[u])) keep[v * nw + u / 64] |= 1ull << (u % 64);
+            at[(size_t)pr.vars[v].pos * nw + v / 64] |= 1ull << (v % 64);
+            at[(size_t)(m.P + pr.vars[v].piece) * nw + v / 64] |= 1ull << (v % 64);
+        }
+    }
+    std::vector<uint64_t> all_live() const {
+        std::vector<uint64_t> live(nw, 0);
+        for (size_t v = 0; v < pr.vars.size(); ++v) live[v / 64] |= 1ull << (v % 64);
+        return live;
+    }
+    void apply(std::vector<uint64_t> &live, size_t v) const {
+        for (size_t w = 0; w < nw; ++w) live[w] &= keep[v * nw + w];
+    }
+    std::vector<uint32_t> profile(const std::vector<uint64_t> &live) const {
+        std::vector<uint32_t> c(2 * m.P, 0);
+        for (size_t i = 0; i < c.size(); ++i)
+            for (size_t w = 0; w < nw; ++w) c[i] += (uint32_t)__builtin_popcountll(live[w] & at[i * nw + w]);
+        std::sort(c.begin(), c.end());
+        return c;
+    }
+    std::vector<uint64_t> rebuild(const Board &b) const {
+        std::vector<uint64_t> live = all_live();
         for (uint32_t cell = 0; cell < m.P; ++cell)
-            if (b[cell] != 0xFFFF) {
-                const int32_t v = pr.var_at[((size_t)cell * m.P + (b[cell] >> 2)) * 4 + (b[cell] & 3)];
-                pr.apply(live, (uint32_t)v);
-            }
+            if (b[cell] != 0xFFFF) apply(live, (size_t)pr.var_at[((size_t)cell * m.P + (b[cell] >> 2)) * 4 + (b[cell] & 3)]);
         return live;
     }
     static uint32_t level_of(const Board &b) {
@@ -1427,22 +1457,24 @@ struct ProfileSearch {
         for (uint32_t l = 0; l < levels && !too_large; ++l) {
             next.clear();
             for (const Board &s : cur) {
-                const std::vector<char> live = rebuild(s);
-                for (size_t v = 0; v < pr.vars.size(); ++v) {
-                    if (!live[v]) continue;
-                    std::vector<char> nl = live;
-                    pr.apply(nl, (uint32_t)v);
-                    Board c = s;
-                    const Profiler::Var &var = pr.vars[v];
-                    c[var.y * m.W + var.x] = (uint16_t)(var.piece << 2 | var.o);
-                    const Profiler::Prof p{level_of(c), pr.profile(nl)};
-                    if (!p.valid() || next.count(c)) continue;
-                    next.insert(c);
-                    if (next.size() > max_situations) too_large = true;
-                    bool leads = solutions.empty() || p.level == m.P;
-                    for (auto it = solutions.begin(); !leads && it != solutions.end(); ++it) leads = predecessor(c, *it);
-                    update(leads ? lead : other, p);
-                }
+                const std::vector<uint64_t> live = rebuild(s);
+                for (size_t w = 0; w < nw; ++w)
+                    for (uint64_t bits = live[w]; bits; bits &= bits - 1) {
+                        const size_t v = w * 64 + (size_t)__builtin_ctzll(bits);
+                        std::vector<uint64_t> nl = live;
+                        apply(nl, v);
+                        Board c = s;
+                        const Profiler::Var &var = pr.vars[v];
+                        c[var.y * m.W + var.x] = (uint16_t)(var.piece << 2 | var.o);
+                        const Profiler::Prof p{level_of(c), profile(nl)};
+                        if (!p.valid() || next.count(c)) continue;
+                        next.insert(c);
+                        if (next.size() > max_situations) too_large = true;
+                        bool leads = solutions.empty() || p.level == m.P;
+                        for (auto it = solutions.begin(); !leads && it != solutions.end(); ++it)
+                            leads = predecessor(c, *it);
+                        update(leads ? lead : other, p);
+                    }
             }
             std::swap(cur, next);
         }
@@ -1459,7 +1491,7 @@ int oracle_profile(uint32_t width, uint32_t height, const uint32_t *colours, uin
     if (!stats || !n_solutions) return ORACLE_BAD_ARGUMENT;
     const Profiler pr(m);
     const uint32_t P = m.P;
-    ProfileSearch ps{m, pr, max_situations};
+    ProfileSearch ps(m, pr, max_situations);
     const std::vector<char> live0(pr.vars.size(), 1);
     const Profiler::Prof p0{0, pr.profile(live0)};
 

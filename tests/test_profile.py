@@ -4,6 +4,7 @@ extrema of the capability profiles.  The oracle against the reference's own
 output files (tests/profile_3_3/references, copied under tests/golden), the
 HIP library against the oracle, and the command line's files against the
 reference's byte for byte."""
+import json
 import os
 import subprocess
 
@@ -171,6 +172,27 @@ def test_gpu_matches_the_oracle_on_generated(gpu_context, w, h, B, I, seed):
     assert r["n_solutions"] == ref["n_solutions"]
     assert sorted(map(tuple, r["solutions"].tolist())) == sorted(map(tuple, ref["solutions"].tolist()))
     assert r["level_sizes"][0] >= 1
+
+
+@pytest.mark.gpu
+def test_gpu_five_by_five_corner_starts_match_the_oracle_golden(gpu_context):
+    """At 25 pieces the search starts from the four corners placed in six
+    orders and runs 21 levels; tests/golden/profile_5x5_seed211.json holds the
+    oracle's result (make_profile_5x5_golden.py)."""
+    g = json.load(open(os.path.join(GOLDEN, "profile_5x5_seed211.json")))
+    w, h = g["width"], g["height"]
+    c, _ = emp.generate(g["seed"], w, h, g["border_colours"], g["interior_colours"])
+    print("case: profile 5x5 golden", flush=True)
+    r = gpu_context.profile(w, h, c)
+    assert r["stats"].tolist() == g["stats"]
+    assert r["n_solutions"] == g["n_solutions"]
+    assert sorted(map(tuple, r["solutions"].tolist())) == sorted(map(tuple, g["solutions"]))
+    # six starts, each with the four corners placed at the four corner cells
+    assert len(r["starts"]) == 6 and r["level_sizes"][4] == 6
+    corners = {0, w - 1, (h - 1) * w, h * w - 1}
+    for s in r["starts"]:
+        assert {i for i, v in enumerate(s) if v != 0xFFFF} == corners
+    assert len({tuple(s) for s in r["starts"].tolist()}) == 6
 
 
 @pytest.mark.gpu
