@@ -122,7 +122,7 @@ def _roofline(nodes, kernel_ms, work, launches, peaks=None):
 BATCHED_SPACE = {
     "waves_per_cu": [12, 16],
     "waves_per_block": [2, 4],
-    "steps_per_round": [16, 32, 64],
+    "steps_per_round": [64, 128, 256],
     "target_per_lane": [2, 4, 8],
     "step_budget": [1 << 14, 1 << 16, 1 << 18],
 }

@@ -14,7 +14,7 @@ import time
 DEFAULT_SPACE = {
     "waves_per_cu": [12, 16, 20],
     "waves_per_block": [2, 4],
-    "steps_per_round": [16, 32, 64],
+    "steps_per_round": [64, 128, 256],
     "target_per_lane": [2, 4, 8],
     # an item over its budget always requeues what remains (only the split-off
     # second half needs free queue room), so small budgets stay bounded

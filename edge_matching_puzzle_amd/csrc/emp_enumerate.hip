@@ -1578,7 +1578,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.lds_table_words = lds_table ? max_table : 0;
     a.avail_words = avail_region;
     a.stride = stride;
-    a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 32;
+    a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 128;
     a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 16);
     a.drain_budget = std::min<uint64_t>(a.step_budget, cfg.drain_budget ? cfg.drain_budget : 512);
     a.nodes = d_nodes;

@@ -102,7 +102,7 @@ typedef struct {
     uint32_t waves_per_block;  /* 0 = sized to fit the most resident waves;
                                   at most 16 (4 for boards with more than 64
                                   pieces of one kind)                          */
-    uint32_t steps_per_round;  /* steps between work refills, 0 = default       */
+    uint32_t steps_per_round;  /* steps between work refills, 0 = default (128) */
     uint64_t step_budget;      /* passes only: steps per work item per pass
                                   before it is requeued (and split in two
                                   while the queue has room), 0 = default      */

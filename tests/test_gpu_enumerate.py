@@ -101,8 +101,9 @@ def test_persistent_ring_wraps(gpu_context):
     c, _ = emp.generate(11, w, h, 6, 16)
     ref = o.enumerate_(w, h, c)
     print("case: persistent, ring reuse 8x8 B=6 I=16 seed 11", ref["n_nodes"], flush=True)
+    # rounds of 8 steps: the queue is checked (and a hand-over made) every 64
     (r,), st = gpu_context.enumerate([(w, h, c)], target_items=1, waves_per_cu=1, step_budget=64,
-                                     schedule="persistent")
+                                     steps_per_round=8, schedule="persistent")
     lanes = st["blocks"] * st["threads"]
     assert st["dfs_launches"] == 1 and st["tickets"] > 16 * lanes, (st["tickets"], lanes)
     _check_same(r, ref)
