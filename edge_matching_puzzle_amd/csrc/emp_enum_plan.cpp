@@ -221,7 +221,7 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
     }
 
     for (uint32_t w = 0; w < 16; ++w) b[h.off_ones + w] = ~0ull;
-    const uint32_t sb = stack_bytes ? stack_bytes : (R <= 4 ? 1 : 2);
+    const uint32_t sb = stack_bytes ? stack_bytes : 1;
     h.stack_bytes = sb;
     {
         uint8_t *col = reinterpret_cast<uint8_t *>(b + h.off_col);

@@ -88,7 +88,9 @@ uint32_t natural_row_words(const Puzzle &puzzle);
 
 // Builds the table with mask rows of max(row_words, natural_row_words) words;
 // throws std::invalid_argument when the first position has no candidate.
-// stack_bytes: bytes per transition in the kernels' LDS stacks (1 or 2).
+// stack_bytes: bytes per level in the byte plane of the kernels' LDS stacks
+// (1, the default; rows wider than four words keep the transitions' two high
+// bits in a second plane).
 EnumTable build_enum_table(const Puzzle &puzzle, int order, uint32_t row_words = 0, uint32_t stack_bytes = 0);
 
 // Converts a stack of transitions (one per strategy index) to a row-major board.

@@ -553,8 +553,53 @@ constexpr uint32_t dfs_max_threads(int w) { return w <= 4 ? 1024u : 256u; }
 // LIMITS: the launch has node limits, checked while items run as well as
 // when they are taken (compiled only where needed: the check's live state
 // costs registers in the step loop).
+// A wave's stacks of placed transitions in LDS, lane-interleaved (level j of
+// lane l at j * 64 + l): one byte per level while every kind has at most 256
+// transitions (rows of up to four words); wider rows (up to 1,024
+// transitions) keep the low byte there and the two high bits of four levels
+// per byte in a second plane -- 1.25 bytes per level instead of 2, which is
+// what fits five waves per CU at 16x16 instead of three.
+template <bool SPLIT>
+struct LaneStack {
+    uint8_t *lo;     // [levels][64]
+    uint8_t *hi;     // [levels / 4][64] (SPLIT)
+    uint32_t lane;
+    __device__ __forceinline__ uint32_t get(uint32_t j) const {
+        uint32_t v = lo[j * kWave + lane];
+        if (SPLIT) v |= ((uint32_t)(hi[(j >> 2) * kWave + lane] >> ((j & 3) * 2)) & 3u) << 8;
+        return v;
+    }
+    __device__ __forceinline__ void set(uint32_t j, uint32_t v) const {
+        lo[j * kWave + lane] = (uint8_t)v;
+        if (SPLIT) {
+            uint8_t &h = hi[(j >> 2) * kWave + lane];
+            const uint32_t sh = (j & 3) * 2;
+            h = (uint8_t)((h & ~(3u << sh)) | (((v >> 8) & 3u) << sh));
+        }
+    }
+    // the level a neighbour record addresses by its byte-plane offset (level * 64)
+    __device__ __forceinline__ uint32_t at(uint32_t off) const {
+        uint32_t v = lo[off + lane];
+        if (SPLIT) v |= ((uint32_t)(hi[((off >> 8) << 6) + lane] >> ((off >> 5) & 6)) & 3u) << 8;
+        return v;
+    }
+};
+
+// S(j) of the kernel: reads and assignments of level j of this lane's stack.
+template <bool SPLIT>
+struct LevelRef {
+    const LaneStack<SPLIT> &s;
+    uint32_t j;
+    __device__ __forceinline__ operator uint32_t() const { return s.get(j); }
+    __device__ __forceinline__ LevelRef &operator=(uint32_t v) {
+        s.set(j, v);
+        return *this;
+    }
+};
+
 template <int W, bool LDS_TABLE, bool RANKED, bool PERSIST, bool LIMITS>
 __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
+    constexpr bool SPLIT = W > 4;
     using STK = typename std::conditional<(W <= 4), uint8_t, uint16_t>::type;
     extern __shared__ uint64_t smem[];
     const int lane = (int)(threadIdx.x & (kWave - 1));
@@ -567,9 +612,10 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     }
     // Per wave: available-transition words then the stack of transitions,
     // u8 when every kind has at most 256 transitions (boards up to 10x10).
-    const uint32_t per_wave = a.avail_words * kWave + (a.stride * kWave * (uint32_t)sizeof(STK)) / 8;
+    const uint32_t per_wave = a.avail_words * kWave + (a.stride * kWave + (SPLIT ? a.stride / 4 * kWave : 0u)) / 8;
     uint64_t *avail = smem + a.lds_table_words + (size_t)wave * per_wave;
-    STK *stk = reinterpret_cast<STK *>(avail + a.avail_words * kWave);
+    uint8_t *stk_lo = reinterpret_cast<uint8_t *>(avail + a.avail_words * kWave);
+    const LaneStack<SPLIT> stack{stk_lo, stk_lo + a.stride * kWave, (uint32_t)lane};
 
     TableView tv;
     if (LDS_TABLE) {
@@ -616,7 +662,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
     // region (sized for W words from the interior kind's offset); the base
     // mask zeroes them.
     auto A = [&](uint32_t ab, uint32_t w) -> uint64_t & { return avail[(ab + w) * kWave + lane]; };
-    auto S = [&](uint32_t j) -> STK & { return stk[j * kWave + lane]; };
+    auto S = [&](uint32_t j) { return LevelRef<SPLIT>{stack, j}; };
     auto finish = [&]() {
         if (a.exec_nodes) a.exec_nodes[a.exec_base + item] = nodes;
         if (a.work) {
@@ -1005,10 +1051,7 @@ __global__ void __launch_bounds__(dfs_max_threads(W)) dfs_kernel(DfsArgs a) {
             }
             uint64_t cc[W];
             candidates<W, false>(tv, tk, d, tl,
-                          [&](uint32_t off) {
-                              return (uint32_t)*reinterpret_cast<const STK *>(
-                                  reinterpret_cast<const uint8_t *>(stk + lane) + off);
-                          },
+                          [&](uint32_t off) { return stack.at(off); },
                           [&](uint64_t dd, int w) { return A(desc_abase(dd), (uint32_t)w); },
                           cc, RANKED ? rank_base + (size_t)tk * rank_stride : nullptr);
             // A placement whose next level has no candidate is a dead end:
@@ -1381,7 +1424,9 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // available words of the kinds, and room for reads of W words at the
     // interior kind's offset when it has fewer words than W
     const uint32_t avail_region = std::max(max_avail, max_reach);
-    const size_t per_wave_bytes = (size_t)avail_region * kWave * 8 + (size_t)stride * kWave * (narrow ? 1 : 2);
+    // stacks: a byte plane of levels, and for wide rows a plane of the
+    // transitions' high bits, four levels per byte
+    const size_t per_wave_bytes = (size_t)avail_region * kWave * 8 + (size_t)stride * kWave + (narrow ? 0 : (size_t)stride / 4 * kWave);
     // One table goes to LDS when it leaves room for four waves per block.
     const bool lds_table = n_tab == 1 && (size_t)max_table * 8 + 4 * per_wave_bytes <= dev.lds_per_block;
     const size_t table_bytes = lds_table ? (size_t)max_table * 8 : 0;
