@@ -130,13 +130,15 @@ __global__ void __launch_bounds__(kProfThreads) profile_kernel(ProfArgs a) {
 //
 //   search_children_kernel  one block per parent: the parent's capability is
 //                           built in LDS from the level-0 one (one transition
-//                           per placed cell), then each thread takes capability
-//                           words and, for each live variable in them, counts
-//                           the child's 2P infos without storing the child's
-//                           capability (a child word is the parent's word
-//                           through the variable's transition); extrema are
-//                           reduced in LDS, then one global atomic per block
-//                           and value; valid children's boards are appended.
+//                           per placed cell) with its 2P counts; its live
+//                           variables are dealt evenly to the threads (a
+//                           running count per capability word, searched by
+//                           bisection), and each child's counts are the
+//                           parent's minus what the placement removes (only
+//                           the placed piece's bits, the placed cell and its
+//                           neighbours are read); extrema are reduced in LDS,
+//                           then one global atomic per block and value; valid
+//                           children's boards are appended.
 //   search_dedup_kernel     one thread per appended board: an open-addressing
 //                           table of board indices (CAS on empty slots, full
 //                           board compare on a hit, linear probing) keeps the
@@ -194,6 +196,8 @@ __device__ __forceinline__ void placement_rows(const SearchArgs &a, uint32_t pos
 __global__ void __launch_bounds__(kSearchThreads) search_children_kernel(SearchArgs a) {
     __shared__ uint64_t cap[256 * kCapWords];
     __shared__ uint16_t board[256];
+    __shared__ uint32_t cnt[512];                  // the parent's counts: by position, then by piece
+    __shared__ uint32_t wsum[256 * kCapWords + 1]; // live variables before each capability word
     __shared__ uint32_t st[3 * kStatValues];
     __shared__ uint32_t precedes[kMaxPrecedes];
     __shared__ uint32_t n_precedes;
@@ -215,85 +219,141 @@ __global__ void __launch_bounds__(kSearchThreads) search_children_kernel(SearchA
             cap[t] = transition_word(cap[t], t / kCapWords, t % kCapWords, cell, bv >> 2, nbm, nrow);
         __syncthreads();
     }
+    // the parent's 2P counts, and a running count of live variables per word
+    for (uint32_t i = tid; i < 2 * P; i += kSearchThreads) {
+        uint32_t c = 0;
+        if (i < P) {
+            for (uint32_t w = 0; w < kCapWords; ++w) c += (uint32_t)__popcll(cap[i * kCapWords + w]);
+        } else {
+            const uint32_t q = i - P, qw = q >> 6, qb = q & 63;
+            for (uint32_t j = 0; j < P; ++j)
+#pragma unroll
+                for (uint32_t oo = 0; oo < 4; ++oo) c += (uint32_t)(cap[j * kCapWords + oo * 4 + qw] >> qb) & 1u;
+        }
+        cnt[i] = c;
+    }
+    if (tid == 0) {
+        // exclusive sum over the words (P <= 256: at most 4,096 words)
+        uint32_t run = 0;
+        for (uint32_t t = 0; t < n_words; ++t) {
+            wsum[t] = run;
+            run += (uint32_t)__popcll(cap[t]);
+        }
+        wsum[n_words] = run;
+    }
     // second pass: the solutions the parent precedes (its placed cells equal theirs)
     bool scan_all = false;
     if (a.sols) {
-        for (uint32_t s = tid; s < a.n_sol; s += kSearchThreads) {
-            const uint16_t *sb = a.sols + (size_t)s * P;
+        for (uint32_t s2 = tid; s2 < a.n_sol; s2 += kSearchThreads) {
+            const uint16_t *sb = a.sols + (size_t)s2 * P;
             bool pre = true;
             for (uint32_t c = 0; c < P && pre; ++c) pre = board[c] == kEmptyCell || board[c] == sb[c];
             if (pre) {
                 const uint32_t i = atomicAdd(&n_precedes, 1u);
-                if (i < kMaxPrecedes) precedes[i] = s;
+                if (i < kMaxPrecedes) precedes[i] = s2;
             }
         }
-        __syncthreads();
-        scan_all = n_precedes > kMaxPrecedes;
     }
-    for (uint32_t t = tid; t < n_words; t += kSearchThreads) {
-        const uint32_t pos = t / kCapWords, w = t % kCapWords;
-        for (uint64_t bits = cap[t]; bits; bits &= bits - 1) {
-            const uint32_t piece = (w & 3) * 64 + (uint32_t)__builtin_ctzll(bits), o = w >> 2;
-            int32_t nbm[4];
-            const uint64_t *nrow[4];
-            placement_rows(a, pos, piece, o, nbm, nrow);
-            // the child's 2P counts: by position, then by piece
-            uint32_t zeros = 0, mn = 0xFFFFFFFFu, mx = 0, total = 0;
-            auto count = [&](uint32_t c) {
-                zeros += c == 0;
-                mn = c ? min(mn, c) : mn;
-                mx = max(mx, c);
-                total += c;
-            };
-            for (uint32_t j = 0; j < P; ++j) {
-                uint32_t c = 0;
+    __syncthreads();
+    scan_all = a.sols && n_precedes > kMaxPrecedes;
+    // children dealt evenly: variable v of the parent (in word order) to thread v mod 256
+    const uint32_t n_vars = wsum[n_words];
+    for (uint32_t v = tid; v < n_vars; v += kSearchThreads) {
+        // the word holding variable v, then its bit
+        uint32_t lo = 0, hi = n_words;   // wsum[lo] <= v < wsum[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (wsum[mid] <= v) lo = mid;
+            else hi = mid;
+        }
+        uint64_t bits = cap[lo];
+        for (uint32_t r = v - wsum[lo]; r; --r) bits &= bits - 1;
+        const uint32_t pos = lo / kCapWords, w = lo % kCapWords;
+        const uint32_t piece = (w & 3) * 64 + (uint32_t)__builtin_ctzll(bits), o = w >> 2;
+        const uint32_t pw = piece >> 6, pbit = piece & 63;
+        int32_t nbm[4];
+        const uint64_t *nrow[4];
+        placement_rows(a, pos, piece, o, nbm, nrow);
+        // The child's counts from the parent's: its position and piece lose
+        // every variable; another position loses the piece's variables, and a
+        // neighbour also those the colour constraint removes; another piece
+        // loses its variables at the position and those removed at the
+        // neighbours.
+        uint32_t zeros = 0, mn = 0xFFFFFFFFu, mx = 0, total = 0;
+        auto count = [&](uint32_t c) {
+            zeros += c == 0;
+            mn = c ? min(mn, c) : mn;
+            mx = max(mx, c);
+            total += c;
+        };
+        for (uint32_t j = 0; j < P; ++j) {
+            uint32_t c = 0;
+            bool neighbour = false;
+#pragma unroll
+            for (uint32_t sd = 0; sd < 4; ++sd) neighbour |= nbm[sd] == (int32_t)j;
+            if (j == pos) {
+                c = 0;
+            } else if (neighbour) {
                 for (uint32_t ww = 0; ww < kCapWords; ++ww)
                     c += (uint32_t)__popcll(transition_word(cap[j * kCapWords + ww], j, ww, pos, piece, nbm, nrow));
-                count(c);
+            } else {
+                c = cnt[j];
+#pragma unroll
+                for (uint32_t oo = 0; oo < 4; ++oo) c -= (uint32_t)(cap[j * kCapWords + oo * 4 + pw] >> pbit) & 1u;
             }
-            for (uint32_t q = 0; q < P; ++q) {
+            count(c);
+        }
+        for (uint32_t q = 0; q < P; ++q) {
+            uint32_t c = 0;
+            if (q != piece) {
                 const uint32_t qw = q >> 6, qb = q & 63;
-                uint32_t c = 0;
-                for (uint32_t j = 0; j < P; ++j)
+                c = cnt[P + q];
+#pragma unroll
+                for (uint32_t oo = 0; oo < 4; ++oo) c -= (uint32_t)(cap[pos * kCapWords + oo * 4 + qw] >> qb) & 1u;
+#pragma unroll
+                for (uint32_t sd = 0; sd < 4; ++sd) {
+                    if (nbm[sd] < 0) continue;
+                    const uint32_t n = (uint32_t)nbm[sd];
 #pragma unroll
                     for (uint32_t oo = 0; oo < 4; ++oo) {
                         const uint32_t ww = oo * 4 + qw;
-                        c += (uint32_t)(transition_word(cap[j * kCapWords + ww], j, ww, pos, piece, nbm, nrow) >> qb) & 1u;
+                        c -= (uint32_t)((cap[n * kCapWords + ww] & ~nrow[sd][ww]) >> qb) & 1u;
                     }
-                count(c);
-            }
-            // every placed position and piece has no variable left; a valid
-            // profile leaves no other empty (include/situation_profile.h:170-178)
-            if (L1 < P && zeros != 2 * L1) continue;
-            const uint32_t pmin = L1 < P ? mn : 0u;
-            const uint32_t cv = piece << 2 | o;
-            uint32_t series = 0;
-            if (a.sols) {
-                bool leads = L1 == P;
-                if (scan_all) {
-                    for (uint32_t s = 0; s < a.n_sol && !leads; ++s) {
-                        const uint16_t *sb = a.sols + (size_t)s * P;
-                        bool pre = sb[pos] == cv;
-                        for (uint32_t c = 0; c < P && pre; ++c) pre = board[c] == kEmptyCell || board[c] == sb[c];
-                        leads = pre;
-                    }
-                } else {
-                    for (uint32_t i = 0; i < n_precedes && !leads; ++i) leads = a.sols[(size_t)precedes[i] * P + pos] == cv;
                 }
-                series = leads ? 1u : 2u;
             }
-            uint32_t *sv = st + series * kStatValues;
-            atomicMin(&sv[0], pmin);
-            atomicMax(&sv[1], pmin);
-            atomicMin(&sv[2], mx);
-            atomicMax(&sv[3], mx);
-            atomicMin(&sv[4], total);
-            atomicMax(&sv[5], total);
-            const unsigned long long slot = atomicAdd(a.n_cand, 1ull);
-            if (slot < a.cand_cap) {
-                uint16_t *dst = a.cand + slot * P;
-                for (uint32_t c = 0; c < P; ++c) dst[c] = c == pos ? (uint16_t)cv : board[c];
+            count(c);
+        }
+        // every placed position and piece has no variable left; a valid
+        // profile leaves no other empty (include/situation_profile.h:170-178)
+        if (L1 < P && zeros != 2 * L1) continue;
+        const uint32_t pmin = L1 < P ? mn : 0u;
+        const uint32_t cv = piece << 2 | o;
+        uint32_t series = 0;
+        if (a.sols) {
+            bool leads = L1 == P;
+            if (scan_all) {
+                for (uint32_t s2 = 0; s2 < a.n_sol && !leads; ++s2) {
+                    const uint16_t *sb = a.sols + (size_t)s2 * P;
+                    bool pre = sb[pos] == cv;
+                    for (uint32_t c = 0; c < P && pre; ++c) pre = board[c] == kEmptyCell || board[c] == sb[c];
+                    leads = pre;
+                }
+            } else {
+                for (uint32_t i = 0; i < n_precedes && !leads; ++i) leads = a.sols[(size_t)precedes[i] * P + pos] == cv;
             }
+            series = leads ? 1u : 2u;
+        }
+        uint32_t *sv = st + series * kStatValues;
+        atomicMin(&sv[0], pmin);
+        atomicMax(&sv[1], pmin);
+        atomicMin(&sv[2], mx);
+        atomicMax(&sv[3], mx);
+        atomicMin(&sv[4], total);
+        atomicMax(&sv[5], total);
+        const unsigned long long slot = atomicAdd(a.n_cand, 1ull);
+        if (slot < a.cand_cap) {
+            uint16_t *dst = a.cand + slot * P;
+            for (uint32_t c = 0; c < P; ++c) dst[c] = c == pos ? (uint16_t)cv : board[c];
         }
     }
     __syncthreads();
