@@ -1125,9 +1125,24 @@ __global__ void fill_u32(unsigned int *p, uint32_t n, unsigned int v) {
 // field), so that the waves of a CU work on few puzzles and their tables
 // stay in the caches.  Order within a puzzle is arbitrary; the enumeration
 // does not depend on the order of its items.
+// Within a wave, consecutive items of one puzzle form a run (the items a pass
+// emits mostly come in such runs); each run counts or reserves with one
+// atomic.  run_of: the run's first lane and its length.
+__device__ __forceinline__ void run_of(uint32_t p, uint32_t lane, uint32_t &first, uint32_t &len) {
+    const uint32_t prev = __shfl_up(p, 1);
+    const uint64_t heads = __ballot(lane == 0 || p != prev);
+    first = 63u - (uint32_t)__builtin_clzll(heads & ((2ull << lane) - 1));   // lane 63: every lane
+    const uint64_t later = lane == 63 ? 0ull : heads >> (lane + 1);
+    const uint32_t next = later ? lane + 1 + (uint32_t)__builtin_ctzll(later) : 64u;
+    len = next - lane;   // meaningful on the run's first lane
+}
+
 __global__ void items_per_puzzle(const uint64_t *hdr, uint32_t n_items, unsigned int *count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_items) atomicAdd(&count[hdr_puzzle(hdr[i])], 1u);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+    const uint32_t p = i < n_items ? hdr_puzzle(hdr[i]) : 0xFFFFFFFFu;
+    uint32_t first, len;
+    run_of(p, lane, first, len);
+    if (first == lane && p != 0xFFFFFFFFu) atomicAdd(&count[p], len);
 }
 
 // In-place exclusive prefix sum of n counts, one block of 1024 threads.
@@ -1152,28 +1167,33 @@ __global__ void __launch_bounds__(1024) exclusive_scan_u32(unsigned int *v, uint
     }
 }
 
-// kScatterLanes lanes per item: the first takes the item's slot, then the
-// lanes copy the placed part of its stack row (the transitions of indices
-// below its level) in u64 words (rows start on 8-byte boundaries: stride is
-// a multiple of 4).
+// Each item takes its slot (one atomic per run of one puzzle's items in the
+// wave), then the wave copies the placed part of its 64 items' stack rows
+// (the transitions of indices below each item's level), kScatterLanes lanes
+// per row in u64 words (rows start on 8-byte boundaries: stride is a
+// multiple of 4), eight rows at a time.
 constexpr uint32_t kScatterLanes = 8;
 __global__ void scatter_items(const uint64_t *hdr, const uint16_t *stack, uint32_t n_items, uint32_t stride,
                               unsigned int *offset, uint64_t *out_hdr, uint16_t *out_stack) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = t / kScatterLanes, sub = t % kScatterLanes;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
     const bool on = i < n_items;
     const uint64_t h = on ? hdr[i] : 0ull;
-    uint32_t o = 0;
-    if (on && sub == 0) {
-        o = atomicAdd(&offset[hdr_puzzle(h)], 1u);
-        out_hdr[o] = h;
+    const uint32_t p = on ? hdr_puzzle(h) : 0xFFFFFFFFu;
+    uint32_t first, len, base = 0;
+    run_of(p, lane, first, len);
+    if (first == lane && on) base = atomicAdd(&offset[p], len);
+    const uint32_t o = __shfl(base, (int)first) + (lane - first);
+    if (on) out_hdr[o] = h;
+    const uint32_t words = on ? min((hdr_level(h) + 3) / 4, stride / 4) : 0u;
+    const uint32_t wave_first = i - lane;
+#pragma unroll
+    for (uint32_t g = 0; g < 64 / kScatterLanes; ++g) {
+        const uint32_t r = g * kScatterLanes + lane / kScatterLanes, sub = lane % kScatterLanes;
+        const uint32_t orow = __shfl(o, (int)r), nw = __shfl(words, (int)r);
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(stack + (size_t)(wave_first + r) * stride);
+        uint64_t *dst = reinterpret_cast<uint64_t *>(out_stack + (size_t)orow * stride);
+        for (uint32_t w = sub; w < nw; w += kScatterLanes) dst[w] = src[w];
     }
-    o = __shfl(o, 0, kScatterLanes);
-    if (!on) return;
-    const uint32_t words = min((hdr_level(h) + 3) / 4, stride / 4);
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(stack + (size_t)i * stride);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(out_stack + (size_t)o * stride);
-    for (uint32_t j = sub; j < words; j += kScatterLanes) dst[j] = src[j];
 }
 
 template <int W>
@@ -1752,8 +1772,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
             hipLaunchKernelGGL(fill_u32, dim3((n + 255) / 256), dim3(256), 0, stream, d_group, n, 0u);
             hipLaunchKernelGGL(items_per_puzzle, dim3(g), dim3(256), 0, stream, d_hdr[cur], ni, d_group);
             hipLaunchKernelGGL(exclusive_scan_u32, dim3(1), dim3(1024), 0, stream, d_group, n);
-            hipLaunchKernelGGL(scatter_items, dim3((uint32_t)(((uint64_t)ni * kScatterLanes + 255) / 256)), dim3(256), 0,
-                               stream, d_hdr[cur], d_stack[cur], ni, stride,
+            hipLaunchKernelGGL(scatter_items, dim3(g), dim3(256), 0, stream, d_hdr[cur], d_stack[cur], ni, stride,
                                d_group, d_group_hdr, d_group_stack);
             EMP_CHECK(hipGetLastError());
             EMP_CHECK(hipMemcpyAsync(d_hdr[cur], d_group_hdr, (size_t)ni * 8, hipMemcpyDeviceToDevice, stream));
