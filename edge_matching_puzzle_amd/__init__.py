@@ -30,6 +30,11 @@ class Puzzle(ctypes.Structure):
                 ("colours", ctypes.c_void_p)]
 
 
+# numpy records laid out as emp_hip_puzzle and emp_hip_enum_result, for batches
+_PUZZLE_RECORD = np.dtype([("width", "<u4"), ("height", "<u4"), ("colours", "<u8")])
+assert _PUZZLE_RECORD.itemsize == ctypes.sizeof(Puzzle)
+
+
 class EnumOptions(ctypes.Structure):
     _fields_ = [("order", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
                 ("waves_per_block", ctypes.c_uint32), ("steps_per_round", ctypes.c_uint32),
@@ -45,6 +50,11 @@ class EnumResult(ctypes.Structure):
                 ("n_stored", ctypes.c_uint64), ("complete", ctypes.c_uint32),
                 ("status", ctypes.c_uint32), ("solve_ms", ctypes.c_double),
                 ("max_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+_ENUM_RESULT_RECORD = np.dtype([(f, {ctypes.c_uint64: "<u8", ctypes.c_uint32: "<u4", ctypes.c_double: "<f8"}[t])
+                                for f, t in EnumResult._fields_])
+assert _ENUM_RESULT_RECORD.itemsize == ctypes.sizeof(EnumResult)
 
 
 class EnumStats(ctypes.Structure):
@@ -651,32 +661,36 @@ class Context:
         (bounded launches, items split into the next one's queue) or "auto"
         (the first for one puzzle, the second for batches)."""
         n = len(puzzles)
-        keep = []
-        arr = (Puzzle * n)()
-        boards = []
-        board_ptrs = (ctypes.c_void_p * n)()
-        for i, (w, h, c) in enumerate(puzzles):
-            c = np.ascontiguousarray(c, dtype=np.uint32)
-            keep.append(c)
-            arr[i] = Puzzle(w, h, c.ctypes.data)
-            b = np.zeros((max_solutions, w * h), dtype=np.uint16)
-            boards.append(b)
-            board_ptrs[i] = b.ctypes.data if max_solutions else None
+        # The batch as flat arrays (one colour buffer, one board store) with
+        # the C structs laid over numpy records: no per-puzzle ctypes objects.
+        cols = [np.ascontiguousarray(c, dtype=np.uint32).ravel() for _, _, c in puzzles]
+        sizes = np.array([w * h for w, h, _ in puzzles], dtype=np.int64)
+        if (np.array([len(c) for c in cols], dtype=np.int64) != 4 * sizes).any():
+            raise ValueError("each puzzle needs 4 colours per piece")
+        flat = np.concatenate(cols) if n else np.zeros(1, dtype=np.uint32)
+        arr = np.zeros(n, dtype=_PUZZLE_RECORD)
+        arr["width"] = [w for w, _, _ in puzzles]
+        arr["height"] = [h for _, h, _ in puzzles]
+        arr["colours"] = flat.ctypes.data + 16 * (np.cumsum(sizes) - sizes)
+        board_off = max_solutions * (np.cumsum(sizes) - sizes)
+        store = np.zeros(max(1, int(max_solutions * sizes.sum())), dtype=np.uint16)
+        board_ptrs = (store.ctypes.data + 2 * board_off).astype(np.uint64) if max_solutions else \
+            np.zeros(n, dtype=np.uint64)
         opt = self._options(order, max_solutions, waves_per_cu, waves_per_block, steps_per_round, step_budget,
                             target_items, node_limit, queue_items, schedule)
         opt.time_limit_ms = time_limit_ms
         opt.use_graph = 1 if use_graph else 0
         opt.count_work = 1 if count_work else 0
-        res = (EnumResult * n)()
+        res = np.zeros(n, dtype=_ENUM_RESULT_RECORD)
         st = EnumStats()
         if noalloc:
             if ranks is not None:
                 raise ValueError("the allocation-free enumeration takes no rank tables")
-            _check(lib().emp_hip_enumerate_noalloc(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
-                                                   ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+            _check(lib().emp_hip_enumerate_noalloc(self._ctx, arr.ctypes.data, n, ctypes.byref(opt),
+                                                   res.ctypes.data, board_ptrs.ctypes.data, ctypes.byref(st)))
         elif ranks is None:
-            _check(lib().emp_hip_enumerate(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt),
-                                           ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+            _check(lib().emp_hip_enumerate(self._ctx, arr.ctypes.data, n, ctypes.byref(opt),
+                                           res.ctypes.data, board_ptrs.ctypes.data, ctypes.byref(st)))
         else:
             rk = [np.ascontiguousarray(r, dtype=np.uint16) for r in ranks]
             if len(rk) != n:
@@ -686,13 +700,15 @@ class Context:
                 if r.shape != (w * h, T):
                     raise ValueError("rank table %d has shape %s, expected (%d, %d)" % (i, r.shape, w * h, T))
             rank_ptrs = (ctypes.c_void_p * n)(*[r.ctypes.data for r in rk])
-            _check(lib().emp_hip_enumerate_ranked(self._ctx, ctypes.byref(arr), n, ctypes.byref(opt), rank_ptrs,
-                                                  ctypes.byref(res), board_ptrs, ctypes.byref(st)))
+            _check(lib().emp_hip_enumerate_ranked(self._ctx, arr.ctypes.data, n, ctypes.byref(opt), rank_ptrs,
+                                                  res.ctypes.data, board_ptrs.ctypes.data, ctypes.byref(st)))
+        stored = np.minimum(res["n_stored"], max_solutions).astype(np.int64)
         out = []
-        for i in range(n):
-            r = res[i]
-            out.append({"n_nodes": r.n_nodes, "n_solutions": r.n_solutions,
-                        "complete": bool(r.complete), "solve_ms": r.solve_ms, "max_depth": r.max_depth,
-                        "boards": boards[i][:r.n_stored].copy()})
+        for i, (nodes, sols, done, ms, depth, k, P, o) in enumerate(zip(
+                res["n_nodes"].tolist(), res["n_solutions"].tolist(), res["complete"].tolist(),
+                res["solve_ms"].tolist(), res["max_depth"].tolist(), stored.tolist(), sizes.tolist(),
+                board_off.tolist())):
+            out.append({"n_nodes": nodes, "n_solutions": sols, "complete": bool(done), "solve_ms": ms,
+                        "max_depth": depth, "boards": store[o:o + k * P].reshape(k, P)})
         stats = {f: getattr(st, f) for f, _ in EnumStats._fields_}
         return out, stats

@@ -19,6 +19,7 @@
 #include "emp_glutton.h"
 #include "emp_glutton_stack.h"
 #include "emp_glutton_wide.h"
+#include "emp_host_threads.h"
 #include "emp_mlp.h"
 #include "emp_model.h"
 #include "emp_profile.h"
@@ -281,15 +282,21 @@ int run_enumeration(emp_hip_context *context, const emp_hip_puzzle *puzzles, uin
         return fail(EMP_HIP_BAD_ARGUMENT, "unknown order");
     if (opt.waves_per_block > 16) return fail(EMP_HIP_BAD_ARGUMENT, "waves_per_block must be at most 16");
     if (hipSetDevice(context->dev.device) != hipSuccess) return fail(EMP_HIP_DEVICE_ERROR, "hipSetDevice failed");
-    std::vector<std::unique_ptr<emp::Puzzle>> owned;
+    // Checked on the host threads; the first puzzle (in batch order) that
+    // fails is reported.
+    std::vector<std::unique_ptr<emp::Puzzle>> owned(n_puzzles);
+    std::vector<std::string> bad(n_puzzles);
+    emp::for_each_item(n_puzzles, [&](uint32_t i) {
+        try {
+            owned[i].reset(new emp::Puzzle(puzzles[i].width, puzzles[i].height, puzzles[i].colours));
+        } catch (const std::invalid_argument &e) {
+            bad[i] = e.what();
+        }
+    });
     std::vector<const emp::Puzzle *> list;
     for (uint32_t i = 0; i < n_puzzles; ++i) {
-        try {
-            owned.emplace_back(new emp::Puzzle(puzzles[i].width, puzzles[i].height, puzzles[i].colours));
-        } catch (const std::invalid_argument &e) {
-            return fail(EMP_HIP_BAD_PUZZLE, "puzzle " + std::to_string(i) + ": " + e.what());
-        }
-        list.push_back(owned.back().get());
+        if (!owned[i]) return fail(EMP_HIP_BAD_PUZZLE, "puzzle " + std::to_string(i) + ": " + bad[i]);
+        list.push_back(owned[i].get());
     }
     std::vector<std::vector<uint16_t>> rank_tables;
     if (ranks) {

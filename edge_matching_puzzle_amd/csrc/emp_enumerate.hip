@@ -32,14 +32,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
-#include <exception>
+#include <memory>
 #include <string>
-#include <thread>
 #include <type_traits>
 #include <vector>
 
 #include "emp_enum_plan.h"
 #include "emp_enumerate.h"
+#include "emp_host_threads.h"
 
 namespace emp {
 
@@ -163,9 +163,6 @@ constexpr uint32_t kQueueRounds = 8;
 // Input regions of a batched pass: one per XCD (MI355X has eight, and the
 // dispatcher deals workgroups to them round-robin).
 constexpr uint32_t kRegions = 8;
-// Host threads that build the tables of a large batch.
-constexpr uint32_t kTableThreads = 16;
-
 // Items a wave of a batched pass takes from its region at a time.
 #ifndef EMP_RUN_ITEMS
 #define EMP_RUN_ITEMS 64
@@ -1366,26 +1363,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // Large batches build their tables on several host threads; an error is
     // reported for the first puzzle (in batch order) that has one.
     std::vector<EnumTable> tables(n_tab);
-    {
-        std::vector<std::exception_ptr> errors(n_tab);
-        auto build = [&](uint32_t first, uint32_t step) {
-            for (uint32_t i = first; i < n_tab; i += step) {
-                try {
-                    tables[i] = build_enum_table(*puzzles[i], cfg.order, row);
-                } catch (...) {
-                    errors[i] = std::current_exception();
-                }
-            }
-        };
-        const uint32_t n_threads =
-            n_tab < 64 ? 1u : std::max(1u, std::min<uint32_t>(kTableThreads, std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool;
-        for (uint32_t t = 1; t < n_threads; ++t) pool.emplace_back(build, t, n_threads);
-        build(0, n_threads);
-        for (std::thread &th : pool) th.join();
-        for (const std::exception_ptr &e : errors)
-            if (e) std::rethrow_exception(e);
-    }
+    for_each_item(n_tab, [&](uint32_t i) { tables[i] = build_enum_table(*puzzles[i], cfg.order, row); });
     std::vector<uint32_t> table_off(n_tab);
     // max_reach: one past the last word a read of W words at a kind's offset
     // touches (the interior kind's words come last in the region)
@@ -1403,9 +1381,12 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     const uint32_t stride = (max_P + 3) & ~3u;   // u16 per item, multiple of 4
     stats.max_words = natural;
 
-    std::vector<uint64_t> blob(total_words);
-    for (uint32_t i = 0; i < n_tab; ++i)
-        std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.begin() + table_off[i]);
+    // The tables side by side, copied on the host threads into memory first
+    // touched there (every word is written: the offsets tile the blob).
+    std::unique_ptr<uint64_t[]> blob(new uint64_t[total_words]);
+    for_each_item(n_tab, [&](uint32_t i) {
+        std::copy(tables[i].blob.begin(), tables[i].blob.end(), blob.get() + table_off[i]);
+    });
     const uint32_t n_init = has_items ? (uint32_t)work->items.size() : n;
     if (has_items) {
         const uint32_t P = tables[0].header().P;
@@ -1483,7 +1464,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     // Device buffers.
     uint64_t *d_tables = ws.get<uint64_t>("tables", total_words);
     uint32_t *d_table_off = ws.get<uint32_t>("table_off", n);
-    EMP_CHECK(hipMemcpyAsync(d_tables, blob.data(), blob.size() * 8, hipMemcpyHostToDevice, stream));
+    EMP_CHECK(hipMemcpyAsync(d_tables, blob.get(), (size_t)total_words * 8, hipMemcpyHostToDevice, stream));
     EMP_CHECK(hipMemcpyAsync(d_table_off, root_off.data(), n * 4, hipMemcpyHostToDevice, stream));
     uint16_t *d_ranks = nullptr;
     uint64_t *d_rank_off = nullptr;
