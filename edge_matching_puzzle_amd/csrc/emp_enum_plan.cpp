@@ -65,14 +65,24 @@ EnumTable build_enum_table(const Puzzle &pz, int order, uint32_t row_words, uint
                 valid[4 * ki + o] = Puzzle::orientation_kept(p, o);
             }
         }
-        std::vector<char> seen(4 * n_kind[k], 0);
-        for (uint32_t a = 0; a < 4 * n_kind[k]; ++a) {
-            if (!valid[a] || seen[a]) continue;
+        // Valid transitions sorted by (look, index): each run of one look is a
+        // class, its first entry the lowest transition; classes are listed
+        // by their lowest transition.
+        std::vector<std::pair<uint32_t, uint32_t>> by_look;
+        by_look.reserve(4 * n_kind[k]);
+        for (uint32_t a = 0; a < 4 * n_kind[k]; ++a)
+            if (valid[a]) by_look.emplace_back(look[a], a);
+        std::sort(by_look.begin(), by_look.end());
+        std::vector<std::pair<uint32_t, size_t>> first_of;   // (lowest transition, run start)
+        for (size_t i = 0; i < by_look.size(); ++i)
+            if (i == 0 || by_look[i].first != by_look[i - 1].first) first_of.emplace_back(by_look[i].second, i);
+        std::sort(first_of.begin(), first_of.end());
+        for (const auto &run : first_of) {
+            const uint32_t a = run.first;
             std::vector<uint64_t> mask(R, 0);
             bool several_pieces = false;
-            for (uint32_t b = a; b < 4 * n_kind[k]; ++b) {
-                if (!valid[b] || look[b] != look[a]) continue;
-                seen[b] = 1;
+            for (size_t i = run.second; i < by_look.size() && by_look[i].first == look[a]; ++i) {
+                const uint32_t b = by_look[i].second;
                 mask[b >> 6] |= 1ull << (b & 63);
                 several_pieces |= (b >> 2) != (a >> 2);
             }
