@@ -165,6 +165,28 @@ def test_tall_board_spiral_is_refused(gpu_context):
     assert e.value.status == emp.UNSUPPORTED
 
 
+def test_batch_reports_its_first_bad_puzzle(gpu_context):
+    """A batch large enough to be checked on several host threads, with two
+    puzzles the reference would refuse: the error names the first of them in
+    batch order, whichever thread found it.  A colour array of the wrong
+    length is refused before the library is called."""
+    good = [(4, 4, np.asarray(emp.generate(10 + i, 4, 4, 2, 3)[0], dtype=np.uint32).reshape(-1, 4)) for i in range(100)]
+    batch = list(good)
+    three_frame_sides = good[70][2].copy()
+    three_frame_sides[5] = [0, 0, 0, 1]
+    batch[70] = (4, 4, three_frame_sides)
+    batch[90] = (4, 4, np.zeros_like(good[90][2]))
+    print("case: batch with bad puzzles 70 and 90", flush=True)
+    with pytest.raises(emp.EmpError) as e:
+        gpu_context.enumerate(batch)
+    assert e.value.status == emp.BAD_PUZZLE and "puzzle 70:" in str(e.value)
+    with pytest.raises(ValueError):
+        gpu_context.enumerate([(4, 4, good[0][2][:-1])])
+    # the same context still enumerates after the refusals
+    (r,), _ = gpu_context.enumerate([good[0]], max_solutions=1 << 12)
+    _check_same(r, o.enumerate_(4, 4, good[0][2].ravel(), max_solutions=1 << 12))
+
+
 def test_solution_capacity_and_counts(gpu_context):
     w, h = 4, 4
     c, _ = emp.generate(3, w, h, 1, 2)
