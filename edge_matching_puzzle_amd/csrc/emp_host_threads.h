@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <exception>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -31,9 +32,17 @@ void for_each_item(uint32_t n, Fn fn, uint32_t serial_below = 64) {
     };
     const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
     const uint32_t n_threads = n < serial_below ? 1u : std::min(kHostThreads, hw);
+    // Shares whose thread could not be started (a process thread limit) are
+    // run by the calling thread.
     std::vector<std::thread> pool;
-    for (uint32_t t = 1; t < n_threads; ++t) pool.emplace_back(run, t, n_threads);
+    pool.reserve(n_threads);
+    uint32_t started = 1;
+    try {
+        for (; started < n_threads; ++started) pool.emplace_back(run, started, n_threads);
+    } catch (const std::system_error &) {
+    }
     run(0, n_threads);
+    for (uint32_t t = started; t < n_threads; ++t) run(t, n_threads);
     for (std::thread &th : pool) th.join();
     for (const std::exception_ptr &e : errors)
         if (e) std::rethrow_exception(e);
