@@ -169,6 +169,13 @@ constexpr uint32_t kRegions = 8;
 #endif
 constexpr uint32_t kRunItems = EMP_RUN_ITEMS;
 
+// Steps a work item runs before it splits once every input item of a pass is
+// taken (and, in a persistent launch, between hand-overs while lanes wait).
+#ifndef EMP_DRAIN_BUDGET
+#define EMP_DRAIN_BUDGET 512
+#endif
+constexpr uint64_t kDrainBudget = EMP_DRAIN_BUDGET;
+
 // Plain order: transitions above `last` in index order.  Ranked order: the
 // transitions whose rank at this index is above the rank of `last`.
 template <int W>
@@ -1626,7 +1633,7 @@ void enumerate_on_device(const DeviceInfo &dev, hipStream_t stream, Workspace &w
     a.stride = stride;
     a.steps_per_round = cfg.steps_per_round ? cfg.steps_per_round : 128;
     a.step_budget = cfg.step_budget ? cfg.step_budget : (1ull << 16);
-    a.drain_budget = std::min<uint64_t>(a.step_budget, cfg.drain_budget ? cfg.drain_budget : 512);
+    a.drain_budget = std::min<uint64_t>(a.step_budget, cfg.drain_budget ? cfg.drain_budget : kDrainBudget);
     a.nodes = d_nodes;
     a.solutions = d_sols;
     a.first_tick = d_first;
